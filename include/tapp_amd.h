@@ -1,0 +1,75 @@
+/*
+ * MI355X extensions to the TAPP C API.  Everything here is additive: a
+ * program written against the reference headers (api/include/tapp/ and tapp.h)
+ * compiles and runs unchanged against this library.
+ */
+#ifndef TAPP_AMD_H_
+#define TAPP_AMD_H_
+
+#include <stdint.h>
+
+#include "tapp.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Error codes beyond the reference's (which are 0-15, 33 and 137,
+ * error.c:15-76 and product.c:197-222, 463). */
+enum {
+    TAPP_AMD_ERR_UNSUPPORTED = 101, /* datatype / precision combination not supported */
+    TAPP_AMD_ERR_DEVICE = 102,      /* HIP runtime error */
+    TAPP_AMD_ERR_NOMEM = 103,       /* device or host memory exhausted */
+    TAPP_AMD_ERR_INVALID = 104,     /* invalid handle, NULL argument, label count mismatch */
+    TAPP_AMD_ERR_KEY = 105,         /* unknown attribute key */
+    TAPP_AMD_ERR_BUDGET = 106,      /* request larger than the serving memory budget */
+    TAPP_AMD_ERR_BUSY = 107         /* serving queue full */
+};
+
+/* Attribute keys for TAPP_attr_set / TAPP_attr_get / TAPP_attr_clear on an
+ * executor (pass the TAPP_executor as the TAPP_attr).  Values are passed by
+ * pointer to an int unless stated. */
+enum {
+    /* FP32 products with TAPP_DEFAULT_PREC or TAPP_F32F32_ACCUM_F32 run as
+     * split-BF16 (x = hi + lo, three BF16 MFMA products, FP32 accumulate).
+     * Opt-in; default 0. */
+    TAPP_AMD_ATTR_SPLIT_BF16 = 0x1001,
+    /* Force a kernel family: 0 auto, 1 one-lane-per-output FMA, 2 MFMA,
+     * 3 split-K.  Default 0. */
+    TAPP_AMD_ATTR_KERNEL = 0x1002,
+    /* Read-only: kernel family used by the last execute (int). */
+    TAPP_AMD_ATTR_LAST_KERNEL = 0x1003,
+    /* Read-only: device time in microseconds of the last execute's kernels,
+     * measured with HIP events on the executor's stream (double). */
+    TAPP_AMD_ATTR_LAST_DEVICE_US = 0x1004,
+    /* The HIP stream (hipStream_t) the executor launches on; set to bind an
+     * external stream, get to read it.  Value: pointer to a void*. */
+    TAPP_AMD_ATTR_STREAM = 0x1005,
+    /* 1: time every execute with HIP events (needed for LAST_DEVICE_US). */
+    TAPP_AMD_ATTR_TIMING = 0x1006
+};
+
+/* Kernel family identifiers reported by TAPP_AMD_ATTR_LAST_KERNEL. */
+enum {
+    TAPP_AMD_KERNEL_NONE = 0,
+    TAPP_AMD_KERNEL_SIMPLE = 1,
+    TAPP_AMD_KERNEL_MFMA = 2,
+    TAPP_AMD_KERNEL_SPLITK = 3
+};
+
+/* Version string of this library. */
+TAPP_EXPORT const char* TAPP_AMD_version(void);
+
+/* Plan-stage introspection for tests: after validation, classification and
+ * stride folding, reports group sizes (L Hadamard, M, N, K contracted, UA
+ * and UB unary) and the chosen compute type (0 F32, 1 F64, 2 C32, 3 C64).
+ * Returns the validation code TAPP_execute_product would return. */
+TAPP_EXPORT TAPP_error TAPP_AMD_plan_info(TAPP_tensor_product plan, TAPP_executor exec,
+                                          const void* beta, const void* C,
+                                          int64_t sizes[6], int* compute_type);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TAPP_AMD_H_ */

@@ -1,0 +1,395 @@
+// Strided GETT kernels for gfx950 (MI355X, CDNA4, wave64).
+//
+// D[l, m, n] = op_D(alpha * sum_k A'[l, m, k] B'[l, k, n] + beta * op_C(C[l, m, n]))
+// with every operand addressed through per-group int64 offset tables (tk.h).
+//
+//  k_simple   one lane per output element, FMA over K: thin and
+//             Hadamard-only products where an MFMA tile would be mostly empty.
+//  k_splitk   one workgroup per output element, K split over 256 lanes and
+//             reduced through LDS: very few outputs with a long K.
+//  k_mfma     LDS-tiled MFMA GETT, 256 lanes = 2x2 waves, double-buffered
+//             LDS, register prefetch of the next K tile, XCD-aware tile order.
+//             FP32: v_mfma_f32_32x32x2f32; FP64: v_mfma_f64_16x16x4f64;
+//             complex: four real MFMAs per step on split re/im LDS planes.
+//             The MFMA's first operand is the B tile and its second the A
+//             tile, so output lanes run along M, the unit-stride direction of
+//             D when the host has oriented the problem that way.
+#include "tk_device.hpp"
+
+namespace tk {
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+// ---------------------------------------------------------------------------
+// scalar multiply-accumulate with the input modes
+// ---------------------------------------------------------------------------
+template <typename R, bool CPLX>
+__device__ __forceinline__ void mac(int mode, Val<R, CPLX> a, Val<R, CPLX> b, R& sr, R& si)
+{
+    if (mode == TK_MODE_SPLIT_BF16) {
+        // x = hi + lo with hi = bf16(x), lo = bf16(x - hi); three products
+        auto split = [](R x, R& hi, R& lo) { hi = (R)round_bf16((float)x); lo = (R)round_bf16((float)(x - hi)); };
+        R ah, al, bh, bl;
+        split(a.re, ah, al); split(b.re, bh, bl);
+        R rr = ah * bh + (ah * bl + al * bh);
+        if (!CPLX) { sr += rr; return; }
+        R aih, ail, bih, bil;
+        split(a.im, aih, ail); split(b.im, bih, bil);
+        R ii = aih * bih + (aih * bil + ail * bih);
+        R ri = ah * bih + (ah * bil + al * bih);
+        R ir = aih * bh + (aih * bl + ail * bh);
+        sr += rr - ii; si += ri + ir;
+        return;
+    }
+    if (mode == TK_MODE_BF16 || mode == TK_MODE_F16) {
+        a.re = round_mode(a.re, mode); b.re = round_mode(b.re, mode);
+        if (CPLX) { a.im = round_mode(a.im, mode); b.im = round_mode(b.im, mode); }
+    }
+    if (CPLX) {
+        sr += a.re * b.re - a.im * b.im;
+        si += a.re * b.im + a.im * b.re;
+    } else {
+        sr += a.re * b.re;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_simple
+// ---------------------------------------------------------------------------
+template <typename R, bool CPLX>
+__global__ __launch_bounds__(256) void k_simple(tk_gett g)
+{
+    const int64_t total = g.M * g.N * g.L;
+    for (int64_t o = (int64_t)blockIdx.x * 256 + threadIdx.x; o < total; o += (int64_t)gridDim.x * 256) {
+        const int64_t m = o % g.M, t = o / g.M, n = t % g.N, l = t / g.N;
+        const int64_t ao = g.lA[l] + g.mA[m], bo = g.lB[l] + g.nB[n];
+        R sr = 0, si = 0;
+        for (int64_t k = 0; k < g.K; ++k) {
+            Val<R, CPLX> a = Elem<R, CPLX>::ld(g.A, ao + g.kA[k]);
+            Val<R, CPLX> b = Elem<R, CPLX>::ld(g.B, bo + g.kB[k]);
+            if (CPLX && g.conjA) a.im = -a.im;
+            if (CPLX && g.conjB) b.im = -b.im;
+            mac<R, CPLX>(g.mode, a, b, sr, si);
+        }
+        epilogue<R, CPLX>(g, l, m, n, sr, si);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_splitk: grid.x = number of outputs
+// ---------------------------------------------------------------------------
+template <typename R, bool CPLX>
+__global__ __launch_bounds__(256) void k_splitk(tk_gett g)
+{
+    __shared__ R red[2][256];
+    const int64_t o = blockIdx.x;
+    const int64_t m = o % g.M, t = o / g.M, n = t % g.N, l = t / g.N;
+    const int64_t ao = g.lA[l] + g.mA[m], bo = g.lB[l] + g.nB[n];
+    R sr = 0, si = 0;
+    for (int64_t k = threadIdx.x; k < g.K; k += 256) {
+        Val<R, CPLX> a = Elem<R, CPLX>::ld(g.A, ao + g.kA[k]);
+        Val<R, CPLX> b = Elem<R, CPLX>::ld(g.B, bo + g.kB[k]);
+        if (CPLX && g.conjA) a.im = -a.im;
+        if (CPLX && g.conjB) b.im = -b.im;
+        mac<R, CPLX>(g.mode, a, b, sr, si);
+    }
+    red[0][threadIdx.x] = sr;
+    red[1][threadIdx.x] = si;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            red[0][threadIdx.x] += red[0][threadIdx.x + s];
+            red[1][threadIdx.x] += red[1][threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) epilogue<R, CPLX>(g, l, m, n, red[0][0], red[1][0]);
+}
+
+// ---------------------------------------------------------------------------
+// MFMA primitives
+// ---------------------------------------------------------------------------
+template <typename R, int S> struct Mfma;
+template <> struct Mfma<float, 32> {
+    typedef f32x16 acc_t;
+    static constexpr int NR = 16;
+    static __device__ __forceinline__ acc_t run(float a, float b, acc_t c)
+    {
+        return __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, c, 0, 0, 0);
+    }
+    // row of accumulator register r in lane `lane` (32x32 output block)
+    static __device__ __forceinline__ int row(int r, int lane) { return 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3); }
+};
+template <> struct Mfma<float, 16> {
+    typedef f32x4 acc_t;
+    static constexpr int NR = 4;
+    static __device__ __forceinline__ acc_t run(float a, float b, acc_t c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int r, int lane) { return 4 * (lane >> 4) + r; }
+};
+template <> struct Mfma<double, 16> {
+    typedef f64x4 acc_t;
+    static constexpr int NR = 4;
+    static __device__ __forceinline__ acc_t run(double a, double b, acc_t c)
+    {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int r, int lane) { return 4 * (lane >> 4) + r; }
+};
+
+// ---------------------------------------------------------------------------
+// tile loader: BR rows (M for A, N for B) x BK contracted, into registers
+// ---------------------------------------------------------------------------
+template <typename R, bool CPLX, int BR, int BK>
+struct TileLoader {
+    static constexpr int EW = CPLX ? 2 : 1;                 // R words per element
+    static constexpr int VE = 16 / (EW * (int)sizeof(R));   // elements per 16-byte vector
+    static constexpr int NS = (BR * BK) / (256 * VE);        // vector slots per lane
+    static_assert(NS * 256 * VE == BR * BK, "tile must split evenly over 256 lanes");
+
+    R v[NS][VE * EW];
+
+    // mode: TK_GATHER / TK_CONTIG_ROW / TK_CONTIG_K; vec: 16-byte loads allowed
+    __device__ __forceinline__ void load(const R* base, const int64_t* rt, const int64_t* kt, int64_t r0,
+                                         int64_t k0, int64_t Rn, int64_t Kn, int mode, int vec)
+    {
+#pragma unroll
+        for (int i = 0; i < NS; ++i) {
+            const int s = threadIdx.x + 256 * i;
+            if (mode == TK_CONTIG_K) {
+                const int kk = (s % (BK / VE)) * VE, rr = s / (BK / VE);
+                const int64_t r = r0 + rr, k = k0 + kk;
+                if (r < Rn && k + VE <= Kn && vec) {
+                    const R* p = base + EW * (rt[r] + kt[k]);
+                    load16(p, v[i]);
+                } else {
+#pragma unroll
+                    for (int j = 0; j < VE; ++j) {
+                        const bool in = r < Rn && k + j < Kn;
+                        const int64_t off = in ? EW * (rt[r] + kt[k + j]) : 0;
+#pragma unroll
+                        for (int w = 0; w < EW; ++w) v[i][j * EW + w] = in ? base[off + w] : (R)0;
+                    }
+                }
+            } else {
+                const int rr = (s % (BR / VE)) * VE, kk = s / (BR / VE);
+                const int64_t r = r0 + rr, k = k0 + kk;
+                if (mode == TK_CONTIG_ROW && vec && k < Kn && r + VE <= Rn) {
+                    const R* p = base + EW * (rt[r] + kt[k]);
+                    load16(p, v[i]);
+                } else {
+#pragma unroll
+                    for (int j = 0; j < VE; ++j) {
+                        const bool in = r + j < Rn && k < Kn;
+                        const int64_t off = in ? EW * (rt[r + j] + kt[k]) : 0;
+#pragma unroll
+                        for (int w = 0; w < EW; ++w) v[i][j * EW + w] = in ? base[off + w] : (R)0;
+                    }
+                }
+            }
+        }
+    }
+
+    static __device__ __forceinline__ void load16(const R* p, R* out)
+    {
+        const float4 q = *reinterpret_cast<const float4*>(p);
+        R tmp[16 / sizeof(R)];
+        __builtin_memcpy(tmp, &q, 16);
+#pragma unroll
+        for (int w = 0; w < (int)(16 / sizeof(R)); ++w) out[w] = tmp[w];
+    }
+
+    // LDS image: planes [EW][BK][BR + PAD], row index fastest
+    template <int LD>
+    __device__ __forceinline__ void store(R* lds, int mode, int conj)
+    {
+#pragma unroll
+        for (int i = 0; i < NS; ++i) {
+            const int s = threadIdx.x + 256 * i;
+            int rr, kk;
+            if (mode == TK_CONTIG_K) { kk = (s % (BK / VE)) * VE; rr = s / (BK / VE); }
+            else { rr = (s % (BR / VE)) * VE; kk = s / (BR / VE); }
+#pragma unroll
+            for (int j = 0; j < VE; ++j) {
+                const int r = mode == TK_CONTIG_K ? rr : rr + j;
+                const int k = mode == TK_CONTIG_K ? kk + j : kk;
+                lds[k * LD + r] = v[i][j * EW];
+                if (CPLX) {
+                    const R im = v[i][j * EW + 1];
+                    lds[BK * LD + k * LD + r] = conj ? -im : im;
+                }
+            }
+        }
+    }
+};
+
+// ---------------------------------------------------------------------------
+// k_mfma
+// ---------------------------------------------------------------------------
+template <typename R, bool CPLX, int S, int FM, int FN, int BK>
+__global__ __launch_bounds__(256) void k_mfma(tk_gett g)
+{
+    typedef Mfma<R, S> MF;
+    typedef typename MF::acc_t acc_t;
+    constexpr int BM = 2 * FM * S, BN = 2 * FN * S;
+    constexpr int P = CPLX ? 2 : 1;
+    constexpr int PAD = 4;
+    constexpr int LDA = BM + PAD, LDB = BN + PAD;
+    constexpr int KSTEP = 64 / S;
+    __shared__ R As[2][P * BK * LDA];
+    __shared__ R Bs[2][P * BK * LDB];
+
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    const int il = lane % S, kl = lane / S;
+
+    const int64_t ntm = (g.M + BM - 1) / BM, ntn = (g.N + BN - 1) / BN;
+    const int64_t tile = xcd_tile(blockIdx.x, ntm * ntn);
+    const int64_t m0 = (tile / ntn) * BM, n0 = (tile % ntn) * BN;
+
+    for (int64_t l = blockIdx.y; l < g.L; l += gridDim.y) {
+        const R* Ab = (const R*)g.A + (CPLX ? 2 : 1) * g.lA[l];
+        const R* Bb = (const R*)g.B + (CPLX ? 2 : 1) * g.lB[l];
+
+        acc_t acc[FN][FM], acci[FN][FM];
+#pragma unroll
+        for (int fn = 0; fn < FN; ++fn)
+#pragma unroll
+            for (int fm = 0; fm < FM; ++fm) {
+                acc[fn][fm] = acc_t{};
+                acci[fn][fm] = acc_t{};
+            }
+
+        TileLoader<R, CPLX, BM, BK> la;
+        TileLoader<R, CPLX, BN, BK> lb;
+        const int64_t nk = (g.K + BK - 1) / BK;
+        if (nk > 0) {
+            la.load(Ab, g.mA, g.kA, m0, 0, g.M, g.K, g.a_load, g.a_vec);
+            lb.load(Bb, g.nB, g.kB, n0, 0, g.N, g.K, g.b_load, g.b_vec);
+            la.template store<LDA>(As[0], g.a_load, g.conjA);
+            lb.template store<LDB>(Bs[0], g.b_load, g.conjB);
+        }
+        __syncthreads();
+        for (int64_t kt = 0; kt < nk; ++kt) {
+            const int cur = kt & 1;
+            if (kt + 1 < nk) {
+                la.load(Ab, g.mA, g.kA, m0, (kt + 1) * BK, g.M, g.K, g.a_load, g.a_vec);
+                lb.load(Bb, g.nB, g.kB, n0, (kt + 1) * BK, g.N, g.K, g.b_load, g.b_vec);
+            }
+            const R* as = As[cur];
+            const R* bs = Bs[cur];
+#pragma unroll
+            for (int kk = 0; kk < BK; kk += KSTEP) {
+                R a[FM], b[FN], ai[FM], bi[FN];
+#pragma unroll
+                for (int fm = 0; fm < FM; ++fm) {
+                    a[fm] = as[(kk + kl) * LDA + wm * FM * S + fm * S + il];
+                    if (CPLX) ai[fm] = as[BK * LDA + (kk + kl) * LDA + wm * FM * S + fm * S + il];
+                }
+#pragma unroll
+                for (int fn = 0; fn < FN; ++fn) {
+                    b[fn] = bs[(kk + kl) * LDB + wn * FN * S + fn * S + il];
+                    if (CPLX) bi[fn] = bs[BK * LDB + (kk + kl) * LDB + wn * FN * S + fn * S + il];
+                }
+#pragma unroll
+                for (int fn = 0; fn < FN; ++fn)
+#pragma unroll
+                    for (int fm = 0; fm < FM; ++fm) {
+                        acc[fn][fm] = MF::run(b[fn], a[fm], acc[fn][fm]);
+                        if (CPLX) {
+                            acc[fn][fm] = MF::run(-bi[fn], ai[fm], acc[fn][fm]);
+                            acci[fn][fm] = MF::run(bi[fn], a[fm], acci[fn][fm]);
+                            acci[fn][fm] = MF::run(b[fn], ai[fm], acci[fn][fm]);
+                        }
+                    }
+            }
+            if (kt + 1 < nk) {
+                la.template store<LDA>(As[cur ^ 1], g.a_load, g.conjA);
+                lb.template store<LDB>(Bs[cur ^ 1], g.b_load, g.conjB);
+            }
+            __syncthreads();
+        }
+
+#pragma unroll
+        for (int fn = 0; fn < FN; ++fn)
+#pragma unroll
+            for (int fm = 0; fm < FM; ++fm) {
+                const int64_t m = m0 + wm * FM * S + fm * S + il;
+#pragma unroll
+                for (int r = 0; r < MF::NR; ++r) {
+                    const int64_t n = n0 + wn * FN * S + fn * S + MF::row(r, lane);
+                    if (m < g.M && n < g.N)
+                        epilogue<R, CPLX>(g, l, m, n, acc[fn][fm][r], CPLX ? acci[fn][fm][r] : (R)0);
+                }
+            }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// dispatch
+// ---------------------------------------------------------------------------
+template <typename R, bool CPLX>
+static hipError_t launch_simple(const tk_gett& g, hipStream_t st)
+{
+    const int64_t total = g.M * g.N * g.L;
+    int64_t blocks = (total + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL((k_simple<R, CPLX>), dim3((unsigned)blocks), dim3(256), 0, st, g);
+    return hipGetLastError();
+}
+
+template <typename R, bool CPLX>
+static hipError_t launch_splitk(const tk_gett& g, hipStream_t st)
+{
+    const int64_t total = g.M * g.N * g.L;
+    hipLaunchKernelGGL((k_splitk<R, CPLX>), dim3((unsigned)total), dim3(256), 0, st, g);
+    return hipGetLastError();
+}
+
+template <typename R, bool CPLX, int S, int FM, int FN, int BK>
+static hipError_t launch_mfma(const tk_gett& g, hipStream_t st)
+{
+    constexpr int BM = 2 * FM * S, BN = 2 * FN * S;
+    const int64_t tiles = ((g.M + BM - 1) / BM) * ((g.N + BN - 1) / BN);
+    const int64_t ly = g.L < 65535 ? g.L : 65535;
+    hipLaunchKernelGGL((k_mfma<R, CPLX, S, FM, FN, BK>), dim3((unsigned)tiles, (unsigned)ly), dim3(256), 0, st, g);
+    return hipGetLastError();
+}
+
+}  // namespace tk
+
+using namespace tk;
+
+extern "C" int tk_launch_gett(const tk_gett* gp, void* stream)
+{
+    const tk_gett& g = *gp;
+    hipStream_t st = (hipStream_t)stream;
+    if (g.M <= 0 || g.N <= 0 || g.L <= 0) return 0;
+    const int kern = g.kernel;
+    switch (g.ctype) {
+    case TK_F32:
+        if (kern == TK_KERNEL_SIMPLE || g.mode != TK_MODE_PLAIN) return launch_simple<float, false>(g, st);
+        if (kern == TK_KERNEL_SPLITK) return launch_splitk<float, false>(g, st);
+        return launch_mfma<float, false, 32, 2, 2, 16>(g, st);
+    case TK_F64:
+        if (kern == TK_KERNEL_SIMPLE) return launch_simple<double, false>(g, st);
+        if (kern == TK_KERNEL_SPLITK) return launch_splitk<double, false>(g, st);
+        return launch_mfma<double, false, 16, 4, 4, 8>(g, st);
+    case TK_C32:
+        if (kern == TK_KERNEL_SIMPLE || g.mode != TK_MODE_PLAIN) return launch_simple<float, true>(g, st);
+        if (kern == TK_KERNEL_SPLITK) return launch_splitk<float, true>(g, st);
+        return launch_mfma<float, true, 32, 2, 2, 8>(g, st);
+    case TK_C64:
+        if (kern == TK_KERNEL_SIMPLE) return launch_simple<double, true>(g, st);
+        if (kern == TK_KERNEL_SPLITK) return launch_splitk<double, true>(g, st);
+        return launch_mfma<double, true, 16, 2, 2, 8>(g, st);
+    default:
+        return (int)hipErrorInvalidValue;
+    }
+}
+
+extern "C" int tk_kernel_count(void) { return 12; }

@@ -1,0 +1,91 @@
+// Bandwidth-bound pre- and post-passes for gfx950.
+//
+// k_prep: unary pre-reduction and storage -> compute conversion.
+//   dst[i0 + n0 * (i1 + n1 * i2)] = sum_u op(src[t0[i0] + t1[i1] + t2[i2] + tu[u]])
+//   The sum over u runs in ascending u, i.e. in the order of the reference's
+//   unary odometer (product.c:370-391), so the reduced value is the same
+//   number the reference's sum_A / sum_B holds when the compute type is the
+//   reference's sum type.  One lane per output; consecutive lanes take
+//   consecutive i0, which is the unit-stride direction the host picks.
+// k_scatter: compute-type dense buffer -> D of another storage type.
+#include "tk_device.hpp"
+
+namespace tk {
+
+template <typename R, bool CPLX>
+__global__ __launch_bounds__(256) void k_prep(tk_prep p)
+{
+    const int64_t total = p.n0 * p.n1 * p.n2;
+    for (int64_t o = (int64_t)blockIdx.x * 256 + threadIdx.x; o < total; o += (int64_t)gridDim.x * 256) {
+        const int64_t i0 = o % p.n0, t = o / p.n0, i1 = t % p.n1, i2 = t / p.n1;
+        const int64_t base = p.t0[i0] + p.t1[i1] + p.t2[i2];
+        R sr = 0, si = 0;
+        for (int64_t u = 0; u < p.nu; ++u) {
+            double re, im;
+            load_storage(p.src, p.src_type, base + p.tu[u], re, im);
+            R vr = (R)re, vi = (R)im;
+            if (p.round) { vr = round_mode(vr, p.round); vi = round_mode(vi, p.round); }
+            if (p.conj) vi = -vi;
+            sr += vr;
+            if (CPLX) si += vi;
+        }
+        Elem<R, CPLX>::st(p.dst, o, Val<R, CPLX>{sr, si});
+    }
+}
+
+template <typename R, bool CPLX>
+__global__ __launch_bounds__(256) void k_scatter(tk_scatter s)
+{
+    const int64_t total = s.M * s.N * s.L;
+    for (int64_t o = (int64_t)blockIdx.x * 256 + threadIdx.x; o < total; o += (int64_t)gridDim.x * 256) {
+        const int64_t m = o % s.M, t = o / s.M, n = t % s.N, l = t / s.N;
+        Val<R, CPLX> v = Elem<R, CPLX>::ld(s.src, o);
+        store_storage(s.dst, s.dst_type, s.lD[l] + s.mD[m] + s.nD[n], (double)v.re, CPLX ? (double)v.im : 0.0);
+    }
+}
+
+static unsigned grid_for(int64_t total)
+{
+    int64_t b = (total + 255) / 256;
+    if (b > 65536) b = 65536;
+    if (b < 1) b = 1;
+    return (unsigned)b;
+}
+
+}  // namespace tk
+
+using namespace tk;
+
+extern "C" int tk_launch_prep(const tk_prep* pp, void* stream)
+{
+    const tk_prep& p = *pp;
+    hipStream_t st = (hipStream_t)stream;
+    const int64_t total = p.n0 * p.n1 * p.n2;
+    if (total <= 0) return 0;
+    const dim3 grid(grid_for(total)), block(256);
+    switch (p.ctype) {
+    case TK_F32: hipLaunchKernelGGL((k_prep<float, false>), grid, block, 0, st, p); break;
+    case TK_F64: hipLaunchKernelGGL((k_prep<double, false>), grid, block, 0, st, p); break;
+    case TK_C32: hipLaunchKernelGGL((k_prep<float, true>), grid, block, 0, st, p); break;
+    case TK_C64: hipLaunchKernelGGL((k_prep<double, true>), grid, block, 0, st, p); break;
+    default: return (int)hipErrorInvalidValue;
+    }
+    return (int)hipGetLastError();
+}
+
+extern "C" int tk_launch_scatter(const tk_scatter* sp, void* stream)
+{
+    const tk_scatter& s = *sp;
+    hipStream_t st = (hipStream_t)stream;
+    const int64_t total = s.M * s.N * s.L;
+    if (total <= 0) return 0;
+    const dim3 grid(grid_for(total)), block(256);
+    switch (s.ctype) {
+    case TK_F32: hipLaunchKernelGGL((k_scatter<float, false>), grid, block, 0, st, s); break;
+    case TK_F64: hipLaunchKernelGGL((k_scatter<double, false>), grid, block, 0, st, s); break;
+    case TK_C32: hipLaunchKernelGGL((k_scatter<float, true>), grid, block, 0, st, s); break;
+    case TK_C64: hipLaunchKernelGGL((k_scatter<double, true>), grid, block, 0, st, s); break;
+    default: return (int)hipErrorInvalidValue;
+    }
+    return (int)hipGetLastError();
+}

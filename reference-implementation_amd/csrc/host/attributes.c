@@ -1,0 +1,63 @@
+/*
+ * TAPP_attr_set / _get / _clear (attributes.h:14-18, declared but never
+ * defined by the reference).  The attribute object is an executor; keys are
+ * listed in tapp_amd.h.
+ */
+#include "internal.h"
+
+static struct tapp_exec* exec_of(TAPP_attr attr)
+{
+    return tk_exec_lookup((TAPP_executor)attr);
+}
+
+TAPP_error TAPP_attr_set(TAPP_attr attr, TAPP_key key, void* value)
+{
+    struct tapp_exec* e = exec_of(attr);
+    if (!e || !value) return TAPP_AMD_ERR_INVALID;
+    switch (key) {
+    case TAPP_AMD_ATTR_SPLIT_BF16: e->split_bf16 = *(int*)value != 0; return 0;
+    case TAPP_AMD_ATTR_KERNEL: {
+        int k = *(int*)value;
+        if (k < 0 || k > 3) return TAPP_AMD_ERR_INVALID;
+        e->force_kernel = k;
+        return 0;
+    }
+    case TAPP_AMD_ATTR_TIMING: e->timing = *(int*)value != 0; return 0;
+    case TAPP_AMD_ATTR_STREAM: {
+        hipStream_t s = *(hipStream_t*)value;
+        if (e->stream) hipStreamSynchronize(e->stream);
+        if (e->own_stream && e->stream) hipStreamDestroy(e->stream);
+        e->stream = s;
+        e->own_stream = false;
+        return 0;
+    }
+    default: return TAPP_AMD_ERR_KEY;
+    }
+}
+
+TAPP_error TAPP_attr_get(TAPP_attr attr, TAPP_key key, void** value)
+{
+    struct tapp_exec* e = exec_of(attr);
+    if (!e || !value) return TAPP_AMD_ERR_INVALID;
+    switch (key) {
+    case TAPP_AMD_ATTR_SPLIT_BF16: *(int*)value = e->split_bf16; return 0;
+    case TAPP_AMD_ATTR_KERNEL: *(int*)value = e->force_kernel; return 0;
+    case TAPP_AMD_ATTR_TIMING: *(int*)value = e->timing; return 0;
+    case TAPP_AMD_ATTR_LAST_KERNEL: *(int*)value = e->last_kernel; return 0;
+    case TAPP_AMD_ATTR_LAST_DEVICE_US: *(double*)value = e->last_device_us; return 0;
+    case TAPP_AMD_ATTR_STREAM: *(hipStream_t*)value = e->stream; return 0;
+    default: return TAPP_AMD_ERR_KEY;
+    }
+}
+
+TAPP_error TAPP_attr_clear(TAPP_attr attr, TAPP_key key)
+{
+    struct tapp_exec* e = exec_of(attr);
+    if (!e) return TAPP_AMD_ERR_INVALID;
+    switch (key) {
+    case TAPP_AMD_ATTR_SPLIT_BF16: e->split_bf16 = 0; return 0;
+    case TAPP_AMD_ATTR_KERNEL: e->force_kernel = 0; return 0;
+    case TAPP_AMD_ATTR_TIMING: e->timing = 0; return 0;
+    default: return TAPP_AMD_ERR_KEY;
+    }
+}

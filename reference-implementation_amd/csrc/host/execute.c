@@ -1,0 +1,392 @@
+/*
+ * Device orchestration of one product (the part of product.c:263-422 that
+ * runs the loop nest, re-designed for the GPU).
+ *
+ * Offset tables: each group (L, M, N, K, UA, UB) is linearised with its
+ * first mode fastest and every operand gets one int64 table per group
+ * holding sum_i coord_i * stride_i.  Tables depend only on the layouts, so
+ * they are built once per (plan, layout) and kept on the device in the
+ * plan's cache; a call with unchanged layouts uploads nothing.
+ */
+#include <stdlib.h>
+#include <string.h>
+
+#include "internal.h"
+
+#define ALIGN256(x) (((x) + 255u) & ~(size_t)255u)
+
+enum {
+    T_mA, T_mC, T_mD, T_nB, T_nC, T_nD, T_kA, T_kB, T_lA, T_lB, T_lC, T_lD, T_uA, T_uB,
+    /* dense layouts of the pre-pass buffers */
+    T_dmA, T_dkA, T_dlA, T_dnB, T_dkB, T_dlB, T_dmD, T_dnD, T_dlD, T_zero,
+    T_COUNT
+};
+
+struct tapp_plan_cache {
+    uint64_t key;
+    size_t off[T_COUNT];     /* element offset of each table in the arena */
+    size_t total;
+    int64_t* host;
+    int64_t* dev;
+    int device;
+};
+
+void tk_plan_cache_free(struct tapp_plan_cache* c)
+{
+    if (!c) return;
+    if (c->dev) hipFree(c->dev);
+    free(c->host);
+    free(c);
+}
+
+static uint64_t mix(uint64_t h, uint64_t v)
+{
+    h ^= v + 0x9e3779b97f4a7c15ull + (h << 6) + (h >> 2);
+    return h * 0xff51afd7ed558ccdull;
+}
+
+static uint64_t group_key(uint64_t h, const tk_group* g)
+{
+    h = mix(h, (uint64_t)g->n);
+    for (int i = 0; i < g->n; i++) {
+        h = mix(h, (uint64_t)g->ext[i]);
+        for (int x = 0; x < 4; x++) h = mix(h, (uint64_t)g->s[x][i]);
+    }
+    return h;
+}
+
+static uint64_t problem_key(const tk_problem* P)
+{
+    uint64_t h = 0x1234567ull;
+    h = group_key(h, &P->L); h = group_key(h, &P->M); h = group_key(h, &P->N);
+    h = group_key(h, &P->K); h = group_key(h, &P->UA); h = group_key(h, &P->UB);
+    h = mix(h, (uint64_t)P->nK);
+    return h;
+}
+
+/* tab[i] = sum_j coord_j(i) * stride_j over the group's modes, first fastest */
+static void fill_table(int64_t* tab, const tk_group* g, int x, int64_t n)
+{
+    if (n <= 0) return;
+    tab[0] = 0;
+    int64_t block = 1;
+    for (int i = 0; i < g->n; i++) {
+        for (int64_t c = 1; c < g->ext[i]; c++)
+            for (int64_t r = 0; r < block; r++) tab[c * block + r] = tab[r] + c * g->s[x][i];
+        block *= g->ext[i];
+    }
+}
+
+static void fill_linear(int64_t* tab, int64_t n, int64_t step)
+{
+    for (int64_t i = 0; i < n; i++) tab[i] = i * step;
+}
+
+static int build_tables(struct tapp_plan* plan, const tk_problem* P, int device, struct tapp_plan_cache** out)
+{
+    uint64_t key = problem_key(P);
+    struct tapp_plan_cache* c = plan->cache;
+    if (c && c->key == key && c->device == device && c->dev) { *out = c; return 0; }
+    if (c) { tk_plan_cache_free(c); plan->cache = NULL; }
+    c = calloc(1, sizeof *c);
+    if (!c) return TAPP_AMD_ERR_NOMEM;
+    const int64_t M = P->nM, N = P->nN, K = P->nK, L = P->nL;
+    const int64_t len[T_COUNT] = {
+        [T_mA] = M, [T_mC] = M, [T_mD] = M, [T_nB] = N, [T_nC] = N, [T_nD] = N,
+        [T_kA] = K, [T_kB] = K, [T_lA] = L, [T_lB] = L, [T_lC] = L, [T_lD] = L,
+        [T_uA] = P->nUA, [T_uB] = P->nUB,
+        [T_dmA] = M, [T_dkA] = K, [T_dlA] = L, [T_dnB] = N, [T_dkB] = K, [T_dlB] = L,
+        [T_dmD] = M, [T_dnD] = N, [T_dlD] = L, [T_zero] = 1};
+    size_t total = 0;
+    for (int t = 0; t < T_COUNT; t++) {
+        c->off[t] = total;
+        total += (size_t)(len[t] > 0 ? len[t] : 1);
+        total = (total + 31) & ~(size_t)31;   /* 256-byte aligned sub-tables */
+    }
+    c->total = total;
+    c->host = malloc(total * sizeof(int64_t));
+    if (!c->host) { free(c); return TAPP_AMD_ERR_NOMEM; }
+    int64_t* h = c->host;
+    fill_table(h + c->off[T_mA], &P->M, 0, M);
+    fill_table(h + c->off[T_mC], &P->M, 2, M);
+    fill_table(h + c->off[T_mD], &P->M, 3, M);
+    fill_table(h + c->off[T_nB], &P->N, 1, N);
+    fill_table(h + c->off[T_nC], &P->N, 2, N);
+    fill_table(h + c->off[T_nD], &P->N, 3, N);
+    fill_table(h + c->off[T_kA], &P->K, 0, K);
+    fill_table(h + c->off[T_kB], &P->K, 1, K);
+    fill_table(h + c->off[T_lA], &P->L, 0, L);
+    fill_table(h + c->off[T_lB], &P->L, 1, L);
+    fill_table(h + c->off[T_lC], &P->L, 2, L);
+    fill_table(h + c->off[T_lD], &P->L, 3, L);
+    fill_table(h + c->off[T_uA], &P->UA, 0, P->nUA);
+    fill_table(h + c->off[T_uB], &P->UB, 1, P->nUB);
+    fill_linear(h + c->off[T_dmA], M, 1);
+    fill_linear(h + c->off[T_dkA], K, M);
+    fill_linear(h + c->off[T_dlA], L, M * K);
+    fill_linear(h + c->off[T_dnB], N, 1);
+    fill_linear(h + c->off[T_dkB], K, N);
+    fill_linear(h + c->off[T_dlB], L, N * K);
+    fill_linear(h + c->off[T_dmD], M, 1);
+    fill_linear(h + c->off[T_dnD], N, M);
+    fill_linear(h + c->off[T_dlD], L, M * N);
+    h[c->off[T_zero]] = 0;
+    if (hipMalloc((void**)&c->dev, total * sizeof(int64_t)) != hipSuccess) {
+        free(c->host); free(c);
+        return TAPP_AMD_ERR_NOMEM;
+    }
+    if (hipMemcpy(c->dev, c->host, total * sizeof(int64_t), hipMemcpyHostToDevice) != hipSuccess) {
+        tk_plan_cache_free(c);
+        return TAPP_AMD_ERR_DEVICE;
+    }
+    c->key = key;
+    c->device = device;
+    plan->cache = c;
+    *out = c;
+    return 0;
+}
+
+/* --- residency ------------------------------------------------------------ */
+
+static bool is_device_ptr(const void* p)
+{
+    if (!p) return false;
+    hipPointerAttribute_t a;
+    hipError_t e = hipPointerGetAttributes(&a, p);
+    if (e != hipSuccess) { (void)hipGetLastError(); return false; }
+    return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged;
+}
+
+static int ensure_ws(struct tapp_exec* ex, size_t bytes)
+{
+    if (bytes <= ex->ws_size) return 0;
+    if (ex->ws) { hipStreamSynchronize(ex->stream); hipFree(ex->ws); ex->ws = NULL; ex->ws_size = 0; }
+    size_t want = bytes + bytes / 4;
+    if (hipMalloc(&ex->ws, want) != hipSuccess) { (void)hipGetLastError(); return TAPP_AMD_ERR_NOMEM; }
+    ex->ws_size = want;
+    return 0;
+}
+
+static size_t ctype_size(int ct)
+{
+    switch (ct) {
+    case TK_F32: return 4;
+    case TK_F64: return 8;
+    case TK_C32: return 8;
+    case TK_C64: return 16;
+    default: return 0;
+    }
+}
+
+static bool compatible(int storage, int ct)
+{
+    return (storage == 0 && ct == TK_F32) || (storage == 1 && ct == TK_F64) || (storage == 2 && ct == TK_C32) ||
+           (storage == 3 && ct == TK_C64);
+}
+
+/* Unit step over groups of `ve` consecutive rows and every vector start a
+ * multiple of `ve`, with the other tables multiples of `ve` and the base
+ * 16-byte aligned: 16-byte loads along the rows are then exact. */
+static bool vec_rows_ok(const int64_t* rt, int64_t n, const int64_t* kt, int64_t nk, const int64_t* lt,
+                        int64_t nl, int ve, const void* base)
+{
+    if (ve <= 1) return false;
+    if (((uintptr_t)base) % 16) return false;
+    for (int64_t r = 0; r < n; r += ve) {
+        if (rt[r] % ve) return false;
+        for (int j = 1; j < ve && r + j < n; j++)
+            if (rt[r + j] != rt[r] + j) return false;
+    }
+    for (int64_t k = 0; k < nk; k++) if (kt[k] % ve) return false;
+    for (int64_t l = 0; l < nl; l++) if (lt[l] % ve) return false;
+    return true;
+}
+
+typedef struct {
+    const void* base;   /* device address of element (0, ..., 0) */
+    bool staged;
+} dptr;
+
+int tk_execute(struct tapp_plan* p, struct tapp_exec* ex, const void* alpha, const void* A, const void* B,
+               const void* beta, const void* C, void* D)
+{
+    tk_problem P;
+    int rc = tk_plan_problem(p, ex, alpha, beta, C, &P);
+    if (rc) return rc;
+    if (P.empty) return 0;
+    if (!ex->stream) return TAPP_AMD_ERR_DEVICE;
+    if (!A || !B || !D || (!P.beta_zero && !C)) return TAPP_AMD_ERR_INVALID;
+    if (hipSetDevice(ex->device) != hipSuccess) return TAPP_AMD_ERR_DEVICE;
+
+    struct tapp_plan_cache* tc;
+    if ((rc = build_tables(p, &P, ex->device, &tc))) return rc;
+    const int64_t* T = tc->dev;
+    const int64_t* H = tc->host;
+#define DT(t) (T + tc->off[t])
+#define HT(t) (H + tc->off[t])
+
+    const int64_t M = P.nM, N = P.nN, K = P.nK, L = P.nL;
+    const int ct = P.ctype;
+    const size_t cs = ctype_size(ct);
+
+    /* what has to be staged or converted */
+    const void* src[4] = {A, B, P.beta_zero ? NULL : C, D};
+    bool on_dev[4];
+    size_t span_bytes[4] = {0, 0, 0, 0};
+    for (int x = 0; x < 4; x++) {
+        on_dev[x] = src[x] ? is_device_ptr(src[x]) : true;
+        if (src[x] && !on_dev[x] && P.hi[x] >= P.lo[x])
+            span_bytes[x] = (size_t)(P.hi[x] - P.lo[x] + 1) * tk_type_size(P.type[x]);
+    }
+    const bool needA = P.nUA > 1 || !compatible(P.type[0], ct) || (P.op_A && P.nUA > 1);
+    const bool needB = P.nUB > 1 || !compatible(P.type[1], ct);
+    const bool needC = !P.beta_zero && !compatible(P.type[2], ct);
+    const bool needD = !compatible(P.type[3], ct);
+    const size_t szA = needA ? (size_t)(M * K * L) * cs : 0;
+    const size_t szB = needB ? (size_t)(N * K * L) * cs : 0;
+    const size_t szC = needC ? (size_t)(M * N * L) * cs : 0;
+    const size_t szD = needD ? (size_t)(M * N * L) * cs : 0;
+
+    size_t off_stage[4], off_tmp[4], total = 0;
+    for (int x = 0; x < 4; x++) { off_stage[x] = total; total = ALIGN256(total + span_bytes[x]); }
+    const size_t tmp_sz[4] = {szA, szB, szC, szD};
+    for (int x = 0; x < 4; x++) { off_tmp[x] = total; total = ALIGN256(total + tmp_sz[x]); }
+    if (total && (rc = ensure_ws(ex, total))) return rc;
+    char* ws = (char*)ex->ws;
+    hipStream_t st = ex->stream;
+
+    /* operands on the device: element (0, ..., 0) addresses */
+    dptr d[4];
+    for (int x = 0; x < 4; x++) {
+        d[x].staged = false;
+        d[x].base = src[x];
+        if (!src[x]) continue;
+        const size_t es = tk_type_size(P.type[x]);
+        if (!on_dev[x]) {
+            if (span_bytes[x] == 0) { d[x].base = ws + off_stage[x]; continue; }
+            char* buf = ws + off_stage[x];
+            const char* hsrc = (const char*)src[x] + P.lo[x] * (int64_t)es;
+            /* D is copied in too so elements the product does not write keep their values */
+            if (hipMemcpyAsync(buf, hsrc, span_bytes[x], hipMemcpyHostToDevice, st) != hipSuccess)
+                return TAPP_AMD_ERR_DEVICE;
+            d[x].base = buf - P.lo[x] * (int64_t)es;
+            d[x].staged = true;
+        }
+        d[x].base = (const char*)d[x].base + P.pin[x] * (int64_t)es;
+    }
+
+    if (ex->timing) hipEventRecord(ex->ev0, st);
+
+    /* pre-passes */
+    tk_gett g;
+    memset(&g, 0, sizeof g);
+    g.ctype = ct;
+    g.mode = P.mode;
+    g.M = M; g.N = N; g.K = K; g.L = L;
+    g.conjA = P.op_A; g.conjB = P.op_B; g.conjC = P.op_C; g.conjD = P.op_D;
+    g.conjD_via_f32 = P.conjD_via_f32;
+    g.round_C = P.round_C;
+    g.alpha[0] = P.alpha[0]; g.alpha[1] = P.alpha[1];
+    g.beta[0] = P.beta[0]; g.beta[1] = P.beta[1];
+    g.beta_zero = P.beta_zero;
+
+    g.A = d[0].base; g.mA = DT(T_mA); g.kA = DT(T_kA); g.lA = DT(T_lA);
+    g.B = d[1].base; g.nB = DT(T_nB); g.kB = DT(T_kB); g.lB = DT(T_lB);
+    g.C = d[2].base; g.mC = DT(T_mC); g.nC = DT(T_nC); g.lC = DT(T_lC);
+    g.D = (void*)d[3].base; g.mD = DT(T_mD); g.nD = DT(T_nD); g.lD = DT(T_lD);
+    const int64_t *hmA = HT(T_mA), *hkA = HT(T_kA), *hlA = HT(T_lA);
+    const int64_t *hnB = HT(T_nB), *hkB = HT(T_kB), *hlB = HT(T_lB);
+
+    if (needA && K > 0) {
+        tk_prep pr = {P.type[0], ct, P.op_A, (P.nUA > 1 && (P.mode == TK_MODE_F16 || P.mode == TK_MODE_BF16)) ? P.mode : 0,
+                      d[0].base, ws + off_tmp[0], M, K, L, P.nUA, DT(T_mA), DT(T_kA), DT(T_lA), DT(T_uA)};
+        if ((rc = tk_launch_prep(&pr, st))) return TAPP_AMD_ERR_DEVICE;
+        g.A = ws + off_tmp[0]; g.mA = DT(T_dmA); g.kA = DT(T_dkA); g.lA = DT(T_dlA); g.conjA = 0;
+        hmA = HT(T_dmA); hkA = HT(T_dkA); hlA = HT(T_dlA);
+    }
+    if (needB && K > 0) {
+        tk_prep pr = {P.type[1], ct, P.op_B, (P.nUB > 1 && (P.mode == TK_MODE_F16 || P.mode == TK_MODE_BF16)) ? P.mode : 0,
+                      d[1].base, ws + off_tmp[1], N, K, L, P.nUB, DT(T_nB), DT(T_kB), DT(T_lB), DT(T_uB)};
+        if ((rc = tk_launch_prep(&pr, st))) return TAPP_AMD_ERR_DEVICE;
+        g.B = ws + off_tmp[1]; g.nB = DT(T_dnB); g.kB = DT(T_dkB); g.lB = DT(T_dlB); g.conjB = 0;
+        hnB = HT(T_dnB); hkB = HT(T_dkB); hlB = HT(T_dlB);
+    }
+    if (needC) {
+        tk_prep pr = {P.type[2], ct, 0, 0, d[2].base, ws + off_tmp[2], M, N, L, 1,
+                      DT(T_mC), DT(T_nC), DT(T_lC), DT(T_zero)};
+        if ((rc = tk_launch_prep(&pr, st))) return TAPP_AMD_ERR_DEVICE;
+        g.C = ws + off_tmp[2]; g.mC = DT(T_dmD); g.nC = DT(T_dnD); g.lC = DT(T_dlD);
+    }
+    if (needD) {
+        g.D = ws + off_tmp[3]; g.mD = DT(T_dmD); g.nD = DT(T_dnD); g.lD = DT(T_dlD);
+    }
+
+    /* orientation: output lanes run along M; swap the roles of A and B when
+     * D's unit-stride mode belongs to N */
+    {
+        int64_t dm = INT64_MAX, dn = INT64_MAX;
+        for (int i = 0; i < P.M.n; i++) if (llabs(P.M.s[3][i]) < dm) dm = llabs(P.M.s[3][i]);
+        for (int i = 0; i < P.N.n; i++) if (llabs(P.N.s[3][i]) < dn) dn = llabs(P.N.s[3][i]);
+        if (needD) { dm = 1; dn = M; }
+        if (dn < dm) {
+            tk_gett s = g;
+            s.M = N; s.N = M;
+            s.A = g.B; s.mA = g.nB; s.kA = g.kB; s.lA = g.lB; s.conjA = g.conjB;
+            s.B = g.A; s.nB = g.mA; s.kB = g.kA; s.lB = g.lA; s.conjB = g.conjA;
+            s.mC = g.nC; s.nC = g.mC; s.mD = g.nD; s.nD = g.mD;
+            g = s;
+            const int64_t *t1 = hmA, *t2 = hkA, *t3 = hlA;
+            hmA = hnB; hkA = hkB; hlA = hlB;
+            hnB = t1; hkB = t2; hlB = t3;
+        }
+    }
+
+    /* kernel choice */
+    const int64_t outs = g.M * g.N * g.L;
+    int kern = ex->force_kernel;
+    if (kern == TK_KERNEL_AUTO) {
+        if (g.K == 0) kern = TK_KERNEL_SIMPLE;
+        else if (outs <= 4096 && g.K >= 1024) kern = TK_KERNEL_SPLITK;
+        else if (g.M >= 16 && g.N >= 16 && g.K >= 4 && outs * g.K >= (1 << 16)) kern = TK_KERNEL_MFMA;
+        else kern = TK_KERNEL_SIMPLE;
+    }
+    if (kern == TK_KERNEL_SPLITK && outs > (int64_t)1 << 31) kern = TK_KERNEL_SIMPLE;
+    g.kernel = kern;
+
+    /* tile access patterns for the MFMA kernel */
+    if (kern == TK_KERNEL_MFMA) {
+        const int ve = (int)(16 / cs) > 0 ? (int)(16 / cs) : 1;
+        if (vec_rows_ok(hmA, g.M, hkA, g.K, hlA, g.L, ve, g.A)) { g.a_load = TK_CONTIG_ROW; g.a_vec = 1; }
+        else if (vec_rows_ok(hkA, g.K, hmA, g.M, hlA, g.L, ve, g.A)) { g.a_load = TK_CONTIG_K; g.a_vec = 1; }
+        else { g.a_load = TK_GATHER; g.a_vec = 0; }
+        if (vec_rows_ok(hnB, g.N, hkB, g.K, hlB, g.L, ve, g.B)) { g.b_load = TK_CONTIG_ROW; g.b_vec = 1; }
+        else if (vec_rows_ok(hkB, g.K, hnB, g.N, hlB, g.L, ve, g.B)) { g.b_load = TK_CONTIG_K; g.b_vec = 1; }
+        else { g.b_load = TK_GATHER; g.b_vec = 0; }
+    }
+
+    if ((rc = tk_launch_gett(&g, st))) return TAPP_AMD_ERR_DEVICE;
+    ex->last_kernel = kern;
+
+    if (needD) {
+        tk_scatter sc = {ct, P.type[3], ws + off_tmp[3], (void*)d[3].base, M, N, L,
+                         DT(T_mD), DT(T_nD), DT(T_lD)};
+        if ((rc = tk_launch_scatter(&sc, st))) return TAPP_AMD_ERR_DEVICE;
+    }
+    if (ex->timing) hipEventRecord(ex->ev1, st);
+
+    if (d[3].staged) {
+        const size_t es = tk_type_size(P.type[3]);
+        char* hdst = (char*)D + P.lo[3] * (int64_t)es;
+        if (hipMemcpyAsync(hdst, ws + off_stage[3], span_bytes[3], hipMemcpyDeviceToHost, st) != hipSuccess)
+            return TAPP_AMD_ERR_DEVICE;
+    }
+    if (hipStreamSynchronize(st) != hipSuccess) return TAPP_AMD_ERR_DEVICE;
+    if (ex->timing) {
+        float ms = 0;
+        hipEventElapsedTime(&ms, ex->ev0, ex->ev1);
+        ex->last_device_us = ms * 1000.0;
+    }
+    return 0;
+#undef DT
+#undef HT
+}

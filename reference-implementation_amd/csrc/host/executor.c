@@ -1,0 +1,105 @@
+/*
+ * Executor = device context: a HIP stream on the device current at creation,
+ * a device workspace grown on demand, timing events and attributes.
+ *
+ * The reference's executor is a malloc'd int holding the kind 1
+ * (executor.c:9-18) and TAPP_execute_product accepts any executor whose int
+ * is 1, 2 or 12 (product.c:878-885).  The first field here is that int, and
+ * a live-executor registry lets TAPP_execute_product tell this library's
+ * executors from a caller's bare int: the latter runs on a process-wide
+ * default context.
+ */
+#include <pthread.h>
+#include <stdlib.h>
+
+#include "internal.h"
+
+#define MAX_EXEC 4096
+static struct tapp_exec* live[MAX_EXEC];
+static int nlive;
+static pthread_mutex_t mu = PTHREAD_MUTEX_INITIALIZER;
+static struct tapp_exec* default_exec;
+
+void tk_exec_register(struct tapp_exec* e)
+{
+    pthread_mutex_lock(&mu);
+    if (nlive < MAX_EXEC) live[nlive++] = e;
+    pthread_mutex_unlock(&mu);
+}
+
+void tk_exec_unregister(struct tapp_exec* e)
+{
+    pthread_mutex_lock(&mu);
+    for (int i = 0; i < nlive; i++)
+        if (live[i] == e) { live[i] = live[--nlive]; break; }
+    pthread_mutex_unlock(&mu);
+}
+
+struct tapp_exec* tk_exec_lookup(TAPP_executor h)
+{
+    struct tapp_exec* e = (struct tapp_exec*)h;
+    struct tapp_exec* found = NULL;
+    pthread_mutex_lock(&mu);
+    for (int i = 0; i < nlive; i++)
+        if (live[i] == e) { found = e; break; }
+    pthread_mutex_unlock(&mu);
+    return found;
+}
+
+static int exec_init(struct tapp_exec* e)
+{
+    e->kind = 1;
+    e->magic = TAPP_AMD_EXEC_MAGIC;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return TAPP_AMD_ERR_DEVICE;
+    e->device = dev;
+    if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) return TAPP_AMD_ERR_DEVICE;
+    e->own_stream = true;
+    if (hipEventCreate(&e->ev0) != hipSuccess || hipEventCreate(&e->ev1) != hipSuccess) return TAPP_AMD_ERR_DEVICE;
+    return 0;
+}
+
+struct tapp_exec* tk_default_exec(void)
+{
+    pthread_mutex_lock(&mu);
+    if (!default_exec) {
+        struct tapp_exec* e = calloc(1, sizeof *e);
+        if (e && exec_init(e) == 0) default_exec = e;
+        else free(e);
+    }
+    struct tapp_exec* d = default_exec;
+    pthread_mutex_unlock(&mu);
+    return d;
+}
+
+TAPP_error TAPP_create_executor(TAPP_executor* exec)
+{
+    struct tapp_exec* e = calloc(1, sizeof *e);
+    if (!e) return TAPP_AMD_ERR_NOMEM;
+    int rc = exec_init(e);
+    if (rc) {
+        /* No usable device: keep the reference's contract (an executor whose
+         * int is 1 is returned with code 0); execution reports the device
+         * error. */
+        e->kind = 1;
+        e->stream = NULL;
+        e->own_stream = false;
+    }
+    tk_exec_register(e);
+    *exec = (TAPP_executor)e;
+    return 0;
+}
+
+TAPP_error TAPP_destroy_executor(TAPP_executor exec)
+{
+    struct tapp_exec* e = tk_exec_lookup(exec);
+    if (!e) return 0;
+    tk_exec_unregister(e);
+    if (e->stream) hipStreamSynchronize(e->stream);
+    if (e->ws) hipFree(e->ws);
+    if (e->ev0) hipEventDestroy(e->ev0);
+    if (e->ev1) hipEventDestroy(e->ev1);
+    if (e->own_stream && e->stream) hipStreamDestroy(e->stream);
+    free(e);
+    return 0;
+}

@@ -1,0 +1,116 @@
+/*
+ * Host-layer internals (C11).  Object layouts and the plan stage.
+ */
+#ifndef TAPP_AMD_INTERNAL_H_
+#define TAPP_AMD_INTERNAL_H_
+
+#include <stdbool.h>
+#include <stdint.h>
+
+#include <hip/hip_runtime_api.h>
+
+#include "tapp.h"
+#include "tapp_amd.h"
+#include "../tk.h"
+
+/* --- objects ------------------------------------------------------------ */
+
+/* Layout descriptor (reference: struct tensor_info, ref_impl.h:13-19). */
+struct tapp_info {
+    TAPP_datatype type;
+    int nmode;
+    int64_t* extents;
+    int64_t* strides;
+};
+
+struct tapp_plan_cache;
+
+/* Product plan (reference: struct plan, ref_impl.h:21-37).  Layouts are held
+ * by handle, so later TAPP_set_* calls on an info change the plan, as in the
+ * reference (product.c:86-107). */
+struct tapp_plan {
+    TAPP_handle handle;
+    TAPP_element_op op[4];
+    TAPP_tensor_info info[4];
+    int64_t* idx[4];
+    int nidx[4];               /* label count copied at creation */
+    TAPP_prectype prec;
+    struct tapp_plan_cache* cache;
+};
+
+#define TAPP_AMD_EXEC_MAGIC 0x54415050u /* "TAPP" */
+
+/* Executor = device context.  The first field is the int kind tag the
+ * reference's executor points at (executor.c:9-18), so the reference's
+ * validity rule (product.c:878-885) reads the same value. */
+struct tapp_exec {
+    int kind;                  /* 1 */
+    uint32_t magic;
+    int device;
+    hipStream_t stream;
+    bool own_stream;
+    hipEvent_t ev0, ev1;
+    /* device workspace: grown on demand, reused across calls */
+    void* ws;
+    size_t ws_size;
+    /* attributes */
+    int split_bf16;
+    int force_kernel;
+    int timing;
+    int last_kernel;
+    double last_device_us;
+};
+
+struct tapp_handle_obj {
+    uint32_t magic;
+    int device;
+};
+
+/* --- plan stage ---------------------------------------------------------- */
+
+#define TK_MAXG 64   /* max modes per group after compression */
+
+typedef struct {
+    int n;
+    int64_t ext[TK_MAXG];
+    int64_t s[4][TK_MAXG];     /* strides of A, B, C, D (0 where absent) */
+} tk_group;
+
+/* Everything TAPP_execute_product derives from the plan before touching
+ * data. */
+typedef struct {
+    int rc;                    /* validation result, 0 = ok */
+    int type[4];               /* storage types A, B, C, D */
+    int ctype;                 /* compute type TK_* */
+    int mode;                  /* TK_MODE_* */
+    int op_A, op_B, op_C, op_D;
+    bool conjD_via_f32;
+    bool round_C;
+    bool beta_zero;
+    double alpha[2], beta[2];
+    tk_group L, M, N, K, UA, UB;
+    int64_t nL, nM, nN, nK, nUA, nUB;
+    int64_t pin[4];            /* element offset added for pinned zero-stride D modes */
+    int64_t lo[4], hi[4];      /* element span of A, B, C, D; hi < lo when empty */
+    bool empty;                /* no D element is written */
+} tk_problem;
+
+int tk_plan_problem(const struct tapp_plan* p, const struct tapp_exec* ex, const void* alpha,
+                    const void* beta, const void* C, tk_problem* out);
+
+/* storage helpers */
+size_t tk_type_size(int type);
+bool tk_type_complex(int type);
+
+/* registry of live executors / handles */
+struct tapp_exec* tk_exec_lookup(TAPP_executor e);
+struct tapp_exec* tk_default_exec(void);
+void tk_exec_register(struct tapp_exec* e);
+void tk_exec_unregister(struct tapp_exec* e);
+
+/* execution */
+int tk_execute(struct tapp_plan* p, struct tapp_exec* ex, const void* alpha, const void* A,
+               const void* B, const void* beta, const void* C, void* D);
+void tk_plan_cache_free(struct tapp_plan_cache* c);
+
+#endif /* TAPP_AMD_INTERNAL_H_ */

@@ -1,0 +1,112 @@
+/*
+ * Internal C ABI between the C host layer (csrc/host) and the gfx950 HIP
+ * kernels (csrc/hip).  Plain C types only: device pointers, sizes, int64
+ * offset tables.  Not part of the public API.
+ *
+ * Index model (shared by every kernel).  After the plan stage a product is
+ *     D[l, m, n] = op_D( alpha * sum_k A'[l, m, k] * B'[l, k, n] + beta * op_C(C[l, m, n]) )
+ * where each group (L = Hadamard/batch labels, M = labels of A and D only,
+ * N = labels of B and D only, K = contracted labels) is linearised and every
+ * operand reaches element (l, m, n, k) through one int64 offset table per
+ * group: address = base + lX[l] + mX[m] + nX[n] + kX[k] (in elements).
+ * Tables carry signed strides, diagonals (summed strides) and zero strides
+ * unchanged.
+ */
+#ifndef TK_H_
+#define TK_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* compute types */
+enum { TK_F32 = 0, TK_F64 = 1, TK_C32 = 2, TK_C64 = 3 };
+
+/* multiply modes for single-precision compute */
+enum {
+    TK_MODE_PLAIN = 0,     /* FP32 or FP64 arithmetic as stored */
+    TK_MODE_F16 = 1,       /* operands rounded to binary16, FP16 MFMA, FP32 accumulate */
+    TK_MODE_BF16 = 2,      /* operands rounded to bfloat16, BF16 MFMA, FP32 accumulate */
+    TK_MODE_SPLIT_BF16 = 3 /* x = hi + lo in bfloat16, hi*hi + hi*lo + lo*hi, FP32 accumulate */
+};
+
+/* how a kernel may read an operand tile */
+enum {
+    TK_GATHER = 0,      /* per-element table lookups */
+    TK_CONTIG_ROW = 1,  /* unit step along the M (A, C, D) or N (B) group inside each tile */
+    TK_CONTIG_K = 2     /* unit step along K inside each tile */
+};
+
+typedef struct tk_gett {
+    int ctype;               /* TK_F32 .. TK_C64 */
+    int mode;                /* TK_MODE_* (single precision only) */
+    int64_t M, N, K, L;
+    const void* A;           /* device addresses of element (0,...,0) */
+    const void* B;
+    const void* C;           /* NULL when beta == 0: C is never read */
+    void* D;
+    const int64_t *mA, *mC, *mD;   /* [M] */
+    const int64_t *nB, *nC, *nD;   /* [N] */
+    const int64_t *kA, *kB;        /* [K] */
+    const int64_t *lA, *lB, *lC, *lD;  /* [L] */
+    int conjA, conjB, conjC, conjD;
+    int conjD_via_f32;       /* conj of the result goes through complex float */
+    int round_C;             /* C rounded like the operands (low-precision modes) */
+    double alpha[2], beta[2];/* (re, im), already in the compute precision */
+    int beta_zero;
+    int a_load, b_load;      /* TK_GATHER / TK_CONTIG_ROW / TK_CONTIG_K */
+    int a_vec, b_vec;        /* vector width (elements) usable on the contiguous dim */
+    int d_contig;            /* D (and C if c_contig) unit step along M inside each tile */
+    int c_contig;
+    int kernel;              /* chosen kernel variant, TK_KERNEL_* */
+} tk_gett;
+
+enum {
+    TK_KERNEL_AUTO = 0,
+    TK_KERNEL_SIMPLE = 1,    /* one lane per output element, FMA over K */
+    TK_KERNEL_MFMA = 2,      /* LDS-tiled MFMA GETT */
+    TK_KERNEL_SPLITK = 3     /* one workgroup per output, K split across lanes */
+};
+
+/* Pre-pass: reduce unary modes, apply conj and input rounding, convert the
+ * storage type to the compute type and write a dense buffer
+ *     dst[i0 + n0 * (i1 + n1 * i2)] = sum_u op(src[t0[i0] + t1[i1] + t2[i2] + tu[u]])
+ * summing u in ascending order (the reference's odometer order). */
+typedef struct tk_prep {
+    int src_type;            /* TAPP_datatype of the source */
+    int ctype;               /* compute type of dst */
+    int conj;
+    int round;               /* 0, TK_MODE_F16, TK_MODE_BF16: rounding of each loaded value */
+    const void* src;
+    void* dst;
+    int64_t n0, n1, n2, nu;
+    const int64_t *t0, *t1, *t2, *tu;
+} tk_prep;
+
+/* Post-pass for mixed storage: D[l, m, n] (type_D, D's tables) =
+ * convert(src dense compute-type buffer, index m + M * (n + N * l)). */
+typedef struct tk_scatter {
+    int ctype;
+    int dst_type;
+    const void* src;
+    void* dst;
+    int64_t M, N, L;
+    const int64_t *mD, *nD, *lD;
+} tk_scatter;
+
+/* All launchers are asynchronous on `stream` (a hipStream_t) and return 0
+ * or a hipError_t value. */
+int tk_launch_gett(const tk_gett* g, void* stream);
+int tk_launch_prep(const tk_prep* p, void* stream);
+int tk_launch_scatter(const tk_scatter* s, void* stream);
+
+/* Number of kernel variants compiled in, for diagnostics. */
+int tk_kernel_count(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TK_H_ */

@@ -1,0 +1,177 @@
+"""ctypes binding of the TAPP C API.
+
+The same binding drives any library that exports the TAPP API: this
+package's libtapp_amd.so (the GPU implementation) and, in the test suite,
+the compiled reference library.  It mirrors the ctypes layer the reference
+uses in its own Python tests (test/test.py:14-120 of
+TAPPorg/reference-implementation): handles are intptr_t, data is passed as
+raw pointers, scalars by pointer.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libtapp_amd.so")
+
+# api/include/tapp/datatype.h:17-95, product.h:17-21
+F32, F64, C32, C64, F16, BF16 = 0, 1, 2, 3, 4, 5
+DEFAULT_PREC, F32F32_ACCUM_F32, F64F64_ACCUM_F64 = -1, 0, 1
+F16F16_ACCUM_F16, F16F16_ACCUM_F32, BF16BF16_ACCUM_F32 = 4, 5, 6
+IDENTITY, CONJUGATE = 0, 1
+
+# tapp_amd.h
+ATTR_SPLIT_BF16 = 0x1001
+ATTR_KERNEL = 0x1002
+ATTR_LAST_KERNEL = 0x1003
+ATTR_LAST_DEVICE_US = 0x1004
+ATTR_STREAM = 0x1005
+ATTR_TIMING = 0x1006
+KERNEL_AUTO, KERNEL_SIMPLE, KERNEL_MFMA, KERNEL_SPLITK = 0, 1, 2, 3
+
+TYPE_SIZE = {F32: 4, F64: 8, C32: 8, C64: 16, F16: 2, BF16: 2}
+
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_H = ctypes.c_ssize_t  # intptr_t
+
+
+def _arr64(vals):
+    vals = [int(v) for v in vals]
+    return (ctypes.c_int64 * max(1, len(vals)))(*vals)
+
+
+class TappLib:
+    """A loaded TAPP implementation.  `path` defaults to this package's GPU
+    library; loading is RTLD_LOCAL so two implementations can share a
+    process."""
+
+    API = (
+        "TAPP_create_tensor_info", "TAPP_destroy_tensor_info", "TAPP_get_nmodes", "TAPP_set_nmodes",
+        "TAPP_get_extents", "TAPP_set_extents", "TAPP_get_strides", "TAPP_set_strides",
+        "TAPP_create_tensor_product", "TAPP_destroy_tensor_product", "TAPP_execute_product",
+        "TAPP_create_executor", "TAPP_destroy_executor", "TAPP_create_handle", "TAPP_destroy_handle",
+        "TAPP_check_success", "TAPP_explain_error",
+    )
+    EXTENSIONS = (
+        "TAPP_execute_batched_product", "TAPP_destroy_status", "TAPP_attr_set", "TAPP_attr_get",
+        "TAPP_attr_clear", "TAPP_AMD_version", "TAPP_AMD_plan_info",
+    )
+
+    def __init__(self, path=None):
+        self.path = path or LIB_PATH
+        if not os.path.exists(self.path):
+            raise FileNotFoundError(
+                f"{self.path} is not built: run `make -C reference-implementation_amd` or __graft_entry__.build()")
+        self.lib = ctypes.CDLL(self.path, mode=os.RTLD_LOCAL | os.RTLD_NOW)
+        L = self.lib
+        L.TAPP_create_tensor_info.argtypes = [ctypes.POINTER(_H), ctypes.c_int, ctypes.c_int, _i64p, _i64p]
+        L.TAPP_destroy_tensor_info.argtypes = [_H]
+        L.TAPP_get_nmodes.argtypes = [_H]
+        L.TAPP_set_nmodes.argtypes = [_H, ctypes.c_int]
+        L.TAPP_get_extents.argtypes = [_H, _i64p]
+        L.TAPP_get_extents.restype = None
+        L.TAPP_set_extents.argtypes = [_H, _i64p]
+        L.TAPP_get_strides.argtypes = [_H, _i64p]
+        L.TAPP_get_strides.restype = None
+        L.TAPP_set_strides.argtypes = [_H, _i64p]
+        L.TAPP_create_tensor_product.argtypes = [ctypes.POINTER(_H), _H,
+                                                 ctypes.c_int, _H, _i64p, ctypes.c_int, _H, _i64p,
+                                                 ctypes.c_int, _H, _i64p, ctypes.c_int, _H, _i64p, ctypes.c_int]
+        L.TAPP_destroy_tensor_product.argtypes = [_H]
+        L.TAPP_execute_product.argtypes = [_H, _H, ctypes.c_void_p] + [ctypes.c_void_p] * 6
+        L.TAPP_create_executor.argtypes = [ctypes.POINTER(_H)]
+        L.TAPP_destroy_executor.argtypes = [_H]
+        L.TAPP_create_handle.argtypes = [ctypes.POINTER(_H)]
+        L.TAPP_destroy_handle.argtypes = [_H]
+        L.TAPP_check_success.argtypes = [ctypes.c_int]
+        L.TAPP_check_success.restype = ctypes.c_bool
+        L.TAPP_explain_error.argtypes = [ctypes.c_int, ctypes.c_size_t, ctypes.c_char_p]
+        L.TAPP_explain_error.restype = ctypes.c_size_t
+        self.has_extensions = hasattr(L, "TAPP_AMD_version")
+        if self.has_extensions:
+            L.TAPP_execute_batched_product.argtypes = [_H, _H, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                                                       ctypes.POINTER(ctypes.c_void_p),
+                                                       ctypes.POINTER(ctypes.c_void_p), ctypes.c_void_p,
+                                                       ctypes.POINTER(ctypes.c_void_p),
+                                                       ctypes.POINTER(ctypes.c_void_p)]
+            L.TAPP_destroy_status.argtypes = [_H]
+            L.TAPP_attr_set.argtypes = [_H, ctypes.c_int, ctypes.c_void_p]
+            L.TAPP_attr_get.argtypes = [_H, ctypes.c_int, ctypes.c_void_p]
+            L.TAPP_attr_clear.argtypes = [_H, ctypes.c_int]
+            L.TAPP_AMD_version.restype = ctypes.c_char_p
+            L.TAPP_AMD_plan_info.argtypes = [_H, _H, ctypes.c_void_p, ctypes.c_void_p, _i64p,
+                                             ctypes.POINTER(ctypes.c_int)]
+
+    # --- objects ---------------------------------------------------------
+    def create_info(self, dtype, extents, strides):
+        h = _H()
+        rc = self.lib.TAPP_create_tensor_info(ctypes.byref(h), int(dtype), len(extents), _arr64(extents),
+                                              _arr64(strides))
+        return rc, h.value
+
+    def destroy_info(self, h):
+        return self.lib.TAPP_destroy_tensor_info(h)
+
+    def create_plan(self, ops, infos, idxs, prec, handle=0):
+        h = _H()
+        args = []
+        for op, info, idx in zip(ops, infos, idxs):
+            args += [int(op), info, _arr64(idx)]
+        rc = self.lib.TAPP_create_tensor_product(ctypes.byref(h), handle, *args, int(prec))
+        return rc, h.value
+
+    def destroy_plan(self, h):
+        return self.lib.TAPP_destroy_tensor_product(h)
+
+    def create_executor(self):
+        h = _H()
+        rc = self.lib.TAPP_create_executor(ctypes.byref(h))
+        return rc, h.value
+
+    def destroy_executor(self, h):
+        return self.lib.TAPP_destroy_executor(h)
+
+    def create_handle(self):
+        h = _H()
+        rc = self.lib.TAPP_create_handle(ctypes.byref(h))
+        return rc, h.value
+
+    def destroy_handle(self, h):
+        return self.lib.TAPP_destroy_handle(h)
+
+    def execute(self, plan, exec_, alpha_ptr, a_ptr, b_ptr, beta_ptr, c_ptr, d_ptr, status_ptr=None):
+        return self.lib.TAPP_execute_product(plan, exec_, status_ptr, alpha_ptr, a_ptr, b_ptr, beta_ptr,
+                                             c_ptr, d_ptr)
+
+    def execute_batched(self, plan, exec_, alpha_ptr, a_ptrs, b_ptrs, beta_ptr, c_ptrs, d_ptrs):
+        n = len(a_ptrs)
+        arr = lambda ps: (ctypes.c_void_p * max(1, n))(*ps)  # noqa: E731
+        c_arr = arr(c_ptrs) if c_ptrs is not None else None
+        return self.lib.TAPP_execute_batched_product(plan, exec_, None, n, alpha_ptr, arr(a_ptrs), arr(b_ptrs),
+                                                     beta_ptr, c_arr, arr(d_ptrs))
+
+    def explain(self, rc):
+        n = self.lib.TAPP_explain_error(rc, 0, None)
+        buf = ctypes.create_string_buffer(n + 1)
+        self.lib.TAPP_explain_error(rc, n + 1, buf)
+        return buf.value.decode()
+
+    # --- extensions ---------------------------------------------------------
+    def attr_set_int(self, exec_, key, value):
+        v = ctypes.c_int(int(value))
+        return self.lib.TAPP_attr_set(exec_, key, ctypes.byref(v))
+
+    def attr_get_int(self, exec_, key):
+        v = ctypes.c_int(0)
+        rc = self.lib.TAPP_attr_get(exec_, key, ctypes.byref(v))
+        return rc, v.value
+
+    def attr_get_double(self, exec_, key):
+        v = ctypes.c_double(0)
+        rc = self.lib.TAPP_attr_get(exec_, key, ctypes.byref(v))
+        return rc, v.value
+
+    def plan_info(self, plan, exec_, beta_ptr, c_ptr):
+        sizes = (ctypes.c_int64 * 6)()
+        ct = ctypes.c_int(-1)
+        rc = self.lib.TAPP_AMD_plan_info(plan, exec_, beta_ptr, c_ptr, sizes, ctypes.byref(ct))
+        return rc, list(sizes), ct.value

@@ -1,0 +1,358 @@
+"""Test-side model of a TAPP product: operands with layouts and data,
+a portable seeded generator, and runners for
+
+  * any library exporting the TAPP API (this package's GPU library, or the
+    compiled reference in oracle/_ref) through tapp_ctypes, and
+  * the oracle (oracle/build/libtapp_oracle.so).
+
+Data generator: SplitMix64 (Steele, Lea, Flood, "Fast splittable
+pseudorandom number generators", OOPSLA 2014); u = (x >> 11) * 2^-53 in
+[0, 1), value = 2u - 1, rounded once to the storage type.
+"""
+import ctypes
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.abspath(os.path.join(os.path.dirname(os.path.abspath(__file__)), ".."))
+PKG = os.path.join(ROOT, "reference-implementation_amd")
+if PKG not in sys.path:
+    sys.path.insert(0, PKG)
+import tapp_ctypes as tc  # noqa: E402
+
+ORACLE_LIB = os.path.join(ROOT, "oracle", "build", "libtapp_oracle.so")
+REF_LIB = os.path.join(ROOT, "oracle", "_ref", "libtapp-reference.so")
+REF_F16_LIB = os.path.join(ROOT, "oracle", "_ref", "libtapp-reference-f16.so")
+
+NP_DTYPE = {tc.F32: np.float32, tc.F64: np.float64, tc.C32: np.complex64, tc.C64: np.complex128,
+            tc.F16: np.float16, tc.BF16: np.uint16}
+REAL_OF = {tc.F32: np.float32, tc.F64: np.float64, tc.C32: np.float32, tc.C64: np.float64,
+           tc.F16: np.float16, tc.BF16: np.float32}
+COMPLEX_TYPES = (tc.C32, tc.C64)
+
+
+# --------------------------------------------------------------------------
+# generator
+# --------------------------------------------------------------------------
+def splitmix64(seed, n):
+    """n successive SplitMix64 outputs from state `seed` (uint64 array)."""
+    with np.errstate(over="ignore"):
+        k = np.arange(1, n + 1, dtype=np.uint64)
+        z = np.uint64(seed) + k * np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return z
+
+
+def uniform(seed, n):
+    """n values in [-1, 1) as float64."""
+    u = (splitmix64(seed, n) >> np.uint64(11)).astype(np.float64) * (2.0 ** -53)
+    return 2.0 * u - 1.0
+
+
+def bf16_round(x):
+    """float32 -> nearest-even bfloat16 value, as float32."""
+    u = np.asarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    r = (u + np.uint64(0x7FFF) + ((u >> np.uint64(16)) & np.uint64(1))) & np.uint64(0xFFFF0000)
+    return r.astype(np.uint32).view(np.float32)
+
+
+def fill(dtype, n, seed):
+    if dtype in COMPLEX_TYPES:
+        v = uniform(seed, 2 * n)
+        return (v[0::2] + 1j * v[1::2]).astype(NP_DTYPE[dtype])
+    v = uniform(seed, n)
+    if dtype == tc.BF16:
+        return (bf16_round(v.astype(np.float32)).view(np.uint32) >> 16).astype(np.uint16)
+    return v.astype(NP_DTYPE[dtype])
+
+
+# --------------------------------------------------------------------------
+# operands
+# --------------------------------------------------------------------------
+class Operand:
+    """A tensor as the API sees it: storage type, layout, labels, element op,
+    and a flat storage buffer with element (0, ..., 0) at buf[offset]."""
+
+    def __init__(self, dtype, extents, strides, idx, op=0, buf=None, offset=0):
+        self.type = dtype
+        self.extents = [int(e) for e in extents]
+        self.strides = [int(s) for s in strides]
+        self.idx = [int(i) for i in idx]
+        self.op = op
+        self.buf = buf
+        self.offset = offset
+
+    @property
+    def nmode(self):
+        return len(self.extents)
+
+    def span(self):
+        lo = hi = 0
+        for e, s in zip(self.extents, self.strides):
+            if e == 0:
+                return 0, -1
+            d = (e - 1) * s
+            if d < 0:
+                lo += d
+            else:
+                hi += d
+        return lo, hi
+
+    def ptr(self):
+        if self.buf is None:
+            return None
+        return self.buf.ctypes.data + self.offset * self.buf.itemsize
+
+    def copy(self):
+        return Operand(self.type, self.extents, self.strides, self.idx, self.op,
+                       None if self.buf is None else self.buf.copy(), self.offset)
+
+
+def labels(s):
+    return [ord(c) for c in s]
+
+
+def layout(extents, order=None, pad=0, negative=(), dense_stride=1):
+    """Strides for `extents` stored with mode `order` (slowest last), each
+    stride padded by `pad` elements, modes in `negative` stored backwards.
+    Returns (strides, offset_of_element0, buffer_length)."""
+    n = len(extents)
+    order = list(range(n)) if order is None else list(order)
+    strides = [0] * n
+    s = dense_stride
+    for m in order:
+        strides[m] = s
+        s = s * max(1, extents[m]) + pad
+    for m in negative:
+        strides[m] = -strides[m]
+    lo = hi = 0
+    for e, st in zip(extents, strides):
+        if e == 0:
+            return strides, 0, 1
+        d = (e - 1) * st
+        if d < 0:
+            lo += d
+        else:
+            hi += d
+    return strides, -lo, hi - lo + 1
+
+
+def make_operand(dtype, extents, idx, seed, order=None, pad=0, negative=(), op=0, strides=None):
+    if strides is None:
+        strides, off, n = layout(extents, order, pad, negative)
+    else:
+        o = Operand(dtype, extents, strides, idx)
+        lo, hi = o.span()
+        off, n = -lo, max(1, hi - lo + 1)
+    buf = fill(dtype, n, seed)
+    return Operand(dtype, extents, strides, idx if not isinstance(idx, str) else labels(idx), op, buf, off)
+
+
+class Case:
+    def __init__(self, A, B, C, D, prec=tc.DEFAULT_PREC, alpha=1.0, beta=0.0, c_null=False, name=""):
+        self.A, self.B, self.C, self.D = A, B, C, D
+        self.prec = prec
+        self.alpha, self.beta = alpha, beta
+        self.c_null = c_null
+        self.name = name
+
+    def operands(self):
+        return [self.A, self.B, self.C, self.D]
+
+    def copy(self):
+        return Case(self.A.copy(), self.B.copy(), self.C.copy(), self.D.copy(), self.prec, self.alpha, self.beta,
+                    self.c_null, self.name)
+
+
+def scalar_bytes(value, dtype):
+    """The scalar as the library reads it: stored as type_D."""
+    if dtype in COMPLEX_TYPES:
+        arr = np.array([complex(value)], dtype=NP_DTYPE[dtype])
+    elif dtype == tc.BF16:
+        arr = (bf16_round(np.array([value], dtype=np.float32)).view(np.uint32) >> 16).astype(np.uint16)
+    else:
+        arr = np.array([complex(value).real], dtype=NP_DTYPE[dtype])
+    return arr
+
+
+# --------------------------------------------------------------------------
+# runners
+# --------------------------------------------------------------------------
+def run_lib(lib, case, exec_=None, own_exec=True):
+    """Runs the case through a TAPP library with host pointers.  Returns
+    (rc, D buffer after the call)."""
+    case = case.copy()
+    infos = []
+    for o in case.operands():
+        rc, h = lib.create_info(o.type, o.extents, o.strides)
+        assert rc == 0, rc
+        infos.append(h)
+    rc, plan = lib.create_plan([o.op for o in case.operands()], infos, [o.idx for o in case.operands()], case.prec)
+    assert rc == 0
+    made = False
+    if exec_ is None:
+        rc, exec_ = lib.create_executor()
+        made = True
+    al = scalar_bytes(case.alpha, case.D.type)
+    be = scalar_bytes(case.beta, case.D.type)
+    c_ptr = None if case.c_null else case.C.ptr()
+    rc = lib.execute(plan, exec_, al.ctypes.data, case.A.ptr(), case.B.ptr(), be.ctypes.data, c_ptr, case.D.ptr())
+    if made:
+        lib.destroy_executor(exec_)
+    lib.destroy_plan(plan)
+    for h in infos:
+        lib.destroy_info(h)
+    return rc, case.D.buf
+
+
+class _OrcTensor(ctypes.Structure):
+    _fields_ = [("type", ctypes.c_int), ("nmode", ctypes.c_int), ("extents", ctypes.POINTER(ctypes.c_int64)),
+                ("strides", ctypes.POINTER(ctypes.c_int64)), ("idx", ctypes.POINTER(ctypes.c_int64)),
+                ("op", ctypes.c_int)]
+
+
+class _OrcClasses(ctypes.Structure):
+    _fields_ = [("n_contracted", ctypes.c_int), ("n_unary_A", ctypes.c_int), ("n_unary_B", ctypes.c_int),
+                ("nmode_D", ctypes.c_int)] + [
+        (name, ctypes.c_int64 * 64) for name in (
+            "contracted", "unary_A", "unary_B", "ext_contracted", "free_strides_A", "free_strides_B",
+            "contracted_strides_A", "contracted_strides_B", "unary_strides_A", "unary_strides_B",
+            "strides_D", "strides_C", "extents_D")]
+
+
+class Oracle:
+    def __init__(self, path=ORACLE_LIB):
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"{path} not built: run `make -C oracle`")
+        self.lib = ctypes.CDLL(path, mode=os.RTLD_LOCAL | os.RTLD_NOW)
+        P = ctypes.POINTER(_OrcTensor)
+        self.lib.orc_execute.argtypes = [P, P, P, P, ctypes.c_int] + [ctypes.c_void_p] * 6 + [
+            ctypes.POINTER(ctypes.c_int)]
+        self.lib.orc_execute_f64.argtypes = [P, P, P, P, ctypes.c_int] + [ctypes.c_void_p] * 6 + [ctypes.c_int]
+        self.lib.orc_check.argtypes = [P, P, P, P, ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]
+        self.lib.orc_classify.argtypes = [P, P, P, P, ctypes.POINTER(_OrcClasses)]
+        self.lib.orc_round_bf16.argtypes = [ctypes.c_float]
+        self.lib.orc_round_bf16.restype = ctypes.c_float
+        self.lib.orc_round_f16.argtypes = [ctypes.c_float]
+        self.lib.orc_round_f16.restype = ctypes.c_float
+
+    @staticmethod
+    def _t(o, keep):
+        e = (ctypes.c_int64 * max(1, o.nmode))(*o.extents)
+        s = (ctypes.c_int64 * max(1, o.nmode))(*o.strides)
+        i = (ctypes.c_int64 * max(1, o.nmode))(*o.idx)
+        keep += [e, s, i]
+        return _OrcTensor(o.type, o.nmode, e, s, i, o.op)
+
+    def tensors(self, case, keep):
+        return [ctypes.byref(self._t(o, keep)) for o in case.operands()]
+
+    def run(self, case, exec_kind=1, f64=False, round_inputs=0):
+        case = case.copy()
+        keep = []
+        ts = self.tensors(case, keep)
+        al = scalar_bytes(case.alpha, case.D.type)
+        be = scalar_bytes(case.beta, case.D.type)
+        c_ptr = None if case.c_null else case.C.ptr()
+        if f64:
+            rc = self.lib.orc_execute_f64(*ts, case.prec, al.ctypes.data, case.A.ptr(), case.B.ptr(), be.ctypes.data,
+                                          c_ptr, case.D.ptr(), round_inputs)
+        else:
+            kind = None if exec_kind is None else ctypes.byref(ctypes.c_int(exec_kind))
+            rc = self.lib.orc_execute(*ts, case.prec, al.ctypes.data, case.A.ptr(), case.B.ptr(), be.ctypes.data,
+                                      c_ptr, case.D.ptr(), kind)
+        return rc, case.D.buf
+
+    def check(self, case, exec_kind=1):
+        keep = []
+        ts = self.tensors(case, keep)
+        be = scalar_bytes(case.beta, case.D.type)
+        c_ptr = None if case.c_null else case.C.ptr()
+        kind = None if exec_kind is None else ctypes.byref(ctypes.c_int(exec_kind))
+        return self.lib.orc_check(*ts, be.ctypes.data, c_ptr, kind)
+
+    def classify(self, case):
+        keep = []
+        out = _OrcClasses()
+        rc = self.lib.orc_classify(*self.tensors(case, keep), ctypes.byref(out))
+        return rc, out
+
+
+# --------------------------------------------------------------------------
+# comparisons
+# --------------------------------------------------------------------------
+def written_mask(D):
+    """Boolean mask over D.buf of the elements the product writes (the
+    compressed diagonal of repeated labels counts once)."""
+    mask = np.zeros(len(D.buf), dtype=bool)
+    # unique labels: first occurrence keeps its extent; stride summed
+    seen = {}
+    for e, s, l in zip(D.extents, D.strides, D.idx):
+        if l in seen:
+            seen[l][1] += s
+        else:
+            seen[l] = [e, s]
+    off = np.zeros(1, dtype=np.int64)
+    for e, s in seen.values():
+        if e == 0:
+            return mask
+        off = (off[:, None] + np.arange(e, dtype=np.int64)[None, :] * s).reshape(-1)
+    mask[off + D.offset] = True
+    return mask
+
+
+def normwise_error(got, ref, mask=None):
+    g = np.asarray(got)
+    r = np.asarray(ref)
+    if mask is not None:
+        g, r = g[mask], r[mask]
+    if g.dtype == np.uint16:
+        g = (g.astype(np.uint32) << 16).view(np.float32)
+        r = (r.astype(np.uint32) << 16).view(np.float32)
+    g = g.astype(np.complex128)
+    r = r.astype(np.complex128)
+    if r.size == 0:
+        return 0.0
+    den = np.max(np.abs(r))
+    num = np.max(np.abs(g - r))
+    if den == 0:
+        return float(num)
+    return float(num / den)
+
+
+# --------------------------------------------------------------------------
+# the benchmark configurations as cases
+# --------------------------------------------------------------------------
+def configs0_case(seed=1, dtype=tc.F32):
+    """D[a,b,c,d] = 1.5 sum_{e,f} A[a,e,b,f] B[d,f,c,e] + 0.5 C[a,b,c,d],
+    all extents 16, dense column-major, values uniform [-1, 1)."""
+    e = [16] * 4
+    A = make_operand(dtype, e, "aebf", seed * 4 + 0)
+    B = make_operand(dtype, e, "dfce", seed * 4 + 1)
+    C = make_operand(dtype, e, "abcd", seed * 4 + 2)
+    D = make_operand(dtype, e, "abcd", seed * 4 + 3)
+    return Case(A, B, C, D, tc.DEFAULT_PREC, 1.5, 0.5, name="configs0")
+
+
+def configs1_case(seed=1, scale=1):
+    """Complex double feature coverage:
+    D[a,b,h,i] = alpha sum_{c,u,v} A[a,c,c,h,u] conj(B[h,b,c,i,v]) + beta C[a,b,h,i]
+    h=32 Hadamard, c=24 contracted and repeated in A, a=40, b=40, i=32 free,
+    u=8 unary in A, v=6 unary in B; A stored with a negative stride on a, B
+    with padded strides.  `scale` divides every extent (for small tests)."""
+    a, b, c, h, i, u, v = [max(1, x // scale) for x in (40, 40, 24, 32, 32, 8, 6)]
+    A = make_operand(tc.C64, [a, c, c, h, u], "acchu", seed * 4 + 0, negative=(0,))
+    B = make_operand(tc.C64, [h, b, c, i, v], "hbciv", seed * 4 + 1, pad=1, op=tc.CONJUGATE)
+    C = make_operand(tc.C64, [a, b, h, i], "abhi", seed * 4 + 2)
+    D = make_operand(tc.C64, [a, b, h, i], "abhi", seed * 4 + 3)
+    return Case(A, B, C, D, tc.F64F64_ACCUM_F64, 0.7 - 0.3j, 1.1 + 0.2j, name="configs1")
+
+
+def run_reference(path, case):
+    lib = tc.TappLib(path)
+    rc, d = run_lib(lib, case)
+    assert rc == 0, rc
+    return d
