@@ -409,6 +409,8 @@ static void store_z(void* base, int type, int64_t off, double complex v)
  *   for each contracted coordinate (first fastest):  (:361-395)
  *     sum_A = 0; sum_A += op_A(A) over unary A      (:370-380)
  *     sum_B = 0; sum_B += op_B(B) over unary B      (:381-391)
+ *       (the addend keeps its storage precision: C promotes the sum, adds
+ *       and converts back, e.g. float sum += double value, :1903)
  *     accum += alpha * sum_A * sum_B                (:393, left to right)
  *   accum = op_D(accum); D = accum                  (:396-397)        */
 #define ORC_LOOP(NAME, T, CONV)                                                        \
@@ -439,13 +441,13 @@ static void NAME(const plan* p, const semantics* s)                             
             for (int64_t u = 0; u < p->size_ua; u++) {                                 \
                 int64_t x = ka;                                                        \
                 for (int l = 0; l < p->nua; l++) x += cua[l] * p->ua_s[l];             \
-                sa += CONV(opz(rnd(load_z(s->A, s->tA, x), s->round), s->op_A));       \
+                sa = CONV((double complex)sa + opz(rnd(load_z(s->A, s->tA, x), s->round), s->op_A)); \
                 odometer(cua, p->nua, p->ua_ext);                                      \
             }                                                                          \
             for (int64_t u = 0; u < p->size_ub; u++) {                                 \
                 int64_t x = kb;                                                        \
                 for (int l = 0; l < p->nub; l++) x += cub[l] * p->ub_s[l];             \
-                sb += CONV(opz(rnd(load_z(s->B, s->tB, x), s->round), s->op_B));       \
+                sb = CONV((double complex)sb + opz(rnd(load_z(s->B, s->tB, x), s->round), s->op_B)); \
                 odometer(cub, p->nub, p->ub_ext);                                      \
             }                                                                          \
             if (s->round) { sa = CONV(rnd(sa, s->round)); sb = CONV(rnd(sb, s->round)); } \

@@ -356,3 +356,87 @@ def run_reference(path, case):
     rc, d = run_lib(lib, case)
     assert rc == 0, rc
     return d
+
+
+# --------------------------------------------------------------------------
+# random scenarios (the scenario classes of the reference's test.cpp:27-56)
+# --------------------------------------------------------------------------
+SCENARIOS = {
+    # name: (free_A, free_B, hadamard, contracted, unary_A, unary_B, repeated_A, repeated_B)
+    "contraction": (2, 2, 0, 2, 0, 0, 0, 0),
+    "hadamard": (0, 0, 2, 0, 0, 0, 0, 0),
+    "outer_product": (2, 2, 0, 0, 0, 0, 0, 0),
+    "full_contraction": (0, 0, 0, 3, 0, 0, 0, 0),
+    "zero_dim_A": (0, 2, 0, 0, 0, 0, 0, 0),
+    "one_dim": (1, 0, 0, 1, 0, 0, 0, 0),
+    "hadamard_free": (1, 1, 1, 0, 0, 0, 0, 0),
+    "hadamard_contraction": (1, 1, 1, 1, 0, 0, 0, 0),
+    "unary": (1, 1, 0, 1, 1, 1, 0, 0),
+    "repeated": (1, 1, 0, 1, 0, 0, 1, 1),
+    "everything": (1, 1, 1, 1, 1, 1, 1, 1),
+}
+
+
+def random_case(seed, scenario="contraction", dtype=tc.F32, max_ext=4, prec=tc.DEFAULT_PREC, layouts=True,
+                repeated_D=False, beta_zero=None, min_ext=1):
+    rng = np.random.default_rng(seed)
+    fa, fb, ha, co, ua, ub, ra, rb = SCENARIOS[scenario]
+    next_label = [100]
+
+    def new(n):
+        out = list(range(next_label[0], next_label[0] + n))
+        next_label[0] += n
+        return out
+
+    FA, FB, HA, CO, UA, UB = new(fa), new(fb), new(ha), new(co), new(ua), new(ub)
+    ext = {l: int(rng.integers(min_ext, max_ext + 1)) for l in FA + FB + HA + CO + UA + UB}
+    A_lab = FA + HA + CO + UA
+    B_lab = FB + HA + CO + UB
+    # repeated labels: duplicate some labels that already occur (diagonals)
+    for _ in range(ra):
+        if A_lab:
+            A_lab.append(A_lab[int(rng.integers(len(A_lab)))])
+    for _ in range(rb):
+        if B_lab:
+            B_lab.append(B_lab[int(rng.integers(len(B_lab)))])
+    D_lab = FA + FB + HA
+    if repeated_D and D_lab:
+        D_lab.append(D_lab[int(rng.integers(len(D_lab)))])
+    A_lab = list(rng.permutation(A_lab)) if A_lab else []
+    B_lab = list(rng.permutation(B_lab)) if B_lab else []
+    D_lab = list(rng.permutation(D_lab)) if D_lab else []
+
+    def op_for(dt):
+        return int(rng.integers(0, 2)) if dt in COMPLEX_TYPES else 0
+
+    def mk(labs, s, out=False):
+        e = [ext[l] for l in labs]
+        if layouts and not out:
+            order = list(rng.permutation(len(e))) if e else None
+            pad = int(rng.choice([0, 0, 1, 3]))
+            neg = tuple(i for i in range(len(e)) if rng.random() < 0.25)
+        elif layouts and out:
+            order = list(rng.permutation(len(e))) if e else None
+            pad = int(rng.choice([0, 1]))
+            neg = tuple(i for i in range(len(e)) if rng.random() < 0.25)
+        else:
+            order, pad, neg = None, 0, ()
+        return make_operand(dtype, e, labs, s, order=order, pad=pad, negative=neg, op=op_for(dtype))
+
+    A = mk(A_lab, seed * 8 + 1)
+    B = mk(B_lab, seed * 8 + 2)
+    C = mk(D_lab, seed * 8 + 3)
+    D = mk(D_lab, seed * 8 + 4, out=True)
+    if dtype in COMPLEX_TYPES:
+        alpha = complex(*uniform(seed * 8 + 5, 2))
+        beta = complex(*uniform(seed * 8 + 6, 2))
+    else:
+        alpha = float(uniform(seed * 8 + 5, 1)[0])
+        beta = float(uniform(seed * 8 + 6, 1)[0])
+    if beta_zero is None:
+        beta_zero = rng.random() < 0.25
+    c_null = False
+    if beta_zero:
+        beta = 0.0
+        c_null = bool(rng.random() < 0.5)
+    return Case(A, B, C, D, prec, alpha, beta, c_null, name=f"{scenario}/{seed}")

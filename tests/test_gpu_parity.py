@@ -1,0 +1,296 @@
+"""GPU parity: the library's TAPP_execute_product on an MI355X against the
+oracle (itself pinned bit-exact to the compiled reference) on the same
+seeded inputs.
+
+Tolerances (normwise, max|D - D_ref| / max|D_ref| over the written
+elements): FP32/C32 1e-5, FP64/C64 1e-12.  Elements of D the product does
+not write must be bit-identical to their input values.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+import tapp_cases as T
+
+tc = T.tc
+pytestmark = pytest.mark.gpu
+
+TOL = {tc.F32: 1e-5, tc.C32: 1e-5, tc.F64: 1e-12, tc.C64: 1e-12, tc.F16: 2e-3, tc.BF16: 1.6e-2}
+
+
+def tol_for(case):
+    t = TOL[case.D.type]
+    if case.prec in (tc.F32F32_ACCUM_F32,):
+        t = max(t, 1e-5)
+    if case.prec == tc.F64F64_ACCUM_F64 and case.D.type in (tc.F32, tc.C32):
+        t = 1e-6   # result stored in binary32
+    return t
+
+
+def assert_parity(got, want, case, tol=None):
+    mask = T.written_mask(case.D)
+    tol = tol_for(case) if tol is None else tol
+    err = T.normwise_error(got, want, mask)
+    assert err <= tol, f"{case.name}: normwise error {err:.3e} > {tol:.1e}"
+    untouched = ~mask
+    assert np.array_equal(np.asarray(got).view(np.uint8).reshape(len(mask), -1)[untouched],
+                          case.D.buf.view(np.uint8).reshape(len(mask), -1)[untouched]), f"{case.name}: wrote outside D"
+    return err
+
+
+def run_both(lib, ex, oracle, case, f64_oracle=False):
+    rc_o, d_o = oracle.run(case, f64=f64_oracle)
+    rc_g, d_g = T.run_lib(lib, case, ex)
+    assert rc_g == rc_o, (case.name, rc_g, rc_o, lib.explain(rc_g))
+    return rc_o, d_g, d_o
+
+
+def run_device(lib, ex, case):
+    """Same call with every operand in device memory (torch allocations)."""
+    import torch
+    case = case.copy()
+    dev = {}
+    ptrs = {}
+    for name, o in zip("ABCD", case.operands()):
+        t = torch.from_numpy(o.buf.view(np.uint8).copy()).cuda()
+        dev[name] = t
+        ptrs[name] = t.data_ptr() + o.offset * o.buf.itemsize
+    al = T.scalar_bytes(case.alpha, case.D.type)
+    be = T.scalar_bytes(case.beta, case.D.type)
+    infos = [lib.create_info(o.type, o.extents, o.strides)[1] for o in case.operands()]
+    _, plan = lib.create_plan([o.op for o in case.operands()], infos, [o.idx for o in case.operands()], case.prec)
+    torch.cuda.synchronize()
+    rc = lib.execute(plan, ex, al.ctypes.data, ptrs["A"], ptrs["B"], be.ctypes.data,
+                     None if case.c_null else ptrs["C"], ptrs["D"])
+    out = dev["D"].cpu().numpy().view(case.D.buf.dtype)
+    lib.destroy_plan(plan)
+    for h in infos:
+        lib.destroy_info(h)
+    return rc, out
+
+
+@pytest.mark.parametrize("dtype", [tc.F32, tc.F64, tc.C32, tc.C64])
+@pytest.mark.parametrize("prec", [tc.DEFAULT_PREC, tc.F32F32_ACCUM_F32, tc.F64F64_ACCUM_F64])
+@pytest.mark.parametrize("scenario", sorted(T.SCENARIOS))
+def test_random_scenarios(lib, gpu_exec, oracle, dtype, prec, scenario):
+    for seed in range(2):
+        c = T.random_case(2000 + seed * 31, scenario, dtype, prec=prec, max_ext=5, repeated_D=(seed == 1))
+        rc, got, want = run_both(lib, gpu_exec, oracle, c)
+        if rc == 0:
+            assert_parity(got, want, c)
+
+
+def test_reference_program_known_answers(lib, gpu_exec):
+    from test_oracle import _case_from_kat, load_kats
+    for k in load_kats():
+        c = _case_from_kat(k)
+        rc, got = T.run_lib(lib, c, gpu_exec)
+        assert rc == k["rc"]
+        want = np.array(k["D_out"]["data"], dtype=np.float64)
+        g = np.asarray(got)
+        if g.dtype.kind == "c":
+            g = np.stack([g.real, g.imag], axis=-1).reshape(-1)
+        assert np.allclose(g.astype(np.float64), want, rtol=1e-6, atol=1e-5), (k["program"], k["call"])
+
+
+def test_configs0_against_reference_golden(lib, gpu_exec):
+    import os
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "ref_configs0.npz"))
+    c = T.configs0_case(seed=int(g["seed"]))
+    rc, got = T.run_lib(lib, c, gpu_exec)
+    assert rc == 0
+    assert_parity(got, g["D"], c)
+
+
+def test_configs1_full_size(lib, gpu_exec, oracle):
+    c = T.configs1_case(seed=1)
+    rc, got, want = run_both(lib, gpu_exec, oracle, c)
+    assert rc == 0
+    assert_parity(got, want, c)
+
+
+@pytest.mark.parametrize("scenario", ["contraction", "everything", "hadamard_contraction", "unary"])
+@pytest.mark.parametrize("dtype", [tc.F32, tc.C64])
+def test_device_pointers_equal_host_pointers(lib, gpu_exec, oracle, scenario, dtype):
+    c = T.random_case(77, scenario, dtype, max_ext=6)
+    rc_h, d_h, want = run_both(lib, gpu_exec, oracle, c)
+    rc_d, d_d = run_device(lib, gpu_exec, c)
+    assert rc_d == rc_h
+    if rc_h == 0:
+        assert_parity(d_d, want, c)
+
+
+def _gemm_case(dtype, M, N, K, seed, transA=False, transB=False, ldpad=0, conj=(0, 0), beta=0.5):
+    a_order = [1, 0] if transA else None
+    b_order = [1, 0] if transB else None
+    A = T.make_operand(dtype, [M, K], "mk", seed, order=a_order, pad=ldpad, op=conj[0])
+    B = T.make_operand(dtype, [K, N], "kn", seed + 1, order=b_order, pad=ldpad, op=conj[1])
+    C = T.make_operand(dtype, [M, N], "mn", seed + 2)
+    D = T.make_operand(dtype, [M, N], "mn", seed + 3)
+    alpha = 0.75 - 0.25j if dtype in T.COMPLEX_TYPES else 0.75
+    return T.Case(A, B, C, D, tc.DEFAULT_PREC, alpha, beta, name=f"gemm{M}x{N}x{K}")
+
+
+@pytest.mark.parametrize("dtype", [tc.F32, tc.F64, tc.C32, tc.C64])
+@pytest.mark.parametrize("kernel", [tc.KERNEL_SIMPLE, tc.KERNEL_MFMA, tc.KERNEL_SPLITK])
+def test_forced_kernels_on_gemm_shapes(lib, oracle, dtype, kernel):
+    _, ex = lib.create_executor()
+    assert lib.attr_set_int(ex, tc.ATTR_KERNEL, kernel) == 0
+    shapes = [(1, 1, 1), (17, 33, 5), (64, 64, 64), (129, 70, 37), (200, 130, 257)]
+    for i, (M, N, K) in enumerate(shapes):
+        if kernel == tc.KERNEL_SPLITK and M * N > 4096:
+            continue
+        for tA, tB in ((False, False), (True, True), (True, False)):
+            c = _gemm_case(dtype, M, N, K, 10 * i + 1, tA, tB, ldpad=i % 2,
+                           conj=(1, 0) if dtype in T.COMPLEX_TYPES else (0, 0))
+            rc, got, want = run_both(lib, ex, oracle, c)
+            assert rc == 0
+            assert_parity(got, want, c)
+            assert lib.attr_get_int(ex, tc.ATTR_LAST_KERNEL)[1] == kernel
+    lib.destroy_executor(ex)
+
+
+def test_mixed_storage_types(lib, gpu_exec, oracle):
+    combos = [(tc.F32, tc.F64, tc.F32, tc.F64), (tc.F32, tc.C32, tc.C32, tc.C32), (tc.C64, tc.F64, tc.C64, tc.C32),
+              (tc.F64, tc.F64, tc.F32, tc.F32), (tc.F16, tc.F16, tc.F32, tc.F32), (tc.BF16, tc.F32, tc.F32, tc.BF16)]
+    for i, (ta, tb, tcc, td) in enumerate(combos):
+        c = T.random_case(900 + i, "everything", tc.F32, max_ext=4, beta_zero=False)
+        for o, t in zip(c.operands(), (ta, tb, tcc, td)):
+            lo, hi = o.span()
+            o.type = t
+            o.buf = T.fill(t, max(1, hi - lo + 1), 900 + i + t)
+            o.op = 1 if t in T.COMPLEX_TYPES else 0
+        c.alpha, c.beta = (0.5 + 0.25j, -0.75 + 0.5j) if td in T.COMPLEX_TYPES else (0.5, -0.75)
+        rc, got, want = run_both(lib, gpu_exec, oracle, c)
+        assert rc == 0
+        tol = TOL[td] if td in (tc.F16, tc.BF16) else (1e-5 if td in (tc.F32, tc.C32) else 1e-6)
+        assert_parity(got, want, c, tol=tol)
+
+
+@pytest.mark.parametrize("prec", [tc.F16F16_ACCUM_F32, tc.BF16BF16_ACCUM_F32, tc.F16F16_ACCUM_F16])
+def test_low_precision_forms(lib, gpu_exec, oracle, prec):
+    for seed in range(3):
+        c = T.random_case(60 + seed, "contraction", tc.F32, prec=prec, max_ext=8, min_ext=4)
+        rc, got, want = run_both(lib, gpu_exec, oracle, c)
+        assert rc == 0
+        # same input rounding as the oracle; only the FP32 summation order differs
+        assert_parity(got, want, c, tol=1e-5)
+        # against FP64 on the unrounded inputs: 2 u_in + gamma_K(u_f32)
+        rc64, exact = oracle.run(c, f64=True)
+        K = int(np.prod([e for e, l in zip(c.A.extents, c.A.idx) if l in c.B.idx and l not in c.D.idx]))
+        u_in = 2.0 ** -11 if prec != tc.BF16BF16_ACCUM_F32 else 2.0 ** -8
+        u32 = 2.0 ** -24
+        bound = 2 * u_in + K * u32 / (1 - K * u32)
+        mask = T.written_mask(c.D)
+        # normwise with the magnitude sum as denominator scale
+        assert T.normwise_error(got, exact, mask) <= 8 * bound
+
+
+def test_split_bf16_accuracy(lib, oracle):
+    _, ex = lib.create_executor()
+    assert lib.attr_set_int(ex, tc.ATTR_SPLIT_BF16, 1) == 0
+    for seed in range(3):
+        c = _gemm_case(tc.F32, 96, 80, 200, 700 + seed, beta=0.0)
+        rc, got = T.run_lib(lib, c, ex)
+        _, exact = oracle.run(c, f64=True)
+        assert rc == 0
+        err = T.normwise_error(got, exact, T.written_mask(c.D))
+        assert err < 1e-5, err
+    lib.destroy_executor(ex)
+
+
+def test_batched_product_equals_single_calls(lib, gpu_exec, oracle):
+    cases = [T.random_case(5, "hadamard_contraction", tc.F64, max_ext=5, beta_zero=False, layouts=False)]
+    base = cases[0]
+    for s in range(1, 3):
+        c = base.copy()
+        for o in c.operands():
+            o.buf = T.fill(o.type, len(o.buf), 5000 + s * 7 + o.type)
+        cases.append(c)
+    bufs = [c.copy() for c in cases]
+    infos = [lib.create_info(o.type, o.extents, o.strides)[1] for o in base.operands()]
+    _, plan = lib.create_plan([o.op for o in base.operands()], infos, [o.idx for o in base.operands()], base.prec)
+    al = T.scalar_bytes(base.alpha, base.D.type)
+    be = T.scalar_bytes(base.beta, base.D.type)
+    rc = lib.execute_batched(plan, gpu_exec, al.ctypes.data, [b.A.ptr() for b in bufs], [b.B.ptr() for b in bufs],
+                             be.ctypes.data, [b.C.ptr() for b in bufs], [b.D.ptr() for b in bufs])
+    assert rc == 0
+    for b, c in zip(bufs, cases):
+        _, want = oracle.run(c)
+        assert_parity(b.D.buf, want, c)
+    lib.destroy_plan(plan)
+    for h in infos:
+        lib.destroy_info(h)
+
+
+def test_zero_extents_and_pinned_output(lib, gpu_exec, oracle):
+    # contracted extent 0: D = beta * C
+    c = _gemm_case(tc.F32, 5, 4, 0, 3)
+    rc, got, want = run_both(lib, gpu_exec, oracle, c)
+    assert rc == 0
+    assert_parity(got, want, c)
+    # D extent 0: nothing written
+    c = _gemm_case(tc.F32, 0, 4, 3, 4)
+    rc, got, want = run_both(lib, gpu_exec, oracle, c)
+    assert rc == 0 and np.array_equal(got, c.D.buf)
+    # one-mode D with stride 0: the last coordinate's value remains
+    A = T.make_operand(tc.F64, [6, 3], "ak", 1)
+    B = T.make_operand(tc.F64, [3], "k", 2)
+    C = T.Operand(tc.F64, [6], [0], T.labels("a"), 0, T.fill(tc.F64, 1, 3), 0)
+    D = T.Operand(tc.F64, [6], [0], T.labels("a"), 0, T.fill(tc.F64, 1, 4), 0)
+    c = T.Case(A, B, C, D, tc.DEFAULT_PREC, 1.5, 0.25, name="pinned")
+    rc, got, want = run_both(lib, gpu_exec, oracle, c)
+    assert rc == 0
+    assert np.allclose(got, want, rtol=1e-12, atol=0)
+    # unary extent 0: every product term is zero
+    A = T.make_operand(tc.F32, [4, 3, 0], "akx", 5)
+    B = T.make_operand(tc.F32, [3, 2], "kn", 6)
+    C = T.make_operand(tc.F32, [4, 2], "an", 7)
+    D = T.make_operand(tc.F32, [4, 2], "an", 8)
+    c = T.Case(A, B, C, D, tc.DEFAULT_PREC, 1.0, 2.0, name="unary0")
+    rc, got, want = run_both(lib, gpu_exec, oracle, c)
+    assert rc == 0
+    assert_parity(got, want, c)
+
+
+def test_layout_change_after_plan_creation(lib, gpu_exec, oracle):
+    """Layouts are held by handle: TAPP_set_strides after plan creation
+    changes the next execute (reference product.c:86-107)."""
+    c = _gemm_case(tc.F32, 40, 30, 20, 9)
+    infos = [lib.create_info(o.type, o.extents, o.strides)[1] for o in c.operands()]
+    _, plan = lib.create_plan([o.op for o in c.operands()], infos, [o.idx for o in c.operands()], c.prec)
+    al = T.scalar_bytes(c.alpha, c.D.type)
+    be = T.scalar_bytes(c.beta, c.D.type)
+    for transposed in (False, True):
+        cc = c.copy()
+        if transposed:
+            # store A transposed in the same buffer and tell the plan through its info
+            M, K = c.A.extents
+            a2 = c.A.buf[:M * K].reshape(K, M).T.copy().reshape(-1)
+            cc.A.buf = a2.astype(np.float32)
+            cc.A.strides = [K, 1]
+            s = (ctypes.c_int64 * 2)(K, 1)
+            assert lib.lib.TAPP_set_strides(infos[0], s) == 0
+        rc = lib.execute(plan, gpu_exec, al.ctypes.data, cc.A.ptr(), cc.B.ptr(), be.ctypes.data, cc.C.ptr(),
+                         cc.D.ptr())
+        assert rc == 0
+        _, want = oracle.run(cc)
+        assert_parity(cc.D.buf, want, cc)
+    lib.destroy_plan(plan)
+    for h in infos:
+        lib.destroy_info(h)
+
+
+def test_reference_style_executor_int(lib, oracle):
+    """A caller's bare int executor (the reference's representation,
+    executor.c:9-18) runs on the default device context; other ints are 33."""
+    c = T.random_case(3, "contraction", tc.F32)
+    kind = ctypes.c_int(1)
+    rc, got = T.run_lib(lib, c, ctypes.addressof(kind))
+    _, want = oracle.run(c)
+    assert rc == 0
+    assert_parity(got, want, c)
+    bad = ctypes.c_int(5)
+    rc, _ = T.run_lib(lib, c, ctypes.addressof(bad))
+    assert rc == 33
