@@ -57,6 +57,10 @@ enum {
     TAPP_AMD_KERNEL_SPLITK = 3
 };
 
+/* Text of the last device error of the calling thread (which HIP call
+ * failed and why), with TAPP_explain_error's maxlen contract. */
+TAPP_EXPORT size_t TAPP_AMD_last_error_message(char* message, size_t maxlen);
+
 /* Version string of this library. */
 TAPP_EXPORT const char* TAPP_AMD_version(void);
 

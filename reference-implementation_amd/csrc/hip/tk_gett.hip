@@ -138,7 +138,10 @@ template <> struct Mfma<double, 16> {
     {
         return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
     }
-    static __device__ __forceinline__ int row(int r, int lane) { return 4 * (lane >> 4) + r; }
+    // FP64 accumulators are spread with stride 4 over the rows: register r
+    // of lane l holds row l / 16 + 4 r (the FP32 16x16x4 form keeps 4
+    // consecutive rows per lane instead)
+    static __device__ __forceinline__ int row(int r, int lane) { return (lane >> 4) + 4 * r; }
 };
 
 // ---------------------------------------------------------------------------
@@ -330,6 +333,163 @@ __global__ __launch_bounds__(256) void k_mfma(tk_gett g)
 }
 
 // ---------------------------------------------------------------------------
+// k_mfma16: FP32 data through 16-bit MFMA (gfx950 v_mfma_f32_32x32x16_*)
+//   TK_MODE_BF16       x -> bf16(x)
+//   TK_MODE_F16        x -> f16(x)
+//   TK_MODE_SPLIT_BF16 x -> hi = bf16(x), lo = bf16(x - hi);
+//                      acc += hi*hi + hi*lo + lo*hi (lo*lo dropped)
+// Conversion happens once per element when a tile is written to LDS.  LDS
+// holds each tile as [row][k] so a lane reads its 8 consecutive k values of
+// one row with a single 16-byte LDS read, which is the MFMA operand layout
+// (lane l: row l % 32, k = 8 * (l / 32) .. +7).  Accumulation is FP32.
+// ---------------------------------------------------------------------------
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ uint16_t bf16_bits(float x) { return float_to_bf16_bits(x); }
+__device__ __forceinline__ uint16_t f16_bits(float x)
+{
+    _Float16 h = (_Float16)x;
+    return __builtin_bit_cast(uint16_t, h);
+}
+
+template <int MODE>
+__device__ __forceinline__ f32x16 mfma16(uint4 b, uint4 a, f32x16 c)
+{
+    if (MODE == TK_MODE_F16)
+        return __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(f16x8, b), __builtin_bit_cast(f16x8, a), c,
+                                                      0, 0, 0);
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, b), __builtin_bit_cast(bf16x8, a), c,
+                                                   0, 0, 0);
+}
+
+template <int MODE, int BR, int BK, int LDK>
+__device__ __forceinline__ void store16(const TileLoader<float, false, BR, BK>& t, uint16_t* hi, uint16_t* lo,
+                                        int mode)
+{
+    typedef TileLoader<float, false, BR, BK> TL;
+#pragma unroll
+    for (int i = 0; i < TL::NS; ++i) {
+        const int s = threadIdx.x + 256 * i;
+        int rr, kk;
+        if (mode == TK_CONTIG_K) { kk = (s % (BK / TL::VE)) * TL::VE; rr = s / (BK / TL::VE); }
+        else { rr = (s % (BR / TL::VE)) * TL::VE; kk = s / (BR / TL::VE); }
+#pragma unroll
+        for (int j = 0; j < TL::VE; ++j) {
+            const int r = mode == TK_CONTIG_K ? rr : rr + j;
+            const int k = mode == TK_CONTIG_K ? kk + j : kk;
+            const float x = t.v[i][j];
+            if (MODE == TK_MODE_F16) {
+                hi[r * LDK + k] = f16_bits(x);
+            } else {
+                const uint16_t h = bf16_bits(x);
+                hi[r * LDK + k] = h;
+                if (MODE == TK_MODE_SPLIT_BF16) lo[r * LDK + k] = bf16_bits(x - bf16_bits_to_float(h));
+            }
+        }
+    }
+}
+
+template <int MODE, int FM, int FN, int BK>
+__global__ __launch_bounds__(256) void k_mfma16(tk_gett g)
+{
+    constexpr int S = 32;
+    constexpr int BM = 2 * FM * S, BN = 2 * FN * S;
+    constexpr int PL = MODE == TK_MODE_SPLIT_BF16 ? 2 : 1;
+    constexpr int LDK = BK + 8;                 // row pitch in 16-bit words (16-byte pad)
+    static_assert(BK % 16 == 0, "BK must hold whole 32x32x16 steps");
+    __shared__ __attribute__((aligned(16))) uint16_t As[2][PL][BM * LDK];
+    __shared__ __attribute__((aligned(16))) uint16_t Bs[2][PL][BN * LDK];
+
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    const int il = lane & 31, kl = lane >> 5;
+
+    const int64_t ntm = (g.M + BM - 1) / BM, ntn = (g.N + BN - 1) / BN;
+    const int64_t tile = xcd_tile(blockIdx.x, ntm * ntn);
+    const int64_t m0 = (tile / ntn) * BM, n0 = (tile % ntn) * BN;
+
+    for (int64_t l = blockIdx.y; l < g.L; l += gridDim.y) {
+        const float* Ab = (const float*)g.A + g.lA[l];
+        const float* Bb = (const float*)g.B + g.lB[l];
+        f32x16 acc[FN][FM];
+#pragma unroll
+        for (int fn = 0; fn < FN; ++fn)
+#pragma unroll
+            for (int fm = 0; fm < FM; ++fm) acc[fn][fm] = f32x16{};
+
+        TileLoader<float, false, BM, BK> la;
+        TileLoader<float, false, BN, BK> lb;
+        const int64_t nk = (g.K + BK - 1) / BK;
+        if (nk > 0) {
+            la.load(Ab, g.mA, g.kA, m0, 0, g.M, g.K, g.a_load, g.a_vec);
+            lb.load(Bb, g.nB, g.kB, n0, 0, g.N, g.K, g.b_load, g.b_vec);
+            store16<MODE, BM, BK, LDK>(la, As[0][0], As[0][PL - 1], g.a_load);
+            store16<MODE, BN, BK, LDK>(lb, Bs[0][0], Bs[0][PL - 1], g.b_load);
+        }
+        __syncthreads();
+        for (int64_t kt = 0; kt < nk; ++kt) {
+            const int cur = kt & 1;
+            if (kt + 1 < nk) {
+                la.load(Ab, g.mA, g.kA, m0, (kt + 1) * BK, g.M, g.K, g.a_load, g.a_vec);
+                lb.load(Bb, g.nB, g.kB, n0, (kt + 1) * BK, g.N, g.K, g.b_load, g.b_vec);
+            }
+#pragma unroll
+            for (int kk = 0; kk < BK; kk += 16) {
+                uint4 a[PL][FM], b[PL][FN];
+#pragma unroll
+                for (int p = 0; p < PL; ++p) {
+#pragma unroll
+                    for (int fm = 0; fm < FM; ++fm)
+                        a[p][fm] = *reinterpret_cast<const uint4*>(
+                            &As[cur][p][(wm * FM * S + fm * S + il) * LDK + kk + 8 * kl]);
+#pragma unroll
+                    for (int fn = 0; fn < FN; ++fn)
+                        b[p][fn] = *reinterpret_cast<const uint4*>(
+                            &Bs[cur][p][(wn * FN * S + fn * S + il) * LDK + kk + 8 * kl]);
+                }
+#pragma unroll
+                for (int fn = 0; fn < FN; ++fn)
+#pragma unroll
+                    for (int fm = 0; fm < FM; ++fm) {
+                        acc[fn][fm] = mfma16<MODE>(b[0][fn], a[0][fm], acc[fn][fm]);
+                        if (MODE == TK_MODE_SPLIT_BF16) {
+                            acc[fn][fm] = mfma16<MODE>(b[0][fn], a[PL - 1][fm], acc[fn][fm]);
+                            acc[fn][fm] = mfma16<MODE>(b[PL - 1][fn], a[0][fm], acc[fn][fm]);
+                        }
+                    }
+            }
+            if (kt + 1 < nk) {
+                store16<MODE, BM, BK, LDK>(la, As[cur ^ 1][0], As[cur ^ 1][PL - 1], g.a_load);
+                store16<MODE, BN, BK, LDK>(lb, Bs[cur ^ 1][0], Bs[cur ^ 1][PL - 1], g.b_load);
+            }
+            __syncthreads();
+        }
+#pragma unroll
+        for (int fn = 0; fn < FN; ++fn)
+#pragma unroll
+            for (int fm = 0; fm < FM; ++fm) {
+                const int64_t m = m0 + wm * FM * S + fm * S + il;
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    const int64_t n = n0 + wn * FN * S + fn * S + 8 * (r >> 2) + 4 * kl + (r & 3);
+                    if (m < g.M && n < g.N) epilogue<float, false>(g, l, m, n, acc[fn][fm][r], 0.0f);
+                }
+            }
+    }
+}
+
+template <int MODE, int FM, int FN, int BK>
+static hipError_t launch_mfma16(const tk_gett& g, hipStream_t st)
+{
+    constexpr int BM = 2 * FM * 32, BN = 2 * FN * 32;
+    const int64_t tiles = ((g.M + BM - 1) / BM) * ((g.N + BN - 1) / BN);
+    const int64_t ly = g.L < 65535 ? g.L : 65535;
+    hipLaunchKernelGGL((k_mfma16<MODE, FM, FN, BK>), dim3((unsigned)tiles, (unsigned)ly), dim3(256), 0, st, g);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // dispatch
 // ---------------------------------------------------------------------------
 template <typename R, bool CPLX>
@@ -372,8 +532,11 @@ extern "C" int tk_launch_gett(const tk_gett* gp, void* stream)
     const int kern = g.kernel;
     switch (g.ctype) {
     case TK_F32:
-        if (kern == TK_KERNEL_SIMPLE || g.mode != TK_MODE_PLAIN) return launch_simple<float, false>(g, st);
+        if (kern == TK_KERNEL_SIMPLE) return launch_simple<float, false>(g, st);
         if (kern == TK_KERNEL_SPLITK) return launch_splitk<float, false>(g, st);
+        if (g.mode == TK_MODE_BF16) return launch_mfma16<TK_MODE_BF16, 2, 2, 32>(g, st);
+        if (g.mode == TK_MODE_F16) return launch_mfma16<TK_MODE_F16, 2, 2, 32>(g, st);
+        if (g.mode == TK_MODE_SPLIT_BF16) return launch_mfma16<TK_MODE_SPLIT_BF16, 2, 2, 16>(g, st);
         return launch_mfma<float, false, 32, 2, 2, 16>(g, st);
     case TK_F64:
         if (kern == TK_KERNEL_SIMPLE) return launch_simple<double, false>(g, st);

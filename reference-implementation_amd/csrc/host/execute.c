@@ -13,6 +13,40 @@
 
 #include "internal.h"
 
+#include <stdio.h>
+
+/* Last device error of this thread, for TAPP_AMD_last_error_message. */
+static _Thread_local char last_error[256];
+
+static int device_error(const char* what, int err, int line)
+{
+    snprintf(last_error, sizeof last_error, "%s failed at execute.c:%d: %s (%d)", what, line,
+             hipGetErrorString((hipError_t)err), err);
+    if (getenv("TAPP_AMD_VERBOSE")) fprintf(stderr, "tapp-amd: %s\n", last_error);
+    return TAPP_AMD_ERR_DEVICE;
+}
+
+#define HIPCHK(call)                                                            \
+    do {                                                                        \
+        hipError_t e_ = (call);                                                 \
+        if (e_ != hipSuccess) return device_error(#call, (int)e_, __LINE__);    \
+    } while (0)
+#define LAUNCHCHK(call)                                                         \
+    do {                                                                        \
+        int e_ = (call);                                                        \
+        if (e_ != 0) return device_error(#call, e_, __LINE__);                  \
+    } while (0)
+
+size_t TAPP_AMD_last_error_message(char* message, size_t maxlen)
+{
+    size_t len = strlen(last_error);
+    if (maxlen == 0) return len;
+    size_t n = maxlen - 1 < len ? maxlen - 1 : len;
+    memcpy(message, last_error, n);
+    message[n] = '\0';
+    return n;
+}
+
 #define ALIGN256(x) (((x) + 255u) & ~(size_t)255u)
 
 enum {
@@ -131,13 +165,17 @@ static int build_tables(struct tapp_plan* plan, const tk_problem* P, int device,
     fill_linear(h + c->off[T_dnD], N, M);
     fill_linear(h + c->off[T_dlD], L, M * N);
     h[c->off[T_zero]] = 0;
-    if (hipMalloc((void**)&c->dev, total * sizeof(int64_t)) != hipSuccess) {
+    hipError_t e = hipMalloc((void**)&c->dev, total * sizeof(int64_t));
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
         free(c->host); free(c);
+        device_error("hipMalloc(tables)", (int)e, __LINE__);
         return TAPP_AMD_ERR_NOMEM;
     }
-    if (hipMemcpy(c->dev, c->host, total * sizeof(int64_t), hipMemcpyHostToDevice) != hipSuccess) {
+    e = hipMemcpy(c->dev, c->host, total * sizeof(int64_t), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
         tk_plan_cache_free(c);
-        return TAPP_AMD_ERR_DEVICE;
+        return device_error("hipMemcpy(tables)", (int)e, __LINE__);
     }
     c->key = key;
     c->device = device;
@@ -153,7 +191,8 @@ static bool is_device_ptr(const void* p)
     if (!p) return false;
     hipPointerAttribute_t a;
     hipError_t e = hipPointerGetAttributes(&a, p);
-    if (e != hipSuccess) { (void)hipGetLastError(); return false; }
+    (void)hipGetLastError();   /* an unregistered host pointer reports an error: not sticky */
+    if (e != hipSuccess) return false;
     return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged;
 }
 
@@ -162,7 +201,12 @@ static int ensure_ws(struct tapp_exec* ex, size_t bytes)
     if (bytes <= ex->ws_size) return 0;
     if (ex->ws) { hipStreamSynchronize(ex->stream); hipFree(ex->ws); ex->ws = NULL; ex->ws_size = 0; }
     size_t want = bytes + bytes / 4;
-    if (hipMalloc(&ex->ws, want) != hipSuccess) { (void)hipGetLastError(); return TAPP_AMD_ERR_NOMEM; }
+    hipError_t e = hipMalloc(&ex->ws, want);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        device_error("hipMalloc(workspace)", (int)e, __LINE__);
+        return TAPP_AMD_ERR_NOMEM;
+    }
     ex->ws_size = want;
     return 0;
 }
@@ -214,9 +258,9 @@ int tk_execute(struct tapp_plan* p, struct tapp_exec* ex, const void* alpha, con
     int rc = tk_plan_problem(p, ex, alpha, beta, C, &P);
     if (rc) return rc;
     if (P.empty) return 0;
-    if (!ex->stream) return TAPP_AMD_ERR_DEVICE;
+    if (!ex->stream) return device_error("executor without a device stream", 0, __LINE__);
     if (!A || !B || !D || (!P.beta_zero && !C)) return TAPP_AMD_ERR_INVALID;
-    if (hipSetDevice(ex->device) != hipSuccess) return TAPP_AMD_ERR_DEVICE;
+    HIPCHK(hipSetDevice(ex->device));
 
     struct tapp_plan_cache* tc;
     if ((rc = build_tables(p, &P, ex->device, &tc))) return rc;
@@ -267,8 +311,7 @@ int tk_execute(struct tapp_plan* p, struct tapp_exec* ex, const void* alpha, con
             char* buf = ws + off_stage[x];
             const char* hsrc = (const char*)src[x] + P.lo[x] * (int64_t)es;
             /* D is copied in too so elements the product does not write keep their values */
-            if (hipMemcpyAsync(buf, hsrc, span_bytes[x], hipMemcpyHostToDevice, st) != hipSuccess)
-                return TAPP_AMD_ERR_DEVICE;
+            HIPCHK(hipMemcpyAsync(buf, hsrc, span_bytes[x], hipMemcpyHostToDevice, st));
             d[x].base = buf - P.lo[x] * (int64_t)es;
             d[x].staged = true;
         }
@@ -300,21 +343,21 @@ int tk_execute(struct tapp_plan* p, struct tapp_exec* ex, const void* alpha, con
     if (needA && K > 0) {
         tk_prep pr = {P.type[0], ct, P.op_A, (P.nUA > 1 && (P.mode == TK_MODE_F16 || P.mode == TK_MODE_BF16)) ? P.mode : 0,
                       d[0].base, ws + off_tmp[0], M, K, L, P.nUA, DT(T_mA), DT(T_kA), DT(T_lA), DT(T_uA)};
-        if ((rc = tk_launch_prep(&pr, st))) return TAPP_AMD_ERR_DEVICE;
+        LAUNCHCHK(tk_launch_prep(&pr, st));
         g.A = ws + off_tmp[0]; g.mA = DT(T_dmA); g.kA = DT(T_dkA); g.lA = DT(T_dlA); g.conjA = 0;
         hmA = HT(T_dmA); hkA = HT(T_dkA); hlA = HT(T_dlA);
     }
     if (needB && K > 0) {
         tk_prep pr = {P.type[1], ct, P.op_B, (P.nUB > 1 && (P.mode == TK_MODE_F16 || P.mode == TK_MODE_BF16)) ? P.mode : 0,
                       d[1].base, ws + off_tmp[1], N, K, L, P.nUB, DT(T_nB), DT(T_kB), DT(T_lB), DT(T_uB)};
-        if ((rc = tk_launch_prep(&pr, st))) return TAPP_AMD_ERR_DEVICE;
+        LAUNCHCHK(tk_launch_prep(&pr, st));
         g.B = ws + off_tmp[1]; g.nB = DT(T_dnB); g.kB = DT(T_dkB); g.lB = DT(T_dlB); g.conjB = 0;
         hnB = HT(T_dnB); hkB = HT(T_dkB); hlB = HT(T_dlB);
     }
     if (needC) {
         tk_prep pr = {P.type[2], ct, 0, 0, d[2].base, ws + off_tmp[2], M, N, L, 1,
                       DT(T_mC), DT(T_nC), DT(T_lC), DT(T_zero)};
-        if ((rc = tk_launch_prep(&pr, st))) return TAPP_AMD_ERR_DEVICE;
+        LAUNCHCHK(tk_launch_prep(&pr, st));
         g.C = ws + off_tmp[2]; g.mC = DT(T_dmD); g.nC = DT(T_dnD); g.lC = DT(T_dlD);
     }
     if (needD) {
@@ -364,23 +407,22 @@ int tk_execute(struct tapp_plan* p, struct tapp_exec* ex, const void* alpha, con
         else { g.b_load = TK_GATHER; g.b_vec = 0; }
     }
 
-    if ((rc = tk_launch_gett(&g, st))) return TAPP_AMD_ERR_DEVICE;
+    LAUNCHCHK(tk_launch_gett(&g, st));
     ex->last_kernel = kern;
 
     if (needD) {
         tk_scatter sc = {ct, P.type[3], ws + off_tmp[3], (void*)d[3].base, M, N, L,
                          DT(T_mD), DT(T_nD), DT(T_lD)};
-        if ((rc = tk_launch_scatter(&sc, st))) return TAPP_AMD_ERR_DEVICE;
+        LAUNCHCHK(tk_launch_scatter(&sc, st));
     }
     if (ex->timing) hipEventRecord(ex->ev1, st);
 
     if (d[3].staged) {
         const size_t es = tk_type_size(P.type[3]);
         char* hdst = (char*)D + P.lo[3] * (int64_t)es;
-        if (hipMemcpyAsync(hdst, ws + off_stage[3], span_bytes[3], hipMemcpyDeviceToHost, st) != hipSuccess)
-            return TAPP_AMD_ERR_DEVICE;
+        HIPCHK(hipMemcpyAsync(hdst, ws + off_stage[3], span_bytes[3], hipMemcpyDeviceToHost, st));
     }
-    if (hipStreamSynchronize(st) != hipSuccess) return TAPP_AMD_ERR_DEVICE;
+    HIPCHK(hipStreamSynchronize(st));
     if (ex->timing) {
         float ms = 0;
         hipEventElapsedTime(&ms, ex->ev0, ex->ev1);

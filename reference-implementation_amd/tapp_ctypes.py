@@ -34,6 +34,19 @@ _i64p = ctypes.POINTER(ctypes.c_int64)
 _H = ctypes.c_ssize_t  # intptr_t
 
 
+def _single_hip_runtime():
+    """PyTorch's ROCm wheel carries its own libamdhip64.so.  If this library
+    were loaded first it would bring /opt/rocm's runtime, and a later
+    `import torch` would add a second HIP runtime to the process, whose
+    contexts do not see each other's allocations.  Importing torch first makes
+    the dynamic loader resolve this library's libamdhip64.so.7 to the copy
+    already in memory, so one runtime serves both."""
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
 def _arr64(vals):
     vals = [int(v) for v in vals]
     return (ctypes.c_int64 * max(1, len(vals)))(*vals)
@@ -53,7 +66,7 @@ class TappLib:
     )
     EXTENSIONS = (
         "TAPP_execute_batched_product", "TAPP_destroy_status", "TAPP_attr_set", "TAPP_attr_get",
-        "TAPP_attr_clear", "TAPP_AMD_version", "TAPP_AMD_plan_info",
+        "TAPP_attr_clear", "TAPP_AMD_version", "TAPP_AMD_plan_info", "TAPP_AMD_last_error_message",
     )
 
     def __init__(self, path=None):
@@ -61,6 +74,8 @@ class TappLib:
         if not os.path.exists(self.path):
             raise FileNotFoundError(
                 f"{self.path} is not built: run `make -C reference-implementation_amd` or __graft_entry__.build()")
+        if self.path == LIB_PATH:
+            _single_hip_runtime()
         self.lib = ctypes.CDLL(self.path, mode=os.RTLD_LOCAL | os.RTLD_NOW)
         L = self.lib
         L.TAPP_create_tensor_info.argtypes = [ctypes.POINTER(_H), ctypes.c_int, ctypes.c_int, _i64p, _i64p]
@@ -98,6 +113,8 @@ class TappLib:
             L.TAPP_attr_get.argtypes = [_H, ctypes.c_int, ctypes.c_void_p]
             L.TAPP_attr_clear.argtypes = [_H, ctypes.c_int]
             L.TAPP_AMD_version.restype = ctypes.c_char_p
+            L.TAPP_AMD_last_error_message.argtypes = [ctypes.c_char_p, ctypes.c_size_t]
+            L.TAPP_AMD_last_error_message.restype = ctypes.c_size_t
             L.TAPP_AMD_plan_info.argtypes = [_H, _H, ctypes.c_void_p, ctypes.c_void_p, _i64p,
                                              ctypes.POINTER(ctypes.c_int)]
 
@@ -153,7 +170,12 @@ class TappLib:
         n = self.lib.TAPP_explain_error(rc, 0, None)
         buf = ctypes.create_string_buffer(n + 1)
         self.lib.TAPP_explain_error(rc, n + 1, buf)
-        return buf.value.decode()
+        text = buf.value.decode()
+        if self.has_extensions and rc == 102:
+            m = ctypes.create_string_buffer(300)
+            self.lib.TAPP_AMD_last_error_message(m, 300)
+            text += " " + m.value.decode()
+        return text
 
     # --- extensions ---------------------------------------------------------
     def attr_set_int(self, exec_, key, value):
