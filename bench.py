@@ -148,6 +148,19 @@ def cpu_baseline(A, B, D_gpu, sample_j=2):
             "gpu_vs_reference_slice_normwise_err": err}
 
 
+def measured_traffic():
+    """HBM bytes per launch of the FP32 ladder kernel from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE passes (scaled by the copy-kernel
+    calibration factors recorded beside them)."""
+    path = os.path.join(ROOT, "profiles", "ladder_counters.json")
+    try:
+        with open(path) as f:
+            c = json.load(f)["ladder_fp32"]
+        return c["fetch_kib"] * 1024 * c["fetch_factor"] + c["write_kib"] * 1024 * c["write_factor"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -155,6 +168,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-forms", action="store_true", help="skip the BF16/F16/split-BF16 forms")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU reference baseline")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 1 GiB device copy (calibrates the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (products shard over D tiles; "
@@ -163,6 +178,14 @@ def main():
 
     import torch
     assert torch.cuda.is_available()
+    if args.copy_calibration:
+        src = torch.ones(1 << 28, device="cuda", dtype=torch.float32)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": src.numel() * 4, "dispatches": 3}))
+        return
     lib = tc.TappLib()
     flops = 2.0 * (V * V) * (O * O) * (V * V)
     log("bench: generating ladder operands")
@@ -213,7 +236,7 @@ def main():
                                "FP32, TAPP_DEFAULT_PREC",
                    "M": V * V, "N": O * O, "K": V * V, "flop_per_step": flops},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": measured_traffic(),
                      "kernel": "k_mfma<float, FP32 32x32x2>", "kernel_us": dev_us, "kernel_family": kern},
         "cpu_baseline": cpu,
         "forms": forms,
