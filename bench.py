@@ -237,7 +237,7 @@ def main():
                    "M": V * V, "N": O * O, "K": V * V, "flop_per_step": flops},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": measured_traffic(),
-                     "kernel": "k_mfma<float, FP32 32x32x2>", "kernel_us": dev_us, "kernel_family": kern},
+                     "kernel": "k_gett_f32<CONTIG_ROW, CONTIG_K> (v_mfma_f32_32x32x2f32)", "kernel_us": dev_us, "kernel_family": kern},
         "cpu_baseline": cpu,
         "forms": forms,
     }

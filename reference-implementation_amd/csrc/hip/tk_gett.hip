@@ -490,6 +490,234 @@ static hipError_t launch_mfma16(const tk_gett& g, hipStream_t st)
 }
 
 // ---------------------------------------------------------------------------
+// k_gett_f32: FP32 GETT specialised on how A and B tiles are read.
+//   AL, BL: TK_CONTIG_ROW (16-byte loads along M / N), TK_CONTIG_K (16-byte
+//   loads along K) or TK_GATHER (per-element table lookups).
+// 128 x 128 x 32 tiles, 2 x 2 waves of 64 x 64 (2 x 2 v_mfma_f32_32x32x2f32
+// blocks each), LDS double-buffered (2 x 33 KB), one barrier per K tile, the
+// next tile's global loads in flight during the current tile's MFMAs.  Row
+// offsets of the rows a lane loads are read from the tables once, before the
+// K loop; interior tiles load without bounds checks.
+// ---------------------------------------------------------------------------
+template <int MODE, int BR, int BK>
+struct F32Tile {
+    static constexpr int NS = (BR * BK) / (256 * 4);   // 16-byte slots per lane
+    float4 v[NS];
+    int64_t roff[MODE == TK_CONTIG_ROW ? 1 : (MODE == TK_CONTIG_K ? NS : 4)];
+
+    // rows this lane loads (relative to the tile) -- fixed for the whole K loop
+    __device__ __forceinline__ static int row_of(int i, int j)
+    {
+        const int s = threadIdx.x + 256 * i;
+        if (MODE == TK_CONTIG_K) return s / (BK / 4);
+        return (s % (BR / 4)) * 4 + j;
+    }
+    __device__ __forceinline__ static int k_of(int i, int j)
+    {
+        const int s = threadIdx.x + 256 * i;
+        if (MODE == TK_CONTIG_K) return (s % (BK / 4)) * 4 + j;
+        return s / (BR / 4);
+    }
+
+    __device__ __forceinline__ void rows(const int64_t* rt, int64_t r0, int64_t Rn)
+    {
+        if (MODE == TK_CONTIG_ROW) {
+            const int64_t r = r0 + row_of(0, 0);
+            roff[0] = r < Rn ? rt[r] : 0;
+        } else if (MODE == TK_CONTIG_K) {
+#pragma unroll
+            for (int i = 0; i < NS; ++i) {
+                const int64_t r = r0 + row_of(i, 0);
+                roff[i] = r < Rn ? rt[r] : 0;
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int64_t r = r0 + row_of(0, j);
+                roff[j] = r < Rn ? rt[r] : 0;
+            }
+        }
+    }
+
+    template <bool CHECK>
+    __device__ __forceinline__ void load(const float* base, const int64_t* kt, int64_t r0, int64_t k0, int64_t Rn,
+                                         int64_t Kn)
+    {
+#pragma unroll
+        for (int i = 0; i < NS; ++i) {
+            if (MODE == TK_CONTIG_ROW) {
+                const int64_t r = r0 + row_of(i, 0), k = k0 + k_of(i, 0);
+                if (!CHECK || (k < Kn && r + 4 <= Rn)) {
+                    v[i] = *reinterpret_cast<const float4*>(base + roff[0] + kt[k]);
+                } else {
+                    float t[4];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) t[j] = (k < Kn && r + j < Rn) ? base[roff[0] + kt[k] + j] : 0.0f;
+                    v[i] = make_float4(t[0], t[1], t[2], t[3]);
+                }
+            } else if (MODE == TK_CONTIG_K) {
+                const int64_t r = r0 + row_of(i, 0), k = k0 + k_of(i, 0);
+                if (!CHECK || (r < Rn && k + 4 <= Kn)) {
+                    v[i] = *reinterpret_cast<const float4*>(base + roff[i] + kt[k]);
+                } else {
+                    float t[4];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) t[j] = (r < Rn && k + j < Kn) ? base[roff[i] + kt[k + j]] : 0.0f;
+                    v[i] = make_float4(t[0], t[1], t[2], t[3]);
+                }
+            } else {
+                const int64_t k = k0 + k_of(i, 0);
+                float t[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int64_t r = r0 + row_of(i, j);
+                    t[j] = (!CHECK || (k < Kn && r < Rn)) ? base[roff[j] + kt[k]] : 0.0f;
+                }
+                v[i] = make_float4(t[0], t[1], t[2], t[3]);
+            }
+        }
+    }
+
+    template <int LD>
+    __device__ __forceinline__ void store(float* lds) const
+    {
+#pragma unroll
+        for (int i = 0; i < NS; ++i) {
+            if (MODE == TK_CONTIG_K) {
+                const int r = row_of(i, 0), k = k_of(i, 0);
+                lds[(k + 0) * LD + r] = v[i].x;
+                lds[(k + 1) * LD + r] = v[i].y;
+                lds[(k + 2) * LD + r] = v[i].z;
+                lds[(k + 3) * LD + r] = v[i].w;
+            } else {
+                *reinterpret_cast<float4*>(&lds[k_of(i, 0) * LD + row_of(i, 0)]) = v[i];
+            }
+        }
+    }
+};
+
+template <int AL, int BL>
+__global__ __launch_bounds__(256, 2) void k_gett_f32(tk_gett g)
+{
+    constexpr int S = 32, FM = 2, FN = 2, BM = 128, BN = 128, BK = 32;
+    constexpr int LDA = BM + 4, LDB = BN + 4;
+    __shared__ __attribute__((aligned(16))) float As[2][BK * LDA];
+    __shared__ __attribute__((aligned(16))) float Bs[2][BK * LDB];
+
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    const int il = lane & 31, kl = lane >> 5;
+
+    const int64_t ntm = (g.M + BM - 1) / BM, ntn = (g.N + BN - 1) / BN;
+    const int64_t tile = xcd_tile(blockIdx.x, ntm * ntn);
+    const int64_t m0 = (tile / ntn) * BM, n0 = (tile % ntn) * BN;
+    const bool interior = m0 + BM <= g.M && n0 + BN <= g.N;
+    const int64_t nk = (g.K + BK - 1) / BK, nk_full = g.K / BK;
+
+    F32Tile<AL, BM, BK> la;
+    F32Tile<BL, BN, BK> lb;
+    la.rows(g.mA, m0, g.M);
+    lb.rows(g.nB, n0, g.N);
+
+    for (int64_t l = blockIdx.y; l < g.L; l += gridDim.y) {
+        const float* Ab = (const float*)g.A + g.lA[l];
+        const float* Bb = (const float*)g.B + g.lB[l];
+        f32x16 acc[FN][FM];
+#pragma unroll
+        for (int fn = 0; fn < FN; ++fn)
+#pragma unroll
+            for (int fm = 0; fm < FM; ++fm) acc[fn][fm] = f32x16{};
+
+        auto fetch = [&](int64_t kt) {
+            if (interior && kt < nk_full) {
+                la.template load<false>(Ab, g.kA, m0, kt * BK, g.M, g.K);
+                lb.template load<false>(Bb, g.kB, n0, kt * BK, g.N, g.K);
+            } else {
+                la.template load<true>(Ab, g.kA, m0, kt * BK, g.M, g.K);
+                lb.template load<true>(Bb, g.kB, n0, kt * BK, g.N, g.K);
+            }
+        };
+        if (nk > 0) {
+            fetch(0);
+            la.template store<LDA>(As[0]);
+            lb.template store<LDB>(Bs[0]);
+        }
+        __syncthreads();
+        for (int64_t kt = 0; kt < nk; ++kt) {
+            const int cur = kt & 1;
+            if (kt + 1 < nk) fetch(kt + 1);
+            const float* as = As[cur];
+            const float* bs = Bs[cur];
+#pragma unroll
+            for (int kk = 0; kk < BK; kk += 2) {
+                float a[FM], b[FN];
+#pragma unroll
+                for (int fm = 0; fm < FM; ++fm) a[fm] = as[(kk + kl) * LDA + wm * FM * S + fm * S + il];
+#pragma unroll
+                for (int fn = 0; fn < FN; ++fn) b[fn] = bs[(kk + kl) * LDB + wn * FN * S + fn * S + il];
+#pragma unroll
+                for (int fn = 0; fn < FN; ++fn)
+#pragma unroll
+                    for (int fm = 0; fm < FM; ++fm)
+                        acc[fn][fm] = __builtin_amdgcn_mfma_f32_32x32x2f32(b[fn], a[fm], acc[fn][fm], 0, 0, 0);
+            }
+            if (kt + 1 < nk) {
+                la.template store<LDA>(As[cur ^ 1]);
+                lb.template store<LDB>(Bs[cur ^ 1]);
+            }
+            __syncthreads();
+        }
+
+        const bool plain = g.beta_zero;
+        const float alpha = (float)g.alpha[0];
+        const int64_t dl = g.lD[l];
+#pragma unroll
+        for (int fm = 0; fm < FM; ++fm) {
+            const int64_t m = m0 + wm * FM * S + fm * S + il;
+            if (m >= g.M) continue;
+            const int64_t dm = dl + g.mD[m];
+#pragma unroll
+            for (int fn = 0; fn < FN; ++fn)
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    const int64_t n = n0 + wn * FN * S + fn * S + 8 * (r >> 2) + 4 * kl + (r & 3);
+                    if (n >= g.N) continue;
+                    if (plain) ((float*)g.D)[dm + g.nD[n]] = alpha * acc[fn][fm][r];
+                    else epilogue<float, false>(g, l, m, n, acc[fn][fm][r], 0.0f);
+                }
+        }
+    }
+}
+
+template <int AL, int BL>
+static hipError_t launch_gett_f32_modes(const tk_gett& g, hipStream_t st)
+{
+    const int64_t tiles = ((g.M + 127) / 128) * ((g.N + 127) / 128);
+    const int64_t ly = g.L < 65535 ? g.L : 65535;
+    hipLaunchKernelGGL((k_gett_f32<AL, BL>), dim3((unsigned)tiles, (unsigned)ly), dim3(256), 0, st, g);
+    return hipGetLastError();
+}
+
+static hipError_t launch_gett_f32(const tk_gett& g, hipStream_t st)
+{
+    const int a = g.a_vec ? g.a_load : TK_GATHER;
+    const int b = g.b_vec ? g.b_load : TK_GATHER;
+#define TK_F32_CASE(X, Y) \
+    if (a == X && b == Y) return launch_gett_f32_modes<X, Y>(g, st);
+    TK_F32_CASE(TK_CONTIG_ROW, TK_CONTIG_ROW)
+    TK_F32_CASE(TK_CONTIG_ROW, TK_CONTIG_K)
+    TK_F32_CASE(TK_CONTIG_ROW, TK_GATHER)
+    TK_F32_CASE(TK_CONTIG_K, TK_CONTIG_ROW)
+    TK_F32_CASE(TK_CONTIG_K, TK_CONTIG_K)
+    TK_F32_CASE(TK_CONTIG_K, TK_GATHER)
+    TK_F32_CASE(TK_GATHER, TK_CONTIG_ROW)
+    TK_F32_CASE(TK_GATHER, TK_CONTIG_K)
+    TK_F32_CASE(TK_GATHER, TK_GATHER)
+#undef TK_F32_CASE
+    return hipErrorInvalidValue;
+}
+
+// ---------------------------------------------------------------------------
 // dispatch
 // ---------------------------------------------------------------------------
 template <typename R, bool CPLX>
@@ -537,7 +765,7 @@ extern "C" int tk_launch_gett(const tk_gett* gp, void* stream)
         if (g.mode == TK_MODE_BF16) return launch_mfma16<TK_MODE_BF16, 2, 2, 32>(g, st);
         if (g.mode == TK_MODE_F16) return launch_mfma16<TK_MODE_F16, 2, 2, 32>(g, st);
         if (g.mode == TK_MODE_SPLIT_BF16) return launch_mfma16<TK_MODE_SPLIT_BF16, 2, 2, 16>(g, st);
-        return launch_mfma<float, false, 32, 2, 2, 16>(g, st);
+        return launch_gett_f32(g, st);
     case TK_F64:
         if (kern == TK_KERNEL_SIMPLE) return launch_simple<double, false>(g, st);
         if (kern == TK_KERNEL_SPLITK) return launch_splitk<double, false>(g, st);
