@@ -54,8 +54,35 @@ enum {
     TAPP_AMD_KERNEL_NONE = 0,
     TAPP_AMD_KERNEL_SIMPLE = 1,
     TAPP_AMD_KERNEL_MFMA = 2,
-    TAPP_AMD_KERNEL_SPLITK = 3
+    TAPP_AMD_KERNEL_SPLITK = 3,
+    TAPP_AMD_KERNEL_GROUP = 4       /* persistent launch of a product group */
 };
+
+/* --- product groups ---------------------------------------------------------
+ * Many independent products, each with its own plan, operands and scalars,
+ * run as one persistent launch (a work queue of tiles over all products,
+ * captured in a hipGraph).  The reference declares only the one-plan
+ * TAPP_execute_batched_product (product.h:66-75) and defines nothing.
+ *
+ * create: validates every product exactly as TAPP_execute_product would and
+ *   returns the first failing code; reads alpha[i]/beta[i] now (values are
+ *   kept); keeps the operand pointers.  Layouts and data pointers must not
+ *   change while the group exists (the group replays the same addresses).
+ * execute: runs every product; returns when all D are written.
+ * Products outside the persistent kernel's scope (non-FP32 compute, unary
+ * labels, storage conversion, host pointers) run one by one inside execute. */
+typedef intptr_t TAPP_AMD_product_group;
+
+TAPP_EXPORT TAPP_error TAPP_AMD_create_product_group(TAPP_AMD_product_group* group, TAPP_executor exec,
+                                                     int count, const TAPP_tensor_product* plans,
+                                                     const void* const* alpha, const void* const* A,
+                                                     const void* const* B, const void* const* beta,
+                                                     const void* const* C, void* const* D);
+TAPP_EXPORT TAPP_error TAPP_AMD_execute_product_group(TAPP_AMD_product_group group);
+TAPP_EXPORT TAPP_error TAPP_AMD_destroy_product_group(TAPP_AMD_product_group group);
+/* info[0] products in the persistent launch, info[1] products run one by
+ * one, info[2] tiles of the persistent launch. */
+TAPP_EXPORT TAPP_error TAPP_AMD_group_info(TAPP_AMD_product_group group, int64_t info[3]);
 
 /* Text of the last device error of the calling thread (which HIP call
  * failed and why), with TAPP_explain_error's maxlen contract. */

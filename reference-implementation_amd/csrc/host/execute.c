@@ -184,6 +184,31 @@ static int build_tables(struct tapp_plan* plan, const tk_problem* P, int device,
     return 0;
 }
 
+/* Device tables of a planned problem (built or taken from the plan cache). */
+int tk_problem_tables(struct tapp_plan* plan, const tk_problem* P, int device, const int64_t** mA,
+                      const int64_t** kA, const int64_t** lA, const int64_t** nB, const int64_t** kB,
+                      const int64_t** lB, const int64_t** mD, const int64_t** nD, const int64_t** lD,
+                      const int64_t** mC, const int64_t** nC, const int64_t** lC)
+{
+    struct tapp_plan_cache* c;
+    int rc = build_tables(plan, P, device, &c);
+    if (rc) return rc;
+    const int64_t* T = c->dev;
+    *mA = T + c->off[T_mA]; *kA = T + c->off[T_kA]; *lA = T + c->off[T_lA];
+    *nB = T + c->off[T_nB]; *kB = T + c->off[T_kB]; *lB = T + c->off[T_lB];
+    *mD = T + c->off[T_mD]; *nD = T + c->off[T_nD]; *lD = T + c->off[T_lD];
+    *mC = T + c->off[T_mC]; *nC = T + c->off[T_nC]; *lC = T + c->off[T_lC];
+    return 0;
+}
+
+/* Host copies of the K tables of A and B (valid after tk_problem_tables). */
+void tk_problem_host_k(struct tapp_plan* plan, const int64_t** kA, const int64_t** kB)
+{
+    struct tapp_plan_cache* c = plan->cache;
+    *kA = c->host + c->off[T_kA];
+    *kB = c->host + c->off[T_kB];
+}
+
 /* --- residency ------------------------------------------------------------ */
 
 static bool is_device_ptr(const void* p)

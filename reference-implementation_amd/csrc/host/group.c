@@ -1,0 +1,302 @@
+/*
+ * Product groups: many independent products, each with its own plan and
+ * operands, executed as one persistent launch (csrc/hip/tk_group.hip).
+ *
+ * The reference declares TAPP_execute_batched_product (product.h:66-75: one
+ * plan, num_batches operand sets) but defines nothing; uneven batches have
+ * no reference API at all.  A group is prepared once -- every product
+ * validated with the codes of TAPP_execute_product, classified, its offset
+ * tables cached, the descriptors uploaded and the zero-counter + persistent
+ * kernel pair captured in a hipGraph -- and then replayed.  Products the
+ * persistent kernel does not cover (non-FP32 compute, unary labels,
+ * conversion pre-passes, host operands) run one by one after it, through the
+ * same path as TAPP_execute_product.
+ */
+#include <stdlib.h>
+#include <string.h>
+
+#include "internal.h"
+
+struct tapp_group {
+    uint32_t magic;
+    struct tapp_exec* ex;
+    int n;                     /* products in the persistent launch */
+    int64_t total;             /* tiles */
+    int nblocks;
+    void* dmem;                /* tk_prob[n] | int64 tile_end[n] | counter */
+    tk_prob* dprobs;
+    int64_t* dtile_end;
+    unsigned long long* dcounter;
+    hipGraph_t graph;
+    hipGraphExec_t gexec;
+    bool have_graph;
+    /* products executed one by one */
+    int nrest;
+    struct tapp_plan** rest_plan;
+    const void **rest_alpha, **rest_A, **rest_B, **rest_beta, **rest_C;
+    void** rest_D;
+};
+
+static bool device_ptr(const void* p)
+{
+    if (!p) return false;
+    hipPointerAttribute_t a;
+    hipError_t e = hipPointerGetAttributes(&a, p);
+    (void)hipGetLastError();
+    return e == hipSuccess && (a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged);
+}
+
+typedef struct { tk_prob p; double cost; } ranked;
+
+static int by_cost(const void* x, const void* y)
+{
+    const ranked* a = x; const ranked* b = y;
+    return a->cost > b->cost ? -1 : a->cost < b->cost;
+}
+
+/* Descriptor of one eligible product; returns false when the product needs
+ * the general path. */
+static bool describe(struct tapp_plan* plan, struct tapp_exec* ex, const tk_problem* P, const void* A,
+                     const void* B, const void* C, void* D, tk_prob* out, int* rc)
+{
+    *rc = 0;
+    if (P->ctype != TK_F32 || P->mode == TK_MODE_SPLIT_BF16) return false;
+    if (P->nUA != 1 || P->nUB != 1) return false;
+    for (int x = 0; x < 4; x++)
+        if (P->type[x] != 0 && !(x == 2 && P->beta_zero)) return false;
+    if (!device_ptr(A) || !device_ptr(B) || !device_ptr(D) || (!P->beta_zero && !device_ptr(C))) return false;
+    const int64_t *mA, *kA, *lA, *nB, *kB, *lB, *mD, *nD, *lD, *mC, *nC, *lC;
+    if ((*rc = tk_problem_tables(plan, P, ex->device, &mA, &kA, &lA, &nB, &kB, &lB, &mD, &nD, &lD, &mC, &nC, &lC)))
+        return false;
+    const int64_t* hkA; const int64_t* hkB;
+    tk_problem_host_k(plan, &hkA, &hkB);
+    memset(out, 0, sizeof *out);
+    const float* a = (const float*)A + P->pin[0];
+    const float* b = (const float*)B + P->pin[1];
+    const float* c = C ? (const float*)C + P->pin[2] : NULL;
+    float* d = (float*)D + P->pin[3];
+    const int64_t M = P->nM, N = P->nN;
+    int64_t dm = INT64_MAX, dn = INT64_MAX;
+    for (int i = 0; i < P->M.n; i++) if (llabs(P->M.s[3][i]) < dm) dm = llabs(P->M.s[3][i]);
+    for (int i = 0; i < P->N.n; i++) if (llabs(P->N.s[3][i]) < dn) dn = llabs(P->N.s[3][i]);
+    bool x_is_b;
+    if (N <= 16 || M <= 16) {
+        out->cls = TK_GROUP_THIN;
+        x_is_b = N <= M;            /* the small side is X */
+    } else {
+        out->cls = TK_GROUP_MFMA;
+        x_is_b = dn < dm;           /* MFMA lanes run along X's rows: put D's unit stride there */
+    }
+    if (x_is_b) {
+        out->S = N; out->G = M;
+        out->X = b; out->sX = nB; out->kX = kB; out->lX = lB;
+        out->Y = a; out->gY = mA; out->kY = kA; out->lY = lA;
+        out->sD = nD; out->gD = mD; out->sC = nC; out->gC = mC;
+        out->y_kfast = P->nK > 1 && hkA[1] - hkA[0] == 1;
+    } else {
+        out->S = M; out->G = N;
+        out->X = a; out->sX = mA; out->kX = kA; out->lX = lA;
+        out->Y = b; out->gY = nB; out->kY = kB; out->lY = lB;
+        out->sD = mD; out->gD = nD; out->sC = mC; out->gC = nC;
+        out->y_kfast = P->nK > 1 && hkB[1] - hkB[0] == 1;
+    }
+    out->lD = lD; out->lC = lC;
+    out->K = P->nK; out->L = P->nL;
+    out->C = c; out->D = d;
+    out->alpha = (float)P->alpha[0];
+    out->beta = (float)P->beta[0];
+    out->beta_zero = P->beta_zero;
+    out->round = P->mode;
+    out->round_C = P->round_C;
+    if (out->cls == TK_GROUP_THIN) { out->tiles_s = 1; out->tiles_g = (out->G + 255) / 256; }
+    else { out->tiles_s = (out->S + 63) / 64; out->tiles_g = (out->G + 63) / 64; }
+    return true;
+}
+
+static void group_free(struct tapp_group* g)
+{
+    if (!g) return;
+    if (g->have_graph) { hipGraphExecDestroy(g->gexec); hipGraphDestroy(g->graph); }
+    if (g->dmem) hipFree(g->dmem);
+    free(g->rest_plan); free(g->rest_alpha); free(g->rest_A); free(g->rest_B);
+    free(g->rest_beta); free(g->rest_C); free(g->rest_D);
+    free(g);
+}
+
+static int create_group(TAPP_AMD_product_group* group, TAPP_executor exec, int count,
+                        const TAPP_tensor_product* plans, const void* const* alpha, const void* const* A,
+                        const void* const* B, const void* const* beta, const void* const* C, void* const* D,
+                        bool capture)
+{
+    if (!group || count < 0 || (count > 0 && (!plans || !alpha || !A || !B || !beta || !D)))
+        return TAPP_AMD_ERR_INVALID;
+    struct tapp_exec* ex = tk_exec_lookup(exec);
+    if (!ex) ex = (exec && (*(const int*)exec == 1 || *(const int*)exec == 2 || *(const int*)exec == 12))
+                      ? tk_default_exec() : NULL;
+    if (!ex) return 33;
+    if (!ex->stream) return TAPP_AMD_ERR_DEVICE;
+    struct tapp_group* g = calloc(1, sizeof *g);
+    if (!g) return TAPP_AMD_ERR_NOMEM;
+    g->magic = TAPP_AMD_EXEC_MAGIC;
+    g->ex = ex;
+    ranked* list = malloc(sizeof(ranked) * (size_t)(count > 0 ? count : 1));
+    g->rest_plan = malloc(sizeof(void*) * (size_t)(count > 0 ? count : 1));
+    g->rest_alpha = malloc(sizeof(void*) * (size_t)(count > 0 ? count : 1));
+    g->rest_A = malloc(sizeof(void*) * (size_t)(count > 0 ? count : 1));
+    g->rest_B = malloc(sizeof(void*) * (size_t)(count > 0 ? count : 1));
+    g->rest_beta = malloc(sizeof(void*) * (size_t)(count > 0 ? count : 1));
+    g->rest_C = malloc(sizeof(void*) * (size_t)(count > 0 ? count : 1));
+    g->rest_D = malloc(sizeof(void*) * (size_t)(count > 0 ? count : 1));
+    if (!list || !g->rest_plan || !g->rest_alpha || !g->rest_A || !g->rest_B || !g->rest_beta || !g->rest_C ||
+        !g->rest_D) {
+        free(list); group_free(g);
+        return TAPP_AMD_ERR_NOMEM;
+    }
+    if (hipSetDevice(ex->device) != hipSuccess) { free(list); group_free(g); return TAPP_AMD_ERR_DEVICE; }
+    int n = 0;
+    for (int i = 0; i < count; i++) {
+        struct tapp_plan* p = (struct tapp_plan*)plans[i];
+        if (!p) { free(list); group_free(g); return TAPP_AMD_ERR_INVALID; }
+        const void* c = C ? C[i] : NULL;
+        tk_problem P;
+        int rc = tk_plan_problem(p, ex, alpha[i], beta[i], c, &P);
+        if (rc) { free(list); group_free(g); return rc; }
+        if (P.empty) continue;
+        tk_prob d;
+        if (describe(p, ex, &P, A[i], B[i], c, D[i], &d, &rc)) {
+            double per_tile = d.cls == TK_GROUP_THIN ? 256.0 * (double)d.K * (double)(d.S + 1)
+                                                     : 64.0 * 64.0 * (double)d.K;
+            list[n].p = d;
+            list[n].cost = per_tile + 1.0;
+            n++;
+        } else {
+            if (rc) { free(list); group_free(g); return rc; }
+            int r = g->nrest++;
+            g->rest_plan[r] = p; g->rest_alpha[r] = alpha[i]; g->rest_A[r] = A[i]; g->rest_B[r] = B[i];
+            g->rest_beta[r] = beta[i]; g->rest_C[r] = c; g->rest_D[r] = D[i];
+        }
+    }
+    qsort(list, (size_t)n, sizeof(ranked), by_cost);
+    g->n = n;
+    if (n > 0) {
+        size_t pb = sizeof(tk_prob) * (size_t)n, tb = sizeof(int64_t) * (size_t)n;
+        size_t bytes = pb + tb + 256;
+        tk_prob* hp = malloc(pb);
+        int64_t* he = malloc(tb);
+        if (!hp || !he) { free(hp); free(he); free(list); group_free(g); return TAPP_AMD_ERR_NOMEM; }
+        int64_t total = 0;
+        for (int i = 0; i < n; i++) {
+            hp[i] = list[i].p;
+            total += hp[i].L * hp[i].tiles_s * hp[i].tiles_g;
+            he[i] = total;
+        }
+        g->total = total;
+        hipError_t e = hipMalloc(&g->dmem, bytes);
+        if (e != hipSuccess) { (void)hipGetLastError(); free(hp); free(he); free(list); group_free(g); return TAPP_AMD_ERR_NOMEM; }
+        g->dprobs = (tk_prob*)g->dmem;
+        g->dtile_end = (int64_t*)((char*)g->dmem + pb);
+        g->dcounter = (unsigned long long*)((char*)g->dmem + ((pb + tb + 255) & ~(size_t)255));
+        e = hipMemcpy(g->dprobs, hp, pb, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(g->dtile_end, he, tb, hipMemcpyHostToDevice);
+        free(hp); free(he);
+        if (e != hipSuccess) { free(list); group_free(g); return TAPP_AMD_ERR_DEVICE; }
+        int cus = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ex->device) != hipSuccess || cus <= 0)
+            cus = 256;
+        int64_t nb = (int64_t)cus * 2;
+        g->nblocks = (int)(total < nb ? total : nb);
+        /* capture zero-counter + persistent kernel once; replays reuse it */
+        if (capture && hipStreamBeginCapture(ex->stream, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+            int lrc = tk_launch_group(g->dprobs, g->dtile_end, n, total, g->dcounter, g->nblocks, ex->stream);
+            hipError_t ce = hipStreamEndCapture(ex->stream, &g->graph);
+            if (lrc == 0 && ce == hipSuccess && hipGraphInstantiate(&g->gexec, g->graph, NULL, NULL, 0) == hipSuccess)
+                g->have_graph = true;
+            else if (ce == hipSuccess && g->graph) hipGraphDestroy(g->graph);
+            (void)hipGetLastError();
+        }
+    }
+    free(list);
+    *group = (TAPP_AMD_product_group)g;
+    return 0;
+}
+
+TAPP_error TAPP_AMD_create_product_group(TAPP_AMD_product_group* group, TAPP_executor exec, int count,
+                                         const TAPP_tensor_product* plans, const void* const* alpha,
+                                         const void* const* A, const void* const* B, const void* const* beta,
+                                         const void* const* C, void* const* D)
+{
+    return create_group(group, exec, count, plans, alpha, A, B, beta, C, D, true);
+}
+
+/* TAPP_execute_batched_product (product.h:66-75): one plan, num_batches
+ * operand sets, one alpha and beta, run as one persistent launch. */
+TAPP_error TAPP_execute_batched_product(TAPP_tensor_product plan, TAPP_executor exec, TAPP_status* status,
+                                        int num_batches, const void* alpha, const void** A, const void** B,
+                                        const void* beta, const void** C, void** D)
+{
+    (void)status;
+    if (!plan || num_batches < 0 || (num_batches > 0 && (!A || !B || !D))) return TAPP_AMD_ERR_INVALID;
+    if (num_batches == 0) return 0;
+    size_t n = (size_t)num_batches;
+    TAPP_tensor_product* plans = malloc(sizeof(TAPP_tensor_product) * n);
+    const void** al = malloc(sizeof(void*) * n);
+    const void** be = malloc(sizeof(void*) * n);
+    if (!plans || !al || !be) { free(plans); free(al); free(be); return TAPP_AMD_ERR_NOMEM; }
+    for (size_t i = 0; i < n; i++) { plans[i] = plan; al[i] = alpha; be[i] = beta; }
+    TAPP_AMD_product_group g = 0;
+    int rc = create_group(&g, exec, num_batches, plans, al, A, B, be, C, D, false);
+    if (!rc) rc = TAPP_AMD_execute_product_group(g);
+    if (g) TAPP_AMD_destroy_product_group(g);
+    free(plans); free(al); free(be);
+    return rc;
+}
+
+TAPP_error TAPP_AMD_execute_product_group(TAPP_AMD_product_group group)
+{
+    struct tapp_group* g = (struct tapp_group*)group;
+    if (!g || g->magic != TAPP_AMD_EXEC_MAGIC) return TAPP_AMD_ERR_INVALID;
+    struct tapp_exec* ex = g->ex;
+    if (hipSetDevice(ex->device) != hipSuccess) return TAPP_AMD_ERR_DEVICE;
+    if (ex->timing) hipEventRecord(ex->ev0, ex->stream);
+    if (g->n > 0) {
+        if (g->have_graph) {
+            if (hipGraphLaunch(g->gexec, ex->stream) != hipSuccess) return TAPP_AMD_ERR_DEVICE;
+        } else if (tk_launch_group(g->dprobs, g->dtile_end, g->n, g->total, g->dcounter, g->nblocks, ex->stream)) {
+            return TAPP_AMD_ERR_DEVICE;
+        }
+    }
+    if (ex->timing) hipEventRecord(ex->ev1, ex->stream);
+    for (int r = 0; r < g->nrest; r++) {
+        int rc = tk_execute(g->rest_plan[r], ex, g->rest_alpha[r], g->rest_A[r], g->rest_B[r], g->rest_beta[r],
+                            g->rest_C[r], g->rest_D[r]);
+        if (rc) return rc;
+    }
+    if (hipStreamSynchronize(ex->stream) != hipSuccess) return TAPP_AMD_ERR_DEVICE;
+    if (ex->timing && g->n > 0) {
+        float ms = 0;
+        hipEventElapsedTime(&ms, ex->ev0, ex->ev1);
+        ex->last_device_us = ms * 1000.0;
+    }
+    ex->last_kernel = TAPP_AMD_KERNEL_GROUP;
+    return 0;
+}
+
+TAPP_error TAPP_AMD_destroy_product_group(TAPP_AMD_product_group group)
+{
+    struct tapp_group* g = (struct tapp_group*)group;
+    if (!g) return 0;
+    if (g->ex && g->ex->stream) hipStreamSynchronize(g->ex->stream);
+    g->magic = 0;
+    group_free(g);
+    return 0;
+}
+
+TAPP_error TAPP_AMD_group_info(TAPP_AMD_product_group group, int64_t info[3])
+{
+    struct tapp_group* g = (struct tapp_group*)group;
+    if (!g || g->magic != TAPP_AMD_EXEC_MAGIC) return TAPP_AMD_ERR_INVALID;
+    info[0] = g->n;
+    info[1] = g->nrest;
+    info[2] = g->total;
+    return 0;
+}

@@ -88,22 +88,6 @@ TAPP_error TAPP_execute_product(TAPP_tensor_product plan, TAPP_executor exec, TA
     return tk_execute(p, ex, alpha, A, B, beta, C, D);
 }
 
-TAPP_error TAPP_execute_batched_product(TAPP_tensor_product plan, TAPP_executor exec, TAPP_status* status,
-                                        int num_batches, const void* alpha, const void** A, const void** B,
-                                        const void* beta, const void** C, void** D)
-{
-    (void)status;
-    struct tapp_plan* p = (struct tapp_plan*)plan;
-    if (!p || num_batches < 0 || (num_batches > 0 && (!A || !B || !D))) return TAPP_AMD_ERR_INVALID;
-    struct tapp_exec shadow;
-    struct tapp_exec* ex = resolve_exec(exec, &shadow);
-    for (int b = 0; b < num_batches; b++) {
-        int rc = tk_execute(p, ex, alpha, A[b], B[b], beta, C ? C[b] : NULL, D[b]);
-        if (rc) return rc;
-    }
-    return 0;
-}
-
 TAPP_error TAPP_AMD_plan_info(TAPP_tensor_product plan, TAPP_executor exec, const void* beta, const void* C,
                               int64_t sizes[6], int* compute_type)
 {

@@ -96,11 +96,44 @@ typedef struct tk_scatter {
     const int64_t *mD, *nD, *lD;
 } tk_scatter;
 
+/* One product of a persistent group launch (FP32 compute, no unary labels,
+ * device operands).  Classes:
+ *   TK_GROUP_THIN: D[l, s, g] = sum_k X[l, s, k] Y[l, k, g] with S <= 16; a
+ *     workgroup takes 256 columns g and every s, streaming Y through LDS.
+ *   TK_GROUP_MFMA: D[l, m, n] over 64 x 64 MFMA tiles (X = rows m, Y = cols n).
+ * tiles = L * tiles_s * tiles_g; tile_end is the inclusive prefix over the
+ * group's problem order. */
+enum { TK_GROUP_THIN = 0, TK_GROUP_MFMA = 1 };
+
+typedef struct tk_prob {
+    int64_t S, G, K, L;
+    int64_t tiles_s, tiles_g;
+    const float* X;
+    const float* Y;
+    const float* C;
+    float* D;
+    const int64_t *sX, *kX, *lX;
+    const int64_t *gY, *kY, *lY;
+    const int64_t *sD, *gD, *lD;
+    const int64_t *sC, *gC, *lC;
+    float alpha, beta;
+    int beta_zero;
+    int cls;
+    int y_kfast;        /* Y's K step is 1: load Y tiles along K */
+    int round;          /* 0 or TK_MODE_BF16 / TK_MODE_F16: inputs rounded (FP32 accumulate) */
+    int round_C;
+    int pad;
+} tk_prob;
+
 /* All launchers are asynchronous on `stream` (a hipStream_t) and return 0
  * or a hipError_t value. */
 int tk_launch_gett(const tk_gett* g, void* stream);
 int tk_launch_prep(const tk_prep* p, void* stream);
 int tk_launch_scatter(const tk_scatter* s, void* stream);
+/* Zeroes *counter, then runs the persistent group kernel: `nblocks`
+ * workgroups take tiles from the atomic counter until `total` is reached. */
+int tk_launch_group(const tk_prob* probs, const int64_t* tile_end, int nprobs, int64_t total,
+                    unsigned long long* counter, int nblocks, void* stream);
 
 /* Number of kernel variants compiled in, for diagnostics. */
 int tk_kernel_count(void);

@@ -26,7 +26,7 @@ ATTR_LAST_KERNEL = 0x1003
 ATTR_LAST_DEVICE_US = 0x1004
 ATTR_STREAM = 0x1005
 ATTR_TIMING = 0x1006
-KERNEL_AUTO, KERNEL_SIMPLE, KERNEL_MFMA, KERNEL_SPLITK = 0, 1, 2, 3
+KERNEL_AUTO, KERNEL_SIMPLE, KERNEL_MFMA, KERNEL_SPLITK, KERNEL_GROUP = 0, 1, 2, 3, 4
 
 TYPE_SIZE = {F32: 4, F64: 8, C32: 8, C64: 16, F16: 2, BF16: 2}
 
@@ -67,6 +67,8 @@ class TappLib:
     EXTENSIONS = (
         "TAPP_execute_batched_product", "TAPP_destroy_status", "TAPP_attr_set", "TAPP_attr_get",
         "TAPP_attr_clear", "TAPP_AMD_version", "TAPP_AMD_plan_info", "TAPP_AMD_last_error_message",
+        "TAPP_AMD_create_product_group", "TAPP_AMD_execute_product_group", "TAPP_AMD_destroy_product_group",
+        "TAPP_AMD_group_info",
     )
 
     def __init__(self, path=None):
@@ -117,6 +119,12 @@ class TappLib:
             L.TAPP_AMD_last_error_message.restype = ctypes.c_size_t
             L.TAPP_AMD_plan_info.argtypes = [_H, _H, ctypes.c_void_p, ctypes.c_void_p, _i64p,
                                              ctypes.POINTER(ctypes.c_int)]
+            VP = ctypes.POINTER(ctypes.c_void_p)
+            L.TAPP_AMD_create_product_group.argtypes = [ctypes.POINTER(_H), _H, ctypes.c_int, ctypes.POINTER(_H),
+                                                        VP, VP, VP, VP, VP, VP]
+            L.TAPP_AMD_execute_product_group.argtypes = [_H]
+            L.TAPP_AMD_destroy_product_group.argtypes = [_H]
+            L.TAPP_AMD_group_info.argtypes = [_H, _i64p]
 
     # --- objects ---------------------------------------------------------
     def create_info(self, dtype, extents, strides):
@@ -191,6 +199,26 @@ class TappLib:
         v = ctypes.c_double(0)
         rc = self.lib.TAPP_attr_get(exec_, key, ctypes.byref(v))
         return rc, v.value
+
+    def create_group(self, exec_, plans, alphas, a_ptrs, b_ptrs, betas, c_ptrs, d_ptrs):
+        n = len(plans)
+        arr = lambda ps: (ctypes.c_void_p * max(1, n))(*ps)  # noqa: E731
+        h = _H()
+        rc = self.lib.TAPP_AMD_create_product_group(ctypes.byref(h), exec_, n, (_H * max(1, n))(*plans),
+                                                    arr(alphas), arr(a_ptrs), arr(b_ptrs), arr(betas),
+                                                    arr(c_ptrs), arr(d_ptrs))
+        return rc, h.value
+
+    def execute_group(self, g):
+        return self.lib.TAPP_AMD_execute_product_group(g)
+
+    def destroy_group(self, g):
+        return self.lib.TAPP_AMD_destroy_product_group(g)
+
+    def group_info(self, g):
+        info = (ctypes.c_int64 * 3)()
+        rc = self.lib.TAPP_AMD_group_info(g, info)
+        return rc, list(info)
 
     def plan_info(self, plan, exec_, beta_ptr, c_ptr):
         sizes = (ctypes.c_int64 * 6)()

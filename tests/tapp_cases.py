@@ -440,3 +440,47 @@ def random_case(seed, scenario="contraction", dtype=tc.F32, max_ext=4, prec=tc.D
         beta = 0.0
         c_null = bool(rng.random() < 0.5)
     return Case(A, B, C, D, prec, alpha, beta, c_null, name=f"{scenario}/{seed}")
+
+
+# --------------------------------------------------------------------------
+# the batching workload (uneven products)
+# --------------------------------------------------------------------------
+def tucker_case(seed, i, j, k, r, dtype=tc.F32, prec=tc.DEFAULT_PREC):
+    """D[r,j,k] = sum_i A[i,j,k] U[r,i] (a mode-1 Tucker product), dense."""
+    A = make_operand(dtype, [i, j, k], "ijk", seed * 4 + 0)
+    U = make_operand(dtype, [r, i], "ri", seed * 4 + 1)
+    C = make_operand(dtype, [r, j, k], "rjk", seed * 4 + 2)
+    D = make_operand(dtype, [r, j, k], "rjk", seed * 4 + 3)
+    return Case(A, U, C, D, prec, 1.0, 0.0, name=f"tucker{i}x{j}x{k}r{r}")
+
+
+def four_index_case(seed, ext, dtype=tc.F32, prec=tc.DEFAULT_PREC):
+    """configs[0] form D[a,b,c,d] = sum_{e,f} A[a,e,b,f] B[d,f,c,e] with
+    per-label extents ext = (a, b, c, d, e, f)."""
+    a, b, c, d, e, f = ext
+    A = make_operand(dtype, [a, e, b, f], "aebf", seed * 4 + 0)
+    B = make_operand(dtype, [d, f, c, e], "dfce", seed * 4 + 1)
+    C = make_operand(dtype, [a, b, c, d], "abcd", seed * 4 + 2)
+    D = make_operand(dtype, [a, b, c, d], "abcd", seed * 4 + 3)
+    return Case(A, B, C, D, prec, 1.5, 0.5, name=f"c0form{ext}")
+
+
+def batching_mix(seed, count, tucker_range=(4, 64), r_range=(2, 16), c0_range=(6, 24), tucker_frac=0.8,
+                 prec=tc.DEFAULT_PREC, per_index=True):
+    """The uneven batching workload: `tucker_frac` Tucker mode products with
+    i, j, k ~ U{tucker_range} and r ~ U{r_range}, the rest configs[0]-form
+    products with each extent ~ U{c0_range} (drawn per index)."""
+    rng = np.random.default_rng(seed)
+    cases = []
+    for n in range(count):
+        if rng.random() < tucker_frac:
+            i, j, k = (int(x) for x in rng.integers(tucker_range[0], tucker_range[1] + 1, 3))
+            r = int(rng.integers(r_range[0], r_range[1] + 1))
+            cases.append(tucker_case(seed * 100003 + n, i, j, k, r, prec=prec))
+        else:
+            if per_index:
+                ext = tuple(int(x) for x in rng.integers(c0_range[0], c0_range[1] + 1, 6))
+            else:
+                ext = (int(rng.integers(c0_range[0], c0_range[1] + 1)),) * 6
+            cases.append(four_index_case(seed * 100003 + n, ext, prec=prec))
+    return cases
