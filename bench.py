@@ -148,6 +148,127 @@ def cpu_baseline(A, B, D_gpu, sample_j=2):
             "gpu_vs_reference_slice_normwise_err": err}
 
 
+def batching_problems(seed=0, count=8192):
+    """The uneven batching workload, drawn per problem: 80 % Tucker mode
+    products D[r,j,k] = sum_i A[i,j,k] U[r,i] with i,j,k ~ U{4..64},
+    r ~ U{2..16}; 20 % configs[0]-form products with each of the six extents
+    ~ U{6..24} (drawn per index).  Returns (kind, extents) tuples."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        if rng.random() < 0.8:
+            i, j, k = (int(x) for x in rng.integers(4, 65, 3))
+            out.append(("tucker", (i, j, k, int(rng.integers(2, 17)))))
+        else:
+            out.append(("c0", tuple(int(x) for x in rng.integers(6, 25, 6))))
+    return out
+
+
+def batching_bench(lib, torch, steps, prec=tc.DEFAULT_PREC):
+    """one persistent group launch vs one TAPP_execute_product per
+    product, same device operands, same GPU."""
+    probs = batching_problems()
+    L = lambda s: [ord(c) for c in s]  # noqa: E731
+    specs, sizes = [], []
+    flops = bytes_ = ideal = 0.0
+    for kind, e in probs:
+        if kind == "tucker":
+            i, j, k, r = e
+            ops = [([i, j, k], "ijk"), ([r, i], "ri"), ([r, j, k], "rjk"), ([r, j, k], "rjk")]
+            f = 2.0 * r * j * k * i
+            b = 4.0 * (i * j * k + r * i + r * j * k)
+            alpha, beta = 1.0, 0.0
+        else:
+            a, bb, c, d, ee, ff = e
+            ops = [([a, ee, bb, ff], "aebf"), ([d, ff, c, ee], "dfce"), ([a, bb, c, d], "abcd"), ([a, bb, c, d], "abcd")]
+            f = 2.0 * a * bb * c * d * ee * ff
+            b = 4.0 * (a * ee * bb * ff + d * ff * c * ee + 2 * a * bb * c * d)
+            alpha, beta = 1.5, 0.5
+        specs.append((ops, alpha, beta))
+        sizes.append([int(np.prod(x[0])) for x in ops])
+        flops += f
+        bytes_ += b
+        ideal += max(f / (PEAK_FP32_TFLOPS * 1e12), b / (PEAK_HBM_GBS * 1e9))
+    total = sum(sum(x) for x in sizes)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(7)
+    pool = torch.rand(total, generator=g, device="cuda", dtype=torch.float32).mul_(2.0).sub_(1.0)
+    base = pool.data_ptr()
+    off = 0
+    plans, infos, ptrs, scal = [], [], [], []
+    for (ops, alpha, beta), sz in zip(specs, sizes):
+        hs = []
+        p4 = []
+        for (ext, lab), n in zip(ops, sz):
+            st, acc = [], 1
+            for x in ext:
+                st.append(acc)
+                acc *= x
+            hs.append(lib.create_info(tc.F32, ext, st)[1])
+            p4.append(base + 4 * off)
+            off += n
+        _, plan = lib.create_plan([0, 0, 0, 0], hs, [L(x[1]) for x in ops], prec)
+        plans.append(plan)
+        infos.append(hs)
+        ptrs.append(p4)
+        scal.append((np.array([alpha], dtype=np.float32), np.array([beta], dtype=np.float32)))
+    rc, ex = lib.create_executor()
+    lib.attr_set_int(ex, tc.ATTR_TIMING, 1)
+    # one call per product
+    for i in range(len(plans)):
+        rc = lib.execute(plans[i], ex, scal[i][0].ctypes.data, ptrs[i][0], ptrs[i][1], scal[i][1].ctypes.data,
+                         ptrs[i][2], ptrs[i][3])
+        assert rc == 0, lib.explain(rc)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(len(plans)):
+        lib.execute(plans[i], ex, scal[i][0].ctypes.data, ptrs[i][0], ptrs[i][1], scal[i][1].ctypes.data,
+                    ptrs[i][2], ptrs[i][3])
+    torch.cuda.synchronize()
+    loop_s = time.perf_counter() - t0
+    D_loop = [pool[(p[3] - base) // 4:(p[3] - base) // 4 + sz[3]].clone() for p, sz in zip(ptrs, sizes)]
+    # one persistent group
+    t0 = time.perf_counter()
+    rc, grp = lib.create_group(ex, plans, [sc[0].ctypes.data for sc in scal], [p[0] for p in ptrs],
+                               [p[1] for p in ptrs], [sc[1].ctypes.data for sc in scal], [p[2] for p in ptrs],
+                               [p[3] for p in ptrs])
+    create_s = time.perf_counter() - t0
+    assert rc == 0, lib.explain(rc)
+    _, info = lib.group_info(grp)
+    assert lib.execute_group(grp) == 0
+    torch.cuda.synchronize()
+    dev_us = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        assert lib.execute_group(grp) == 0
+        dev_us.append(lib.attr_get_double(ex, tc.ATTR_LAST_DEVICE_US)[1])
+    torch.cuda.synchronize()
+    group_s = (time.perf_counter() - t0) / steps
+    err = 0.0
+    for p, sz, dl in zip(ptrs, sizes, D_loop):
+        dg = pool[(p[3] - base) // 4:(p[3] - base) // 4 + sz[3]]
+        den = dl.abs().max().item()
+        if den > 0:
+            err = max(err, (dg - dl).abs().max().item() / den)
+    lib.destroy_group(grp)
+    lib.destroy_executor(ex)
+    for plan, hs in zip(plans, infos):
+        lib.destroy_plan(plan)
+        for h in hs:
+            lib.destroy_info(h)
+    kernel_s = float(np.mean(dev_us)) * 1e-6
+    return {"workload": "8192 uneven products: 80% Tucker D[r,j,k]=sum_i A[i,j,k]U[r,i] (i,j,k~U{4..64}, "
+                        "r~U{2..16}), 20% configs[0] form (extents ~U{6..24} per index), seed 0",
+            "precision": "FP32" if prec == tc.DEFAULT_PREC else "BF16BF16_ACCUM_F32",
+            "flop": flops, "bytes": bytes_, "group_gflops": flops / group_s / 1e9,
+            "group_ms": group_s * 1e3, "group_kernel_ms": kernel_s * 1e3,
+            "loop_gflops": flops / loop_s / 1e9, "loop_ms": loop_s * 1e3,
+            "speedup_vs_loop": loop_s / group_s, "create_ms": create_s * 1e3,
+            "persistent_products": info[0], "one_by_one_products": info[1], "tiles": info[2],
+            "roofline_frac": ideal / kernel_s, "ideal_ms": ideal * 1e3,
+            "max_rel_err_group_vs_loop": err}
+
+
 def measured_traffic():
     """HBM bytes per launch of the FP32 ladder kernel from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE passes (scaled by the copy-kernel
@@ -168,6 +289,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-forms", action="store_true", help="skip the BF16/F16/split-BF16 forms")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU reference baseline")
+    ap.add_argument("--no-batch", action="store_true", help="skip the uneven-batch (product group) measurement")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (calibrates the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
@@ -219,6 +341,14 @@ def main():
         log("bench: CPU reference baseline on a D-slice")
         cpu = cpu_baseline(A, B, D_fp32)
 
+    batched = {}
+    if not args.no_batch:
+        del A, B, D, D_fp32
+        torch.cuda.empty_cache()
+        log("bench: uneven batch, one persistent launch vs one call per product")
+        batched["fp32"] = batching_bench(lib, torch, max(2, args.steps // 2))
+        batched["bf16"] = batching_bench(lib, torch, max(2, args.steps // 2), prec=tc.BF16BF16_ACCUM_F32)
+
     out = {
         "metric": "GFLOP/s + % MFMA/HBM roofline, split-BF16 vs FP32 GPU; p50/p99 req latency",
         "value": value,
@@ -240,6 +370,7 @@ def main():
                      "kernel": "k_gett_f32<CONTIG_ROW, CONTIG_K> (v_mfma_f32_32x32x2f32)", "kernel_us": dev_us, "kernel_family": kern},
         "cpu_baseline": cpu,
         "forms": forms,
+        "batched": batched,
     }
     print(json.dumps(out))
 

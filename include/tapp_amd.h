@@ -84,6 +84,40 @@ TAPP_EXPORT TAPP_error TAPP_AMD_destroy_product_group(TAPP_AMD_product_group gro
  * one, info[2] tiles of the persistent launch. */
 TAPP_EXPORT TAPP_error TAPP_AMD_group_info(TAPP_AMD_product_group group, int64_t info[3]);
 
+/* --- serving -----------------------------------------------------------------
+ * A request loop with continuous batching on one GPU.  A dispatcher thread
+ * owns `slots` HIP streams; whenever one is idle it admits queued requests
+ * (FIFO) while their staging footprint -- host operands plus pre-pass
+ * buffers; device operands cost nothing -- fits `hbm_budget_bytes`, launches
+ * them as one batch (small FP32 products in one persistent launch, the rest
+ * on the general path) and keeps admitting while batches run on other
+ * streams.  Each request returns a TAPP_status (status.h:9-17) that
+ * TAPP_AMD_status_wait waits on and TAPP_destroy_status releases.  Operands
+ * must stay valid, and plans unchanged, until the request completes; the
+ * plans of queued requests must not be executed elsewhere concurrently. */
+typedef intptr_t TAPP_AMD_server;
+
+enum { TAPP_AMD_SERVER_PAUSED = 1 };   /* create paused: queue and admission only */
+
+TAPP_EXPORT TAPP_error TAPP_AMD_create_server(TAPP_AMD_server* server, TAPP_executor exec, size_t hbm_budget_bytes,
+                                              int slots, int max_batch, int flags);
+/* Validates like TAPP_execute_product; TAPP_AMD_ERR_BUDGET when the request
+ * alone exceeds the budget.  alpha/beta are copied. */
+TAPP_EXPORT TAPP_error TAPP_AMD_server_submit(TAPP_AMD_server server, TAPP_tensor_product plan, const void* alpha,
+                                              const void* A, const void* B, const void* beta, const void* C,
+                                              void* D, TAPP_status* status);
+TAPP_EXPORT TAPP_error TAPP_AMD_status_wait(TAPP_AMD_server server, TAPP_status status, TAPP_error* result);
+/* Removes a still-queued request; TAPP_AMD_ERR_BUSY once it was admitted. */
+TAPP_EXPORT TAPP_error TAPP_AMD_server_cancel(TAPP_AMD_server server, TAPP_status status);
+TAPP_EXPORT TAPP_error TAPP_AMD_server_set_paused(TAPP_AMD_server server, int paused);
+/* Waits until nothing is queued (or the server is paused) and no batch runs. */
+TAPP_EXPORT TAPP_error TAPP_AMD_server_drain(TAPP_AMD_server server);
+/* stats: completed, p50 us, p99 us, mean us, admission stalls, max in flight,
+ * batches, products run in persistent launches, GFLOP/s, queued, budget bytes
+ * in use.  Latency = submit to observed completion. */
+TAPP_EXPORT TAPP_error TAPP_AMD_server_stats(TAPP_AMD_server server, double stats[11], int reset);
+TAPP_EXPORT TAPP_error TAPP_AMD_destroy_server(TAPP_AMD_server server);
+
 /* Text of the last device error of the calling thread (which HIP call
  * failed and why), with TAPP_explain_error's maxlen contract. */
 TAPP_EXPORT size_t TAPP_AMD_last_error_message(char* message, size_t maxlen);

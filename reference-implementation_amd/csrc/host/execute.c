@@ -279,6 +279,15 @@ typedef struct {
 int tk_execute(struct tapp_plan* p, struct tapp_exec* ex, const void* alpha, const void* A, const void* B,
                const void* beta, const void* C, void* D)
 {
+    return tk_execute_on(p, ex, alpha, A, B, beta, C, D, true);
+}
+
+/* With sync == false every step stays ordered on ex->stream and the call
+ * returns after enqueueing: host operands must stay valid, and D is written,
+ * once the stream has reached this point. */
+int tk_execute_on(struct tapp_plan* p, struct tapp_exec* ex, const void* alpha, const void* A, const void* B,
+                  const void* beta, const void* C, void* D, bool sync)
+{
     tk_problem P;
     int rc = tk_plan_problem(p, ex, alpha, beta, C, &P);
     if (rc) return rc;
@@ -447,6 +456,7 @@ int tk_execute(struct tapp_plan* p, struct tapp_exec* ex, const void* alpha, con
         char* hdst = (char*)D + P.lo[3] * (int64_t)es;
         HIPCHK(hipMemcpyAsync(hdst, ws + off_stage[3], span_bytes[3], hipMemcpyDeviceToHost, st));
     }
+    if (!sync) return 0;
     HIPCHK(hipStreamSynchronize(st));
     if (ex->timing) {
         float ms = 0;

@@ -56,8 +56,8 @@ static int by_cost(const void* x, const void* y)
 
 /* Descriptor of one eligible product; returns false when the product needs
  * the general path. */
-static bool describe(struct tapp_plan* plan, struct tapp_exec* ex, const tk_problem* P, const void* A,
-                     const void* B, const void* C, void* D, tk_prob* out, int* rc)
+bool tk_group_describe(struct tapp_plan* plan, struct tapp_exec* ex, const tk_problem* P, const void* A,
+                       const void* B, const void* C, void* D, tk_prob* out, int* rc)
 {
     *rc = 0;
     if (P->ctype != TK_F32 || P->mode == TK_MODE_SPLIT_BF16) return false;
@@ -163,7 +163,7 @@ static int create_group(TAPP_AMD_product_group* group, TAPP_executor exec, int c
         if (rc) { free(list); group_free(g); return rc; }
         if (P.empty) continue;
         tk_prob d;
-        if (describe(p, ex, &P, A[i], B[i], c, D[i], &d, &rc)) {
+        if (tk_group_describe(p, ex, &P, A[i], B[i], c, D[i], &d, &rc)) {
             double per_tile = d.cls == TK_GROUP_THIN ? 256.0 * (double)d.K * (double)(d.S + 1)
                                                      : 64.0 * 64.0 * (double)d.K;
             list[n].p = d;

@@ -111,11 +111,15 @@ void tk_exec_unregister(struct tapp_exec* e);
 /* execution */
 int tk_execute(struct tapp_plan* p, struct tapp_exec* ex, const void* alpha, const void* A,
                const void* B, const void* beta, const void* C, void* D);
+int tk_execute_on(struct tapp_plan* p, struct tapp_exec* ex, const void* alpha, const void* A,
+                  const void* B, const void* beta, const void* C, void* D, bool sync);
 void tk_plan_cache_free(struct tapp_plan_cache* c);
 int tk_problem_tables(struct tapp_plan* plan, const tk_problem* P, int device, const int64_t** mA,
                       const int64_t** kA, const int64_t** lA, const int64_t** nB, const int64_t** kB,
                       const int64_t** lB, const int64_t** mD, const int64_t** nD, const int64_t** lD,
                       const int64_t** mC, const int64_t** nC, const int64_t** lC);
 void tk_problem_host_k(struct tapp_plan* plan, const int64_t** kA, const int64_t** kB);
+bool tk_group_describe(struct tapp_plan* plan, struct tapp_exec* ex, const tk_problem* P, const void* A,
+                       const void* B, const void* C, void* D, tk_prob* out, int* rc);
 
 #endif /* TAPP_AMD_INTERNAL_H_ */

@@ -1,0 +1,572 @@
+/*
+ * Serving loop: continuous batching of TAPP_execute_product requests on one
+ * GPU.
+ *
+ *   client threads --submit--> FIFO queue --dispatcher thread--> stream slots
+ *
+ * The dispatcher owns `slots` device contexts (stream + workspace each).
+ * Whenever a slot is idle it admits requests from the head of the queue while
+ * their staging footprint fits the HBM budget, launches them as one batch on
+ * that slot's stream and goes on: requests arriving while batches run are
+ * admitted as soon as a slot and budget free up, and batches on different
+ * slots overlap on the device.  Inside a batch, FP32 products on device
+ * operands that fit the persistent kernel (small and thin products) run as
+ * one persistent group launch -- a work queue over all their tiles, mixed
+ * shape classes in one kernel -- and every other request (large GETTs,
+ * complex, FP64, split-BF16, unary labels, host operands) runs through the
+ * TAPP_execute_product path on the same stream without waiting.  Completion is
+ * tracked with one HIP event per batch.
+ *
+ * Footprint of a request = bytes of its host operands (staged into HBM) plus
+ * the pre-pass buffers it needs; device operands and resident tensors cost
+ * nothing.  A request larger than the whole budget is rejected at submit
+ * (TAPP_AMD_ERR_BUDGET); a request that does not fit now waits (a stall).
+ *
+ * A request's completion is a TAPP_status object (status.h:9-17: "status
+ * objects are created by execution functions"), waited on with
+ * TAPP_AMD_status_wait and released with TAPP_destroy_status.
+ */
+#define _POSIX_C_SOURCE 200809L
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+#include "internal.h"
+
+#define SRV_MAX_SLOTS 16
+#define STATUS_MAGIC 0x53544154u
+
+typedef struct srv_request {
+    uint32_t magic;
+    int refs;                   /* client + server */
+    int state;                  /* 0 queued, 1 running, 2 done, 3 cancelled */
+    int rc;
+    struct tapp_plan* plan;
+    unsigned char alpha[16], beta[16];
+    const void *A, *B, *C;
+    void* D;
+    size_t footprint;
+    double t_submit, t_start, t_done;
+    struct srv_request* next;   /* queue / batch list */
+} srv_request;
+
+typedef struct {
+    struct tapp_exec ex;        /* private context: stream, workspace, events */
+    bool busy;
+    srv_request* batch;         /* requests of the batch in flight */
+    size_t batch_bytes;
+    hipEvent_t done;
+    /* descriptor staging for the persistent launch */
+    tk_prob* hprobs;
+    int64_t* hend;
+    size_t cap;
+    void* dmem;
+    size_t dcap;
+} srv_slot;
+
+struct tapp_server {
+    uint32_t magic;
+    int device;
+    int nslots, max_batch;
+    size_t budget, in_use;
+    int64_t large_flop;         /* products above this run on the general GETT path */
+    int cus;
+    srv_slot slot[SRV_MAX_SLOTS];
+    pthread_t thread;
+    bool running, stop, paused;
+    pthread_mutex_t mu;
+    pthread_cond_t cv_work, cv_done;
+    srv_request *head, *tail;
+    int queued;
+    /* statistics */
+    int64_t completed, stalls, batches, group_products, max_inflight, inflight;
+    double* lat;                /* completed latencies, microseconds */
+    size_t nlat, caplat;
+    double flop_done, t_first, t_last;
+};
+
+static double now_us(void)
+{
+    struct timespec t;
+    clock_gettime(CLOCK_MONOTONIC, &t);
+    return (double)t.tv_sec * 1e6 + (double)t.tv_nsec * 1e-3;
+}
+
+static bool on_device(const void* p)
+{
+    if (!p) return false;
+    hipPointerAttribute_t a;
+    hipError_t e = hipPointerGetAttributes(&a, p);
+    (void)hipGetLastError();
+    return e == hipSuccess && (a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged);
+}
+
+static void release(srv_request* r)
+{
+    if (__atomic_sub_fetch(&r->refs, 1, __ATOMIC_ACQ_REL) == 0) { r->magic = 0; free(r); }
+}
+
+/* --- execution of one batch on one slot ----------------------------------- */
+
+static int64_t product_flop(const tk_problem* P)
+{
+    return 2 * P->nL * P->nM * P->nN * (P->nK > 0 ? P->nK : 1);
+}
+
+static int ensure_desc(srv_slot* s, size_t n)
+{
+    if (n <= s->cap) return 0;
+    size_t cap = n < 64 ? 64 : n * 2;
+    tk_prob* hp = NULL;
+    int64_t* he = NULL;
+    if (hipHostMalloc((void**)&hp, sizeof(tk_prob) * cap, 0) != hipSuccess ||
+        hipHostMalloc((void**)&he, sizeof(int64_t) * cap, 0) != hipSuccess) {
+        (void)hipGetLastError();
+        if (hp) hipHostFree(hp);
+        return TAPP_AMD_ERR_NOMEM;
+    }
+    if (s->hprobs) { hipHostFree(s->hprobs); hipHostFree(s->hend); }
+    s->hprobs = hp; s->hend = he; s->cap = cap;
+    size_t bytes = (sizeof(tk_prob) + sizeof(int64_t)) * cap + 512;
+    if (s->dmem) hipFree(s->dmem);
+    s->dmem = NULL;
+    if (hipMalloc(&s->dmem, bytes) != hipSuccess) { (void)hipGetLastError(); s->cap = 0; return TAPP_AMD_ERR_NOMEM; }
+    s->dcap = bytes;
+    return 0;
+}
+
+static void launch_batch(struct tapp_server* sv, srv_slot* s)
+{
+    struct tapp_exec* ex = &s->ex;
+    int n = 0;
+    int64_t total = 0;
+    size_t count = 0;
+    for (srv_request* r = s->batch; r; r = r->next) count++;
+    if (ensure_desc(s, count)) {
+        for (srv_request* r = s->batch; r; r = r->next) r->rc = TAPP_AMD_ERR_NOMEM;
+        hipEventRecord(s->done, ex->stream);
+        return;
+    }
+    for (srv_request* r = s->batch; r; r = r->next) {
+        r->t_start = now_us();
+        tk_problem P;
+        int rc = tk_plan_problem(r->plan, ex, r->alpha, r->beta, r->C, &P);
+        if (rc) { r->rc = rc; continue; }
+        if (P.empty) { r->rc = 0; continue; }
+        sv->flop_done += (double)product_flop(&P);
+        tk_prob d;
+        if (product_flop(&P) <= sv->large_flop &&
+            tk_group_describe(r->plan, ex, &P, r->A, r->B, r->C, r->D, &d, &rc)) {
+            s->hprobs[n] = d;
+            total += d.L * d.tiles_s * d.tiles_g;
+            s->hend[n] = total;
+            n++;
+            r->rc = 0;
+        } else if (rc) {
+            r->rc = rc;
+        } else {
+            r->rc = tk_execute_on(r->plan, ex, r->alpha, r->A, r->B, r->beta, r->C, r->D, false);
+        }
+    }
+    if (n > 0) {
+        tk_prob* dprobs = (tk_prob*)s->dmem;
+        int64_t* dend = (int64_t*)((char*)s->dmem + sizeof(tk_prob) * s->cap);
+        unsigned long long* counter =
+            (unsigned long long*)((char*)s->dmem + ((sizeof(tk_prob) + sizeof(int64_t)) * s->cap + 255) / 256 * 256);
+        int e = (int)hipMemcpyAsync(dprobs, s->hprobs, sizeof(tk_prob) * (size_t)n, hipMemcpyHostToDevice, ex->stream);
+        if (!e) e = (int)hipMemcpyAsync(dend, s->hend, sizeof(int64_t) * (size_t)n, hipMemcpyHostToDevice, ex->stream);
+        int64_t nb = (int64_t)sv->cus * 2;
+        if (!e) e = tk_launch_group(dprobs, dend, n, total, counter, (int)(total < nb ? total : nb), ex->stream);
+        if (e)
+            for (srv_request* r = s->batch; r; r = r->next)
+                if (!r->rc) r->rc = TAPP_AMD_ERR_DEVICE;
+        sv->group_products += n;
+    }
+    hipEventRecord(s->done, ex->stream);
+}
+
+/* --- dispatcher ----------------------------------------------------------- */
+
+static void finish_slot(struct tapp_server* sv, srv_slot* s)
+{
+    double t = now_us();
+    srv_request* r = s->batch;
+    while (r) {
+        srv_request* nx = r->next;
+        r->t_done = t;
+        r->state = 2;
+        if (sv->nlat == sv->caplat) {
+            size_t cap = sv->caplat ? sv->caplat * 2 : 4096;
+            double* l = realloc(sv->lat, sizeof(double) * cap);
+            if (l) { sv->lat = l; sv->caplat = cap; }
+        }
+        if (sv->nlat < sv->caplat) sv->lat[sv->nlat++] = r->t_done - r->t_submit;
+        sv->completed++;
+        sv->inflight--;
+        sv->t_last = t;
+        r->next = NULL;
+        release(r);
+        r = nx;
+    }
+    sv->in_use -= s->batch_bytes;
+    s->batch = NULL;
+    s->batch_bytes = 0;
+    s->busy = false;
+}
+
+static void* dispatcher(void* arg)
+{
+    struct tapp_server* sv = arg;
+    hipSetDevice(sv->device);
+    pthread_mutex_lock(&sv->mu);
+    for (;;) {
+        bool progressed = false;
+        /* completions */
+        for (int i = 0; i < sv->nslots; i++) {
+            srv_slot* s = &sv->slot[i];
+            if (s->busy && hipEventQuery(s->done) == hipSuccess) {
+                finish_slot(sv, s);
+                progressed = true;
+            }
+        }
+        if (progressed) pthread_cond_broadcast(&sv->cv_done);
+        /* admissions */
+        if (!sv->paused) {
+            for (int i = 0; i < sv->nslots && sv->head; i++) {
+                srv_slot* s = &sv->slot[i];
+                if (s->busy) continue;
+                srv_request *first = NULL, *last = NULL;
+                int taken = 0;
+                size_t bytes = 0;
+                while (sv->head && taken < sv->max_batch) {
+                    srv_request* r = sv->head;
+                    if (r->state == 3) {          /* cancelled while queued */
+                        sv->head = r->next;
+                        if (!sv->head) sv->tail = NULL;
+                        sv->queued--;
+                        r->next = NULL;
+                        release(r);
+                        continue;
+                    }
+                    if (sv->in_use + bytes + r->footprint > sv->budget) {
+                        if (taken == 0) sv->stalls++;
+                        break;
+                    }
+                    sv->head = r->next;
+                    if (!sv->head) sv->tail = NULL;
+                    sv->queued--;
+                    r->next = NULL;
+                    r->state = 1;
+                    if (last) last->next = r; else first = r;
+                    last = r;
+                    bytes += r->footprint;
+                    taken++;
+                }
+                if (!taken) break;
+                s->batch = first;
+                s->batch_bytes = bytes;
+                sv->in_use += bytes;
+                sv->inflight += taken;
+                if (sv->inflight > sv->max_inflight) sv->max_inflight = sv->inflight;
+                sv->batches++;
+                s->busy = true;
+                pthread_mutex_unlock(&sv->mu);
+                launch_batch(sv, s);
+                pthread_mutex_lock(&sv->mu);
+                progressed = true;
+            }
+        }
+        bool any_busy = false;
+        for (int i = 0; i < sv->nslots; i++) any_busy |= sv->slot[i].busy;
+        if (sv->stop && !sv->head && !any_busy) break;
+        if (!progressed) {
+            if (any_busy) {
+                /* poll completions at a short interval */
+                pthread_mutex_unlock(&sv->mu);
+                struct timespec ts = {0, 5000};
+                nanosleep(&ts, NULL);
+                pthread_mutex_lock(&sv->mu);
+            } else {
+                pthread_cond_wait(&sv->cv_work, &sv->mu);
+            }
+        }
+    }
+    pthread_mutex_unlock(&sv->mu);
+    return NULL;
+}
+
+/* --- public API ------------------------------------------------------------- */
+
+TAPP_error TAPP_AMD_create_server(TAPP_AMD_server* server, TAPP_executor exec, size_t hbm_budget_bytes, int slots,
+                                  int max_batch, int flags)
+{
+    if (!server || slots <= 0 || slots > SRV_MAX_SLOTS || max_batch <= 0) return TAPP_AMD_ERR_INVALID;
+    struct tapp_exec* base = tk_exec_lookup(exec);
+    if (!base) return 33;
+    struct tapp_server* sv = calloc(1, sizeof *sv);
+    if (!sv) return TAPP_AMD_ERR_NOMEM;
+    sv->magic = TAPP_AMD_EXEC_MAGIC;
+    sv->device = base->device;
+    sv->nslots = slots;
+    sv->max_batch = max_batch;
+    sv->budget = hbm_budget_bytes;
+    sv->large_flop = (int64_t)1 << 31;
+    sv->paused = (flags & TAPP_AMD_SERVER_PAUSED) != 0;
+    if (!base->stream || hipDeviceGetAttribute(&sv->cus, hipDeviceAttributeMultiprocessorCount, base->device) !=
+                             hipSuccess || sv->cus <= 0) {
+        (void)hipGetLastError();
+        sv->cus = 256;
+    }
+    pthread_mutex_init(&sv->mu, NULL);
+    pthread_cond_init(&sv->cv_work, NULL);
+    pthread_cond_init(&sv->cv_done, NULL);
+    for (int i = 0; i < slots; i++) {
+        struct tapp_exec* e = &sv->slot[i].ex;
+        e->kind = 1;
+        e->magic = TAPP_AMD_EXEC_MAGIC;
+        e->device = base->device;
+        e->split_bf16 = base->split_bf16;
+        e->force_kernel = base->force_kernel;
+        if (base->stream) {
+            if (hipSetDevice(base->device) != hipSuccess ||
+                hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess ||
+                hipEventCreateWithFlags(&sv->slot[i].done, hipEventDisableTiming) != hipSuccess ||
+                hipEventCreate(&e->ev0) != hipSuccess || hipEventCreate(&e->ev1) != hipSuccess) {
+                (void)hipGetLastError();
+                TAPP_AMD_destroy_server((TAPP_AMD_server)sv);
+                return TAPP_AMD_ERR_DEVICE;
+            }
+            e->own_stream = true;
+        }
+    }
+    if (!base->stream) sv->paused = true;   /* no device: queue-only (admission logic) */
+    if (pthread_create(&sv->thread, NULL, dispatcher, sv) != 0) {
+        TAPP_AMD_destroy_server((TAPP_AMD_server)sv);
+        return TAPP_AMD_ERR_NOMEM;
+    }
+    sv->running = true;
+    *server = (TAPP_AMD_server)sv;
+    return 0;
+}
+
+/* staging footprint of a request: host operands over their address spans,
+ * pre-pass buffers for unary labels or storage conversion */
+static int footprint(struct tapp_plan* p, const void* alpha, const void* A, const void* B, const void* beta,
+                     const void* C, void* D, size_t* out)
+{
+    struct tapp_exec probe;
+    memset(&probe, 0, sizeof probe);
+    probe.kind = 1;
+    tk_problem P;
+    int rc = tk_plan_problem(p, &probe, alpha, beta, C, &P);
+    if (rc) return rc;
+    size_t f = 0;
+    const void* ops[4] = {A, B, P.beta_zero ? NULL : C, D};
+    for (int x = 0; x < 4; x++)
+        if (ops[x] && !on_device(ops[x]) && P.hi[x] >= P.lo[x])
+            f += (size_t)(P.hi[x] - P.lo[x] + 1) * tk_type_size(P.type[x]);
+    size_t cs = (P.ctype == TK_F32) ? 4 : (P.ctype == TK_C64) ? 16 : 8;
+    if (P.nUA > 1 || P.type[0] != (P.ctype == TK_F32 ? 0 : P.ctype == TK_F64 ? 1 : P.ctype == TK_C32 ? 2 : 3))
+        f += (size_t)(P.nM * P.nK * P.nL) * cs;
+    if (P.nUB > 1 || P.type[1] != (P.ctype == TK_F32 ? 0 : P.ctype == TK_F64 ? 1 : P.ctype == TK_C32 ? 2 : 3))
+        f += (size_t)(P.nN * P.nK * P.nL) * cs;
+    *out = f;
+    return 0;
+}
+
+TAPP_error TAPP_AMD_server_submit(TAPP_AMD_server server, TAPP_tensor_product plan, const void* alpha,
+                                  const void* A, const void* B, const void* beta, const void* C, void* D,
+                                  TAPP_status* status)
+{
+    struct tapp_server* sv = (struct tapp_server*)server;
+    if (!sv || sv->magic != TAPP_AMD_EXEC_MAGIC || !plan || !status) return TAPP_AMD_ERR_INVALID;
+    struct tapp_plan* p = (struct tapp_plan*)plan;
+    size_t f = 0;
+    int rc = footprint(p, alpha, A, B, beta, C, D, &f);
+    if (rc) return rc;
+    if (f > sv->budget) return TAPP_AMD_ERR_BUDGET;
+    srv_request* r = calloc(1, sizeof *r);
+    if (!r) return TAPP_AMD_ERR_NOMEM;
+    r->magic = STATUS_MAGIC;
+    r->refs = 2;
+    r->plan = p;
+    const struct tapp_info* dinfo = (const struct tapp_info*)p->info[3];
+    size_t ss = tk_type_size(dinfo->type);
+    if (alpha && ss) memcpy(r->alpha, alpha, ss);
+    if (beta && ss) memcpy(r->beta, beta, ss);
+    r->A = A; r->B = B; r->C = C; r->D = D;
+    r->footprint = f;
+    r->t_submit = now_us();
+    pthread_mutex_lock(&sv->mu);
+    if (!sv->t_first) sv->t_first = r->t_submit;
+    if (sv->tail) sv->tail->next = r; else sv->head = r;
+    sv->tail = r;
+    sv->queued++;
+    pthread_cond_signal(&sv->cv_work);
+    pthread_mutex_unlock(&sv->mu);
+    *status = (TAPP_status)r;
+    return 0;
+}
+
+/* Waits for the request behind `status`; *result receives its TAPP code. */
+TAPP_error TAPP_AMD_status_wait(TAPP_AMD_server server, TAPP_status status, TAPP_error* result)
+{
+    struct tapp_server* sv = (struct tapp_server*)server;
+    srv_request* r = (srv_request*)status;
+    if (!sv || !r || r->magic != STATUS_MAGIC) return TAPP_AMD_ERR_INVALID;
+    pthread_mutex_lock(&sv->mu);
+    while (r->state < 2) pthread_cond_wait(&sv->cv_done, &sv->mu);
+    if (result) *result = r->state == 3 ? TAPP_AMD_ERR_INVALID : r->rc;
+    pthread_mutex_unlock(&sv->mu);
+    return 0;
+}
+
+/* Cancels a request still in the queue; returns TAPP_AMD_ERR_BUSY when it has
+ * already been admitted. */
+TAPP_error TAPP_AMD_server_cancel(TAPP_AMD_server server, TAPP_status status)
+{
+    struct tapp_server* sv = (struct tapp_server*)server;
+    srv_request* r = (srv_request*)status;
+    if (!sv || !r || r->magic != STATUS_MAGIC) return TAPP_AMD_ERR_INVALID;
+    pthread_mutex_lock(&sv->mu);
+    int rc = 0;
+    if (r->state == 0) {
+        r->state = 3;
+        pthread_cond_broadcast(&sv->cv_done);
+    } else if (r->state == 1) {
+        rc = TAPP_AMD_ERR_BUSY;
+    }
+    pthread_mutex_unlock(&sv->mu);
+    return rc;
+}
+
+TAPP_error TAPP_destroy_status(TAPP_status status)
+{
+    srv_request* r = (srv_request*)status;
+    if (!r) return 0;
+    if (r->magic != STATUS_MAGIC) return TAPP_AMD_ERR_INVALID;
+    /* the server's reference is dropped when the request completes or its
+     * cancellation is observed by the dispatcher; callers destroy statuses of
+     * finished or cancelled requests */
+    release(r);
+    return 0;
+}
+
+TAPP_error TAPP_AMD_server_set_paused(TAPP_AMD_server server, int paused)
+{
+    struct tapp_server* sv = (struct tapp_server*)server;
+    if (!sv || sv->magic != TAPP_AMD_EXEC_MAGIC) return TAPP_AMD_ERR_INVALID;
+    pthread_mutex_lock(&sv->mu);
+    bool has_device = sv->slot[0].ex.stream != NULL;
+    sv->paused = paused || !has_device;
+    pthread_cond_signal(&sv->cv_work);
+    pthread_mutex_unlock(&sv->mu);
+    return 0;
+}
+
+TAPP_error TAPP_AMD_server_drain(TAPP_AMD_server server)
+{
+    struct tapp_server* sv = (struct tapp_server*)server;
+    if (!sv || sv->magic != TAPP_AMD_EXEC_MAGIC) return TAPP_AMD_ERR_INVALID;
+    pthread_mutex_lock(&sv->mu);
+    for (;;) {
+        bool busy = false;
+        for (int i = 0; i < sv->nslots; i++) busy |= sv->slot[i].busy;
+        if (!busy && (!sv->head || sv->paused)) break;
+        pthread_cond_wait(&sv->cv_done, &sv->mu);
+    }
+    pthread_mutex_unlock(&sv->mu);
+    return 0;
+}
+
+static int cmp_d(const void* a, const void* b)
+{
+    double x = *(const double*)a, y = *(const double*)b;
+    return x < y ? -1 : x > y;
+}
+
+/* stats: [0] completed, [1] p50 us, [2] p99 us, [3] mean us, [4] stalls,
+ * [5] max in flight, [6] batches, [7] products in persistent launches,
+ * [8] GFLOP/s over first submit .. last completion, [9] queued now,
+ * [10] budget bytes in use now.  reset != 0 clears the counters. */
+TAPP_error TAPP_AMD_server_stats(TAPP_AMD_server server, double stats[11], int reset)
+{
+    struct tapp_server* sv = (struct tapp_server*)server;
+    if (!sv || sv->magic != TAPP_AMD_EXEC_MAGIC || !stats) return TAPP_AMD_ERR_INVALID;
+    pthread_mutex_lock(&sv->mu);
+    memset(stats, 0, sizeof(double) * 11);
+    stats[0] = (double)sv->completed;
+    if (sv->nlat) {
+        double* c = malloc(sizeof(double) * sv->nlat);
+        if (c) {
+            memcpy(c, sv->lat, sizeof(double) * sv->nlat);
+            qsort(c, sv->nlat, sizeof(double), cmp_d);
+            stats[1] = c[(size_t)floor(0.50 * (double)(sv->nlat - 1))];
+            stats[2] = c[(size_t)floor(0.99 * (double)(sv->nlat - 1))];
+            double s = 0;
+            for (size_t i = 0; i < sv->nlat; i++) s += c[i];
+            stats[3] = s / (double)sv->nlat;
+            free(c);
+        }
+    }
+    stats[4] = (double)sv->stalls;
+    stats[5] = (double)sv->max_inflight;
+    stats[6] = (double)sv->batches;
+    stats[7] = (double)sv->group_products;
+    if (sv->t_last > sv->t_first) stats[8] = sv->flop_done / ((sv->t_last - sv->t_first) * 1e-6) / 1e9;
+    stats[9] = (double)sv->queued;
+    stats[10] = (double)sv->in_use;
+    if (reset) {
+        sv->completed = sv->stalls = sv->batches = sv->group_products = sv->max_inflight = 0;
+        sv->nlat = 0;
+        sv->flop_done = 0;
+        sv->t_first = sv->t_last = 0;
+    }
+    pthread_mutex_unlock(&sv->mu);
+    return 0;
+}
+
+TAPP_error TAPP_AMD_destroy_server(TAPP_AMD_server server)
+{
+    struct tapp_server* sv = (struct tapp_server*)server;
+    if (!sv) return 0;
+    if (sv->running) {
+        pthread_mutex_lock(&sv->mu);
+        sv->stop = true;
+        if (sv->paused) {
+            /* drop what never ran */
+            while (sv->head) {
+                srv_request* r = sv->head;
+                sv->head = r->next;
+                r->state = 3;
+                r->next = NULL;
+                release(r);
+            }
+            sv->tail = NULL;
+        }
+        pthread_cond_signal(&sv->cv_work);
+        pthread_cond_broadcast(&sv->cv_done);
+        pthread_mutex_unlock(&sv->mu);
+        pthread_join(sv->thread, NULL);
+    }
+    for (int i = 0; i < sv->nslots; i++) {
+        srv_slot* s = &sv->slot[i];
+        if (s->ex.stream) hipStreamSynchronize(s->ex.stream);
+        if (s->ex.ws) hipFree(s->ex.ws);
+        if (s->dmem) hipFree(s->dmem);
+        if (s->hprobs) { hipHostFree(s->hprobs); hipHostFree(s->hend); }
+        if (s->done) hipEventDestroy(s->done);
+        if (s->ex.ev0) hipEventDestroy(s->ex.ev0);
+        if (s->ex.ev1) hipEventDestroy(s->ex.ev1);
+        if (s->ex.own_stream && s->ex.stream) hipStreamDestroy(s->ex.stream);
+    }
+    free(sv->lat);
+    pthread_mutex_destroy(&sv->mu);
+    pthread_cond_destroy(&sv->cv_work);
+    pthread_cond_destroy(&sv->cv_done);
+    sv->magic = 0;
+    free(sv);
+    return 0;
+}

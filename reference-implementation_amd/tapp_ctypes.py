@@ -12,6 +12,7 @@ import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "libtapp_amd.so")
+TRACE_PATH = os.path.join(HERE, "lib", "libtapp_trace.so")
 
 # api/include/tapp/datatype.h:17-95, product.h:17-21
 F32, F64, C32, C64, F16, BF16 = 0, 1, 2, 3, 4, 5
@@ -68,7 +69,9 @@ class TappLib:
         "TAPP_execute_batched_product", "TAPP_destroy_status", "TAPP_attr_set", "TAPP_attr_get",
         "TAPP_attr_clear", "TAPP_AMD_version", "TAPP_AMD_plan_info", "TAPP_AMD_last_error_message",
         "TAPP_AMD_create_product_group", "TAPP_AMD_execute_product_group", "TAPP_AMD_destroy_product_group",
-        "TAPP_AMD_group_info",
+        "TAPP_AMD_group_info", "TAPP_AMD_create_server", "TAPP_AMD_server_submit", "TAPP_AMD_status_wait",
+        "TAPP_AMD_server_cancel", "TAPP_AMD_server_set_paused", "TAPP_AMD_server_drain", "TAPP_AMD_server_stats",
+        "TAPP_AMD_destroy_server",
     )
 
     def __init__(self, path=None):
@@ -125,6 +128,15 @@ class TappLib:
             L.TAPP_AMD_execute_product_group.argtypes = [_H]
             L.TAPP_AMD_destroy_product_group.argtypes = [_H]
             L.TAPP_AMD_group_info.argtypes = [_H, _i64p]
+            L.TAPP_AMD_create_server.argtypes = [ctypes.POINTER(_H), _H, ctypes.c_size_t, ctypes.c_int, ctypes.c_int,
+                                                 ctypes.c_int]
+            L.TAPP_AMD_server_submit.argtypes = [_H, _H] + [ctypes.c_void_p] * 6 + [ctypes.POINTER(_H)]
+            L.TAPP_AMD_status_wait.argtypes = [_H, _H, ctypes.POINTER(ctypes.c_int)]
+            L.TAPP_AMD_server_cancel.argtypes = [_H, _H]
+            L.TAPP_AMD_server_set_paused.argtypes = [_H, ctypes.c_int]
+            L.TAPP_AMD_server_drain.argtypes = [_H]
+            L.TAPP_AMD_server_stats.argtypes = [_H, ctypes.POINTER(ctypes.c_double), ctypes.c_int]
+            L.TAPP_AMD_destroy_server.argtypes = [_H]
 
     # --- objects ---------------------------------------------------------
     def create_info(self, dtype, extents, strides):
@@ -220,8 +232,67 @@ class TappLib:
         rc = self.lib.TAPP_AMD_group_info(g, info)
         return rc, list(info)
 
+    # --- serving -------------------------------------------------------------
+    SERVER_PAUSED = 1
+    STAT_NAMES = ("completed", "p50_us", "p99_us", "mean_us", "stalls", "max_inflight", "batches",
+                  "persistent_products", "gflops", "queued", "budget_in_use")
+
+    def create_server(self, exec_, budget_bytes, slots=4, max_batch=64, flags=0):
+        h = _H()
+        rc = self.lib.TAPP_AMD_create_server(ctypes.byref(h), exec_, int(budget_bytes), slots, max_batch, flags)
+        return rc, h.value
+
+    def submit(self, srv, plan, alpha_ptr, a_ptr, b_ptr, beta_ptr, c_ptr, d_ptr):
+        st = _H()
+        rc = self.lib.TAPP_AMD_server_submit(srv, plan, alpha_ptr, a_ptr, b_ptr, beta_ptr, c_ptr, d_ptr,
+                                             ctypes.byref(st))
+        return rc, st.value
+
+    def wait(self, srv, status):
+        r = ctypes.c_int(-1)
+        rc = self.lib.TAPP_AMD_status_wait(srv, status, ctypes.byref(r))
+        return rc, r.value
+
+    def cancel(self, srv, status):
+        return self.lib.TAPP_AMD_server_cancel(srv, status)
+
+    def destroy_status(self, status):
+        return self.lib.TAPP_destroy_status(status)
+
+    def set_paused(self, srv, paused):
+        return self.lib.TAPP_AMD_server_set_paused(srv, int(paused))
+
+    def drain(self, srv):
+        return self.lib.TAPP_AMD_server_drain(srv)
+
+    def server_stats(self, srv, reset=False):
+        v = (ctypes.c_double * 11)()
+        rc = self.lib.TAPP_AMD_server_stats(srv, v, int(reset))
+        return rc, dict(zip(self.STAT_NAMES, list(v)))
+
+    def destroy_server(self, srv):
+        return self.lib.TAPP_AMD_destroy_server(srv)
+
     def plan_info(self, plan, exec_, beta_ptr, c_ptr):
         sizes = (ctypes.c_int64 * 6)()
         ct = ctypes.c_int(-1)
         rc = self.lib.TAPP_AMD_plan_info(plan, exec_, beta_ptr, c_ptr, sizes, ctypes.byref(ct))
         return rc, list(sizes), ct.value
+
+
+def trace_replay(lib, srv, plans, alphas, a_ptrs, b_ptrs, betas, c_ptrs, d_ptrs, arrival_us):
+    """Open-loop submit of request i at arrival_us[i] (microseconds after the
+    start) through the serving loop, waiting for all.  Returns (failed
+    submits, per-request codes, total submit lateness in microseconds)."""
+    _single_hip_runtime()
+    tr = ctypes.CDLL(TRACE_PATH, mode=os.RTLD_LOCAL)
+    n = len(plans)
+    VP = ctypes.c_void_p * max(1, n)
+    tr.tapp_trace_replay.argtypes = [_H, ctypes.c_int, ctypes.POINTER(_H)] + [ctypes.POINTER(ctypes.c_void_p)] * 6 + [
+        ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double)]
+    rcs = (ctypes.c_int * max(1, n))()
+    late = ctypes.c_double(0)
+    arr = (ctypes.c_double * max(1, n))(*arrival_us)
+    failed = tr.tapp_trace_replay(srv, n, (_H * max(1, n))(*plans), VP(*alphas), VP(*a_ptrs), VP(*b_ptrs),
+                                  VP(*betas), VP(*c_ptrs), VP(*d_ptrs), arr, rcs, ctypes.byref(late))
+    return failed, list(rcs)[:n], late.value

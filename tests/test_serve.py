@@ -1,0 +1,125 @@
+"""Serving loop: admission by HBM budget, cancellation, continuous batching,
+and every result the same alone and in the mix."""
+import numpy as np
+import pytest
+
+import tapp_cases as T
+
+tc = T.tc
+
+
+def _plan(lib, case):
+    infos = [lib.create_info(o.type, o.extents, o.strides)[1] for o in case.operands()]
+    _, plan = lib.create_plan([o.op for o in case.operands()], infos, [o.idx for o in case.operands()], case.prec)
+    return plan, infos
+
+
+def _free(lib, plan, infos):
+    lib.destroy_plan(plan)
+    for h in infos:
+        lib.destroy_info(h)
+
+
+# --- queue and admission logic (no kernel runs: the server is paused) ----------
+def test_budget_rejection_and_cancel_on_paused_server(lib):
+    rc, ex = lib.create_executor()
+    assert rc == 0
+    case = T.tucker_case(1, 8, 9, 10, 4)            # host operands: footprint = their spans
+    footprint = sum(len(o.buf) * o.buf.itemsize for o in (case.A, case.B, case.D))
+    plan, infos = _plan(lib, case)
+    al = T.scalar_bytes(case.alpha, case.D.type)
+    be = T.scalar_bytes(case.beta, case.D.type)
+    # budget below one request: rejected at submit, nothing queued
+    rc, small = lib.create_server(ex, footprint - 1, 2, 8, lib.SERVER_PAUSED)
+    assert rc == 0
+    rc, st = lib.submit(small, plan, al.ctypes.data, case.A.ptr(), case.B.ptr(), be.ctypes.data, None, case.D.ptr())
+    assert rc == 106
+    assert lib.server_stats(small)[1]["queued"] == 0
+    lib.destroy_server(small)
+    # enough budget: queued while paused; cancel one; validation still applies
+    rc, srv = lib.create_server(ex, 10 * footprint, 2, 8, lib.SERVER_PAUSED)
+    sts = []
+    for _ in range(3):
+        rc, st = lib.submit(srv, plan, al.ctypes.data, case.A.ptr(), case.B.ptr(), be.ctypes.data, None,
+                            case.D.ptr())
+        assert rc == 0
+        sts.append(st)
+    assert lib.server_stats(srv)[1]["queued"] == 3
+    assert lib.cancel(srv, sts[1]) == 0
+    assert lib.wait(srv, sts[1]) == (0, 104)          # a cancelled request reports an invalid-argument code
+    lib.destroy_status(sts[1])
+    bad = T.random_case(4, "contraction", tc.F32)
+    bad.D.idx[0] = 999
+    bplan, binfos = _plan(lib, bad)
+    rc, _ = lib.submit(srv, bplan, al.ctypes.data, bad.A.ptr(), bad.B.ptr(), be.ctypes.data, None, bad.D.ptr())
+    assert rc == 4
+    lib.destroy_server(srv)                             # drops the never-run requests
+    for st in (sts[0], sts[2]):
+        lib.destroy_status(st)
+    _free(lib, plan, infos)
+    _free(lib, bplan, binfos)
+    lib.destroy_executor(ex)
+
+
+# --- on the GPU ----------------------------------------------------------------
+@pytest.mark.gpu
+def test_results_same_alone_and_in_the_mix(lib, gpu_exec, oracle):
+    from test_group import DevCase, mixed_cases
+    from test_gpu_parity import assert_parity
+    cases = mixed_cases(31) + [T.configs0_case(2)]
+    dcs = [DevCase(lib, c) for c in cases]
+    alone = []
+    for d in dcs:
+        assert d.run_single(gpu_exec) == 0
+        alone.append(d.d().copy())
+        d.reset_d()
+    rc, srv = lib.create_server(gpu_exec, 1 << 30, 4, 16)
+    assert rc == 0
+    # all submitted back to back: they land in several batches on 4 streams
+    failed, rcs, late = tc.trace_replay(lib, srv, [d.plan for d in dcs], [d.al.ctypes.data for d in dcs],
+                                        [d.ptr["A"] for d in dcs], [d.ptr["B"] for d in dcs],
+                                        [d.be.ctypes.data for d in dcs], [d.c_ptr() for d in dcs],
+                                        [d.ptr["D"] for d in dcs], [0.0] * len(dcs))
+    assert failed == 0 and all(r == 0 for r in rcs), rcs
+    rc, stats = lib.server_stats(srv)
+    assert stats["completed"] == len(dcs) and stats["persistent_products"] > 0, stats
+    for d, a in zip(dcs, alone):
+        _, want = oracle.run(d.case)
+        assert_parity(d.d(), want, d.case)
+        assert_parity(d.d(), a, d.case)
+    lib.destroy_server(srv)
+    for d in dcs:
+        d.close()
+
+
+@pytest.mark.gpu
+def test_budget_stalls_then_completes(lib, gpu_exec, oracle):
+    """Host-operand requests whose staging footprints do not all fit the
+    budget at once: they wait (stalls) and all complete correctly."""
+    from test_gpu_parity import assert_parity
+    cases = [T.tucker_case(100 + i, 24, 20, 16, 6) for i in range(8)]
+    fp = max(sum(len(o.buf) * o.buf.itemsize for o in (c.A, c.B, c.D)) for c in cases)
+    rc, srv = lib.create_server(gpu_exec, int(2.5 * fp), 2, 4, lib.SERVER_PAUSED)
+    assert rc == 0
+    keep, sts = [], []
+    for c in cases:
+        c2 = c.copy()
+        plan, infos = _plan(lib, c2)
+        al = T.scalar_bytes(c2.alpha, c2.D.type)
+        be = T.scalar_bytes(c2.beta, c2.D.type)
+        rc, st = lib.submit(srv, plan, al.ctypes.data, c2.A.ptr(), c2.B.ptr(), be.ctypes.data, None, c2.D.ptr())
+        assert rc == 0
+        keep.append((c2, plan, infos, al, be))
+        sts.append(st)
+    assert lib.server_stats(srv)[1]["queued"] == 8
+    assert lib.set_paused(srv, 0) == 0          # all 8 queued: at most 2 fit the budget at a time
+    for st in sts:
+        assert lib.wait(srv, st) == (0, 0)
+        lib.destroy_status(st)
+    rc, stats = lib.server_stats(srv)
+    assert stats["completed"] == 8 and stats["stalls"] > 0, stats
+    for c2, plan, infos, al, be in keep:
+        _, want = oracle.run(c2)
+        assert_parity(c2.D.buf, want, c2)
+        _free(lib, plan, infos)
+    lib.destroy_server(srv)
