@@ -269,6 +269,103 @@ def batching_bench(lib, torch, steps, prec=tc.DEFAULT_PREC):
             "max_rel_err_group_vs_loop": err}
 
 
+def serving_bench(lib, torch, rates, nreq, split=False, seed=3):
+    """open-loop Poisson arrivals into the serving loop.
+    Requests: 5 % large ladder-form products (virtual extent 64:
+    D[a,b,i,j] = sum A[a,b,c,d] B[c,i,d,j], a..d = 64, i,j = 32), 20 %
+    configs[0] products (extents 16), 75 % drawn from 256 configs[3]-mix shapes.
+    70 % of requests use the shape's device-resident A; 30 % bring A from
+    pinned host memory (staged by the server, counted against the 64 GB
+    budget).  B, C and D are device tensors (D from a ring per shape)."""
+    rng = np.random.default_rng(seed)
+    L = lambda t: [ord(c) for c in t]  # noqa: E731
+
+    def dense(ext):
+        st, acc = [], 1
+        for x in ext:
+            st.append(acc)
+            acc *= x
+        return st, acc
+
+    shapes = [("ladder64", [([64, 64, 64, 64], "abcd"), ([64, 32, 64, 32], "cidj"), ([64, 64, 32, 32], "abij"),
+                            ([64, 64, 32, 32], "abij")], 1.0, 0.0, 2.0 * 64 ** 4 * 32 * 32),
+              ("c0", [([16] * 4, "aebf"), ([16] * 4, "dfce"), ([16] * 4, "abcd"), ([16] * 4, "abcd")], 1.5, 0.5,
+               2.0 * 16 ** 6)]
+    for kind, e in batching_problems(seed=1, count=256):
+        if kind == "tucker":
+            i, j, k, r = e
+            shapes.append(("tucker", [([i, j, k], "ijk"), ([r, i], "ri"), ([r, j, k], "rjk"), ([r, j, k], "rjk")],
+                           1.0, 0.0, 2.0 * r * j * k * i))
+        else:
+            a, b, c, d, ee, ff = e
+            shapes.append(("c0mix", [([a, ee, b, ff], "aebf"), ([d, ff, c, ee], "dfce"), ([a, b, c, d], "abcd"),
+                                     ([a, b, c, d], "abcd")], 1.5, 0.5, 2.0 * a * b * c * d * ee * ff))
+    ring = 16
+    cat = []
+    for name, ops, alpha, beta, fl in shapes:
+        infos = []
+        for ext, lab in ops:
+            st, _ = dense(ext)
+            infos.append(lib.create_info(tc.F32, ext, st)[1])
+        _, plan = lib.create_plan([0, 0, 0, 0], infos, [L(x[1]) for x in ops], tc.DEFAULT_PREC)
+        n = [dense(x[0])[1] for x in ops]
+        A = torch.rand(n[0], device="cuda") * 2 - 1
+        Ah = (torch.rand(n[0]) * 2 - 1).pin_memory()
+        B = torch.rand(n[1], device="cuda") * 2 - 1
+        C = torch.rand(n[2], device="cuda") * 2 - 1
+        Ds = torch.zeros((ring, n[3]), device="cuda")
+        cat.append({"plan": plan, "infos": infos, "A": A, "Ah": Ah, "B": B, "C": C, "D": Ds, "flop": fl,
+                    "al": np.array([alpha], np.float32), "be": np.array([beta], np.float32), "next": 0})
+    torch.cuda.synchronize()
+    rc, ex = lib.create_executor()
+    if split:
+        lib.attr_set_int(ex, tc.ATTR_SPLIT_BF16, 1)
+    rc, srv = lib.create_server(ex, 64 << 30, 4, 64)
+    assert rc == 0
+
+    def draw(n):
+        u = rng.random(n)
+        idx = np.where(u < 0.05, 0, np.where(u < 0.25, 1, 2 + rng.integers(0, 256, n)))
+        host_a = rng.random(n) >= 0.7
+        return idx, host_a
+
+    results = []
+    for rate in rates:
+        idx, host_a = draw(nreq)
+        arrivals = np.cumsum(rng.exponential(1e6 / rate, nreq))
+        plans, als, As, Bs, bes, Cs, Ds = [], [], [], [], [], [], []
+        flop = 0.0
+        for i, h in zip(idx, host_a):
+            c = cat[int(i)]
+            slot = c["next"] % ring
+            c["next"] += 1
+            plans.append(c["plan"]); als.append(c["al"].ctypes.data); bes.append(c["be"].ctypes.data)
+            As.append(c["Ah"].data_ptr() if h else c["A"].data_ptr())
+            Bs.append(c["B"].data_ptr()); Cs.append(c["C"].data_ptr()); Ds.append(c["D"][slot].data_ptr())
+            flop += c["flop"]
+        lib.server_stats(srv, reset=True)
+        t0 = time.perf_counter()
+        failed, rcs, late = tc.trace_replay(lib, srv, plans, als, As, Bs, bes, Cs, Ds, list(arrivals))
+        wall = time.perf_counter() - t0
+        _, st = lib.server_stats(srv)
+        assert failed == 0 and all(r == 0 for r in rcs), (failed, [r for r in rcs if r][:5])
+        results.append({"rate_req_s": rate, "requests": nreq, "p50_us": st["p50_us"], "p99_us": st["p99_us"],
+                        "mean_us": st["mean_us"], "sustained_gflops": st["gflops"],
+                        "offered_gflops": flop / (arrivals[-1] * 1e-6) / 1e9, "admission_stalls": st["stalls"],
+                        "max_in_flight": st["max_inflight"], "batches": st["batches"],
+                        "persistent_products": st["persistent_products"], "wall_s": wall,
+                        "mean_submit_late_us": late / nreq})
+        log(f"bench: serving {'split-BF16' if split else 'FP32'} {rate} req/s: p50 {st['p50_us']:.0f} us "
+            f"p99 {st['p99_us']:.0f} us, {st['gflops']:.0f} GFLOP/s, stalls {st['stalls']:.0f}")
+    lib.destroy_server(srv)
+    lib.destroy_executor(ex)
+    for c in cat:
+        lib.destroy_plan(c["plan"])
+        for h in c["infos"]:
+            lib.destroy_info(h)
+    return results
+
+
 def measured_traffic():
     """HBM bytes per launch of the FP32 ladder kernel from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE passes (scaled by the copy-kernel
@@ -290,6 +387,9 @@ def main():
     ap.add_argument("--no-forms", action="store_true", help="skip the BF16/F16/split-BF16 forms")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU reference baseline")
     ap.add_argument("--no-batch", action="store_true", help="skip the uneven-batch (product group) measurement")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop sweep")
+    ap.add_argument("--serve-requests", type=int, default=2000,
+                    help="requests per arrival rate in the serving sweep (the full workload is 200000)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (calibrates the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
@@ -349,6 +449,14 @@ def main():
         batched["fp32"] = batching_bench(lib, torch, max(2, args.steps // 2))
         batched["bf16"] = batching_bench(lib, torch, max(2, args.steps // 2), prec=tc.BF16BF16_ACCUM_F32)
 
+    serving = {}
+    if not args.no_serve:
+        torch.cuda.empty_cache()
+        log("bench: serving loop, open-loop Poisson arrivals")
+        rates = [1000, 5000, 20000, 50000]
+        serving["fp32"] = serving_bench(lib, torch, rates, args.serve_requests)
+        serving["split_bf16"] = serving_bench(lib, torch, [5000], args.serve_requests, split=True)
+
     out = {
         "metric": "GFLOP/s + % MFMA/HBM roofline, split-BF16 vs FP32 GPU; p50/p99 req latency",
         "value": value,
@@ -371,6 +479,7 @@ def main():
         "cpu_baseline": cpu,
         "forms": forms,
         "batched": batched,
+        "serving": serving,
     }
     print(json.dumps(out))
 
