@@ -7,10 +7,13 @@
 //             Hadamard-only products where an MFMA tile would be mostly empty.
 //  k_splitk   one workgroup per output element, K split over 256 lanes and
 //             reduced through LDS: very few outputs with a long K.
+//  k_gett_f32 FP32 data, load modes as template parameters; PLAIN runs
+//             v_mfma_f32_32x32x2f32, BF16 / F16 / SPLIT_BF16 convert in
+//             registers and run v_mfma_f32_32x32x16_{bf16,f16}.
 //  k_mfma     LDS-tiled MFMA GETT, 256 lanes = 2x2 waves, double-buffered
 //             LDS, register prefetch of the next K tile, XCD-aware tile order.
-//             FP32: v_mfma_f32_32x32x2f32; FP64: v_mfma_f64_16x16x4f64;
-//             complex: four real MFMAs per step on split re/im LDS planes.
+//             FP64: v_mfma_f64_16x16x4f64; complex: four real MFMAs per step
+//             on split re/im LDS planes; also the generic FP32 fallback.
 //             The MFMA's first operand is the B tile and its second the A
 //             tile, so output lanes run along M, the unit-stride direction of
 //             D when the host has oriented the problem that way.
@@ -333,25 +336,11 @@ __global__ __launch_bounds__(256) void k_mfma(tk_gett g)
 }
 
 // ---------------------------------------------------------------------------
-// k_mfma16: FP32 data through 16-bit MFMA (gfx950 v_mfma_f32_32x32x16_*)
-//   TK_MODE_BF16       x -> bf16(x)
-//   TK_MODE_F16        x -> f16(x)
-//   TK_MODE_SPLIT_BF16 x -> hi = bf16(x), lo = bf16(x - hi);
-//                      acc += hi*hi + hi*lo + lo*hi (lo*lo dropped)
-// Conversion happens once per element when a tile is written to LDS.  LDS
-// holds each tile as [row][k] so a lane reads its 8 consecutive k values of
-// one row with a single 16-byte LDS read, which is the MFMA operand layout
-// (lane l: row l % 32, k = 8 * (l / 32) .. +7).  Accumulation is FP32.
+// 16-bit MFMA (gfx950 v_mfma_f32_32x32x16_{bf16,f16}): lane l holds row
+// l % 32 and k = 8 * (l / 32) .. +7 of each operand; FP32 accumulate.
 // ---------------------------------------------------------------------------
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
-
-__device__ __forceinline__ uint16_t bf16_bits(float x) { return float_to_bf16_bits(x); }
-__device__ __forceinline__ uint16_t f16_bits(float x)
-{
-    _Float16 h = (_Float16)x;
-    return __builtin_bit_cast(uint16_t, h);
-}
 
 template <int MODE>
 __device__ __forceinline__ f32x16 mfma16(uint4 b, uint4 a, f32x16 c)
@@ -361,132 +350,6 @@ __device__ __forceinline__ f32x16 mfma16(uint4 b, uint4 a, f32x16 c)
                                                       0, 0, 0);
     return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, b), __builtin_bit_cast(bf16x8, a), c,
                                                    0, 0, 0);
-}
-
-template <int MODE, int BR, int BK, int LDK>
-__device__ __forceinline__ void store16(const TileLoader<float, false, BR, BK>& t, uint16_t* hi, uint16_t* lo,
-                                        int mode)
-{
-    typedef TileLoader<float, false, BR, BK> TL;
-#pragma unroll
-    for (int i = 0; i < TL::NS; ++i) {
-        const int s = threadIdx.x + 256 * i;
-        int rr, kk;
-        if (mode == TK_CONTIG_K) { kk = (s % (BK / TL::VE)) * TL::VE; rr = s / (BK / TL::VE); }
-        else { rr = (s % (BR / TL::VE)) * TL::VE; kk = s / (BR / TL::VE); }
-#pragma unroll
-        for (int j = 0; j < TL::VE; ++j) {
-            const int r = mode == TK_CONTIG_K ? rr : rr + j;
-            const int k = mode == TK_CONTIG_K ? kk + j : kk;
-            const float x = t.v[i][j];
-            if (MODE == TK_MODE_F16) {
-                hi[r * LDK + k] = f16_bits(x);
-            } else {
-                const uint16_t h = bf16_bits(x);
-                hi[r * LDK + k] = h;
-                if (MODE == TK_MODE_SPLIT_BF16) lo[r * LDK + k] = bf16_bits(x - bf16_bits_to_float(h));
-            }
-        }
-    }
-}
-
-template <int MODE, int FM, int FN, int BK>
-__global__ __launch_bounds__(256) void k_mfma16(tk_gett g)
-{
-    constexpr int S = 32;
-    constexpr int BM = 2 * FM * S, BN = 2 * FN * S;
-    constexpr int PL = MODE == TK_MODE_SPLIT_BF16 ? 2 : 1;
-    constexpr int LDK = BK + 8;                 // row pitch in 16-bit words (16-byte pad)
-    static_assert(BK % 16 == 0, "BK must hold whole 32x32x16 steps");
-    __shared__ __attribute__((aligned(16))) uint16_t As[2][PL][BM * LDK];
-    __shared__ __attribute__((aligned(16))) uint16_t Bs[2][PL][BN * LDK];
-
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int wm = wave & 1, wn = wave >> 1;
-    const int il = lane & 31, kl = lane >> 5;
-
-    const int64_t ntm = (g.M + BM - 1) / BM, ntn = (g.N + BN - 1) / BN;
-    const int64_t tile = xcd_tile(blockIdx.x, ntm * ntn);
-    const int64_t m0 = (tile / ntn) * BM, n0 = (tile % ntn) * BN;
-
-    for (int64_t l = blockIdx.y; l < g.L; l += gridDim.y) {
-        const float* Ab = (const float*)g.A + g.lA[l];
-        const float* Bb = (const float*)g.B + g.lB[l];
-        f32x16 acc[FN][FM];
-#pragma unroll
-        for (int fn = 0; fn < FN; ++fn)
-#pragma unroll
-            for (int fm = 0; fm < FM; ++fm) acc[fn][fm] = f32x16{};
-
-        TileLoader<float, false, BM, BK> la;
-        TileLoader<float, false, BN, BK> lb;
-        const int64_t nk = (g.K + BK - 1) / BK;
-        if (nk > 0) {
-            la.load(Ab, g.mA, g.kA, m0, 0, g.M, g.K, g.a_load, g.a_vec);
-            lb.load(Bb, g.nB, g.kB, n0, 0, g.N, g.K, g.b_load, g.b_vec);
-            store16<MODE, BM, BK, LDK>(la, As[0][0], As[0][PL - 1], g.a_load);
-            store16<MODE, BN, BK, LDK>(lb, Bs[0][0], Bs[0][PL - 1], g.b_load);
-        }
-        __syncthreads();
-        for (int64_t kt = 0; kt < nk; ++kt) {
-            const int cur = kt & 1;
-            if (kt + 1 < nk) {
-                la.load(Ab, g.mA, g.kA, m0, (kt + 1) * BK, g.M, g.K, g.a_load, g.a_vec);
-                lb.load(Bb, g.nB, g.kB, n0, (kt + 1) * BK, g.N, g.K, g.b_load, g.b_vec);
-            }
-#pragma unroll
-            for (int kk = 0; kk < BK; kk += 16) {
-                uint4 a[PL][FM], b[PL][FN];
-#pragma unroll
-                for (int p = 0; p < PL; ++p) {
-#pragma unroll
-                    for (int fm = 0; fm < FM; ++fm)
-                        a[p][fm] = *reinterpret_cast<const uint4*>(
-                            &As[cur][p][(wm * FM * S + fm * S + il) * LDK + kk + 8 * kl]);
-#pragma unroll
-                    for (int fn = 0; fn < FN; ++fn)
-                        b[p][fn] = *reinterpret_cast<const uint4*>(
-                            &Bs[cur][p][(wn * FN * S + fn * S + il) * LDK + kk + 8 * kl]);
-                }
-#pragma unroll
-                for (int fn = 0; fn < FN; ++fn)
-#pragma unroll
-                    for (int fm = 0; fm < FM; ++fm) {
-                        acc[fn][fm] = mfma16<MODE>(b[0][fn], a[0][fm], acc[fn][fm]);
-                        if (MODE == TK_MODE_SPLIT_BF16) {
-                            acc[fn][fm] = mfma16<MODE>(b[0][fn], a[PL - 1][fm], acc[fn][fm]);
-                            acc[fn][fm] = mfma16<MODE>(b[PL - 1][fn], a[0][fm], acc[fn][fm]);
-                        }
-                    }
-            }
-            if (kt + 1 < nk) {
-                store16<MODE, BM, BK, LDK>(la, As[cur ^ 1][0], As[cur ^ 1][PL - 1], g.a_load);
-                store16<MODE, BN, BK, LDK>(lb, Bs[cur ^ 1][0], Bs[cur ^ 1][PL - 1], g.b_load);
-            }
-            __syncthreads();
-        }
-#pragma unroll
-        for (int fn = 0; fn < FN; ++fn)
-#pragma unroll
-            for (int fm = 0; fm < FM; ++fm) {
-                const int64_t m = m0 + wm * FM * S + fm * S + il;
-#pragma unroll
-                for (int r = 0; r < 16; ++r) {
-                    const int64_t n = n0 + wn * FN * S + fn * S + 8 * (r >> 2) + 4 * kl + (r & 3);
-                    if (m < g.M && n < g.N) epilogue<float, false>(g, l, m, n, acc[fn][fm][r], 0.0f);
-                }
-            }
-    }
-}
-
-template <int MODE, int FM, int FN, int BK>
-static hipError_t launch_mfma16(const tk_gett& g, hipStream_t st)
-{
-    constexpr int BM = 2 * FM * 32, BN = 2 * FN * 32;
-    const int64_t tiles = ((g.M + BM - 1) / BM) * ((g.N + BN - 1) / BN);
-    const int64_t ly = g.L < 65535 ? g.L : 65535;
-    hipLaunchKernelGGL((k_mfma16<MODE, FM, FN, BK>), dim3((unsigned)tiles, (unsigned)ly), dim3(256), 0, st, g);
-    return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------
@@ -596,7 +459,30 @@ struct F32Tile {
     }
 };
 
-template <int AL, int BL>
+// 8 consecutive k values of one row of an LDS tile ([k][row] layout, pitch LD)
+// as a 16-bit MFMA operand; SPLIT also returns the bfloat16 remainder.
+template <int MODE, int LD>
+__device__ __forceinline__ void operand16(const float* t, int col, uint4& hi, uint4& lo)
+{
+    if (MODE == TK_MODE_F16) {
+        f16x8 h;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) h[e] = (_Float16)t[e * LD + col];
+        hi = __builtin_bit_cast(uint4, h);
+    } else {
+        bf16x8 h, l;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const float x = t[e * LD + col];
+            h[e] = (__bf16)x;                                  // v_cvt_pk_bf16_f32, round to nearest even
+            if (MODE == TK_MODE_SPLIT_BF16) l[e] = (__bf16)(x - (float)h[e]);
+        }
+        hi = __builtin_bit_cast(uint4, h);
+        if (MODE == TK_MODE_SPLIT_BF16) lo = __builtin_bit_cast(uint4, l);
+    }
+}
+
+template <int AL, int BL, int MODE>
 __global__ __launch_bounds__(256, 2) void k_gett_f32(tk_gett g)
 {
     constexpr int S = 32, FM = 2, FN = 2, BM = 128, BN = 128, BK = 32;
@@ -648,18 +534,42 @@ __global__ __launch_bounds__(256, 2) void k_gett_f32(tk_gett g)
             if (kt + 1 < nk) fetch(kt + 1);
             const float* as = As[cur];
             const float* bs = Bs[cur];
+            if (MODE == TK_MODE_PLAIN) {
 #pragma unroll
-            for (int kk = 0; kk < BK; kk += 2) {
-                float a[FM], b[FN];
+                for (int kk = 0; kk < BK; kk += 2) {
+                    float a[FM], b[FN];
 #pragma unroll
-                for (int fm = 0; fm < FM; ++fm) a[fm] = as[(kk + kl) * LDA + wm * FM * S + fm * S + il];
+                    for (int fm = 0; fm < FM; ++fm) a[fm] = as[(kk + kl) * LDA + wm * FM * S + fm * S + il];
 #pragma unroll
-                for (int fn = 0; fn < FN; ++fn) b[fn] = bs[(kk + kl) * LDB + wn * FN * S + fn * S + il];
+                    for (int fn = 0; fn < FN; ++fn) b[fn] = bs[(kk + kl) * LDB + wn * FN * S + fn * S + il];
 #pragma unroll
-                for (int fn = 0; fn < FN; ++fn)
+                    for (int fn = 0; fn < FN; ++fn)
+#pragma unroll
+                        for (int fm = 0; fm < FM; ++fm)
+                            acc[fn][fm] = __builtin_amdgcn_mfma_f32_32x32x2f32(b[fn], a[fm], acc[fn][fm], 0, 0, 0);
+                }
+            } else {
+                // 16-bit MFMA: lane holds k = 8 * kl .. 8 * kl + 7 of its row
+#pragma unroll
+                for (int kk = 0; kk < BK; kk += 16) {
+                    uint4 ah[FM], al[FM], bh[FN], bl[FN];
 #pragma unroll
                     for (int fm = 0; fm < FM; ++fm)
-                        acc[fn][fm] = __builtin_amdgcn_mfma_f32_32x32x2f32(b[fn], a[fm], acc[fn][fm], 0, 0, 0);
+                        operand16<MODE, LDA>(as + (kk + 8 * kl) * LDA, wm * FM * S + fm * S + il, ah[fm], al[fm]);
+#pragma unroll
+                    for (int fn = 0; fn < FN; ++fn)
+                        operand16<MODE, LDB>(bs + (kk + 8 * kl) * LDB, wn * FN * S + fn * S + il, bh[fn], bl[fn]);
+#pragma unroll
+                    for (int fn = 0; fn < FN; ++fn)
+#pragma unroll
+                        for (int fm = 0; fm < FM; ++fm) {
+                            acc[fn][fm] = mfma16<MODE>(bh[fn], ah[fm], acc[fn][fm]);
+                            if (MODE == TK_MODE_SPLIT_BF16) {
+                                acc[fn][fm] = mfma16<MODE>(bh[fn], al[fm], acc[fn][fm]);
+                                acc[fn][fm] = mfma16<MODE>(bl[fn], ah[fm], acc[fn][fm]);
+                            }
+                        }
+                }
             }
             if (kt + 1 < nk) {
                 la.template store<LDA>(As[cur ^ 1]);
@@ -668,7 +578,7 @@ __global__ __launch_bounds__(256, 2) void k_gett_f32(tk_gett g)
             __syncthreads();
         }
 
-        const bool plain = g.beta_zero;
+        const bool plain = g.beta_zero && !g.round_C;
         const float alpha = (float)g.alpha[0];
         const int64_t dl = g.lD[l];
 #pragma unroll
@@ -694,7 +604,13 @@ static hipError_t launch_gett_f32_modes(const tk_gett& g, hipStream_t st)
 {
     const int64_t tiles = ((g.M + 127) / 128) * ((g.N + 127) / 128);
     const int64_t ly = g.L < 65535 ? g.L : 65535;
-    hipLaunchKernelGGL((k_gett_f32<AL, BL>), dim3((unsigned)tiles, (unsigned)ly), dim3(256), 0, st, g);
+    const dim3 grid((unsigned)tiles, (unsigned)ly), block(256);
+    switch (g.mode) {
+    case TK_MODE_PLAIN: hipLaunchKernelGGL((k_gett_f32<AL, BL, TK_MODE_PLAIN>), grid, block, 0, st, g); break;
+    case TK_MODE_BF16: hipLaunchKernelGGL((k_gett_f32<AL, BL, TK_MODE_BF16>), grid, block, 0, st, g); break;
+    case TK_MODE_F16: hipLaunchKernelGGL((k_gett_f32<AL, BL, TK_MODE_F16>), grid, block, 0, st, g); break;
+    default: hipLaunchKernelGGL((k_gett_f32<AL, BL, TK_MODE_SPLIT_BF16>), grid, block, 0, st, g); break;
+    }
     return hipGetLastError();
 }
 
@@ -762,9 +678,6 @@ extern "C" int tk_launch_gett(const tk_gett* gp, void* stream)
     case TK_F32:
         if (kern == TK_KERNEL_SIMPLE) return launch_simple<float, false>(g, st);
         if (kern == TK_KERNEL_SPLITK) return launch_splitk<float, false>(g, st);
-        if (g.mode == TK_MODE_BF16) return launch_mfma16<TK_MODE_BF16, 2, 2, 32>(g, st);
-        if (g.mode == TK_MODE_F16) return launch_mfma16<TK_MODE_F16, 2, 2, 32>(g, st);
-        if (g.mode == TK_MODE_SPLIT_BF16) return launch_mfma16<TK_MODE_SPLIT_BF16, 2, 2, 16>(g, st);
         return launch_gett_f32(g, st);
     case TK_F64:
         if (kern == TK_KERNEL_SIMPLE) return launch_simple<double, false>(g, st);
