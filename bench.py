@@ -366,6 +366,76 @@ def serving_bench(lib, torch, rates, nreq, split=False, seed=3):
     return results
 
 
+def config_cases(lib, torch, steps):
+    """Per-call device time and rate of configs[0] (FP32, extents 16) and
+    configs[1] (C64 with a diagonal, unary labels, conj(B), a negative and
+    padded strides, TAPP_F64F64_ACCUM_F64) on device operands."""
+    L = lambda t: [ord(c) for c in t]  # noqa: E731
+    out = {}
+
+    def span(ext, st):
+        lo = sum(min(0, (e - 1) * s) for e, s in zip(ext, st))
+        hi = sum(max(0, (e - 1) * s) for e, s in zip(ext, st))
+        return -lo, hi - lo + 1
+
+    def run(name, specs, prec, alpha, beta, ctype, flop, bytes_):
+        dt = torch.float32 if ctype == tc.F32 else torch.complex128
+        infos, bufs, ptrs = [], [], []
+        for ext, st, lab, dtype in specs:
+            infos.append(lib.create_info(dtype, ext, st)[1])
+            off, n = span(ext, st)
+            t = (torch.rand(n, device="cuda", dtype=torch.float64) * 2 - 1)
+            if dt == torch.complex128:
+                t = torch.complex(t, torch.rand(n, device="cuda", dtype=torch.float64) * 2 - 1)
+            else:
+                t = t.float()
+            bufs.append(t)
+            ptrs.append(t.data_ptr() + off * t.element_size())
+        ops = [0, 1 if name == "configs1" else 0, 0, 0]
+        _, plan = lib.create_plan(ops, infos, [L(x[2]) for x in specs], prec)
+        npd = np.complex128 if ctype == tc.C64 else np.float32
+        al = np.array([alpha], dtype=npd)
+        be = np.array([beta], dtype=npd)
+        _, ex = lib.create_executor()
+        lib.attr_set_int(ex, tc.ATTR_TIMING, 1)
+        rc = lib.execute(plan, ex, al.ctypes.data, ptrs[0], ptrs[1], be.ctypes.data, ptrs[2], ptrs[3])
+        assert rc == 0, lib.explain(rc)
+        torch.cuda.synchronize()
+        dev = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            lib.execute(plan, ex, al.ctypes.data, ptrs[0], ptrs[1], be.ctypes.data, ptrs[2], ptrs[3])
+            dev.append(lib.attr_get_double(ex, tc.ATTR_LAST_DEVICE_US)[1])
+        wall = (time.perf_counter() - t0) / steps
+        d_us = float(np.mean(dev))
+        lib.destroy_executor(ex)
+        lib.destroy_plan(plan)
+        for h in infos:
+            lib.destroy_info(h)
+        ideal = max(flop / (PEAK_FP32_TFLOPS * 1e12 if ctype == tc.F32 else 78.6e12), bytes_ / (PEAK_HBM_GBS * 1e9))
+        out[name] = {"call_us": wall * 1e6, "device_us": d_us, "gflops": flop / (d_us * 1e-6) / 1e9,
+                     "algorithmic_bytes": bytes_, "achieved_gbs": bytes_ / (d_us * 1e-6) / 1e9,
+                     "roofline_frac": ideal / (d_us * 1e-6), "ideal_us": ideal * 1e6}
+
+    e = [16] * 4
+    dense = [1, 16, 256, 4096]
+    run("configs0", [(e, dense, "aebf", tc.F32), (e, dense, "dfce", tc.F32), (e, dense, "abcd", tc.F32),
+                     (e, dense, "abcd", tc.F32)], tc.DEFAULT_PREC, 1.5, 0.5, tc.F32,
+        2.0 * 16 ** 6, 4.0 * 4 * 16 ** 4)
+    a, b, c, h, i, u, v = 40, 40, 24, 32, 32, 8, 6
+    sA = [-1, 40, 40 * 24, 40 * 24 * 24, 40 * 24 * 24 * 32]
+    sB, acc = [], 1
+    for x in (h, b, c, i, v):
+        sB.append(acc)
+        acc = acc * x + 1
+    sD = [1, a, a * b, a * b * h]
+    run("configs1", [([a, c, c, h, u], sA, "acchu", tc.C64), ([h, b, c, i, v], sB, "hbciv", tc.C64),
+                     ([a, b, h, i], sD, "abhi", tc.C64), ([a, b, h, i], sD, "abhi", tc.C64)],
+        tc.F64F64_ACCUM_F64, 0.7 - 0.3j, 1.1 + 0.2j, tc.C64,
+        8.0 * h * a * (b * i) * c, 16.0 * (a * c * h * u + h * b * c * i * v + 2 * a * b * h * i))
+    return out
+
+
 def measured_traffic():
     """HBM bytes per launch of the FP32 ladder kernel from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE passes (scaled by the copy-kernel
@@ -441,6 +511,9 @@ def main():
         log("bench: CPU reference baseline on a D-slice")
         cpu = cpu_baseline(A, B, D_fp32)
 
+    log("bench: configs[0] and configs[1] per call")
+    others = config_cases(lib, torch, max(5, args.steps))
+
     batched = {}
     if not args.no_batch:
         del A, B, D, D_fp32
@@ -478,6 +551,7 @@ def main():
                      "kernel": "k_gett_f32<CONTIG_ROW, CONTIG_K> (v_mfma_f32_32x32x2f32)", "kernel_us": dev_us, "kernel_family": kern},
         "cpu_baseline": cpu,
         "forms": forms,
+        "configs": others,
         "batched": batched,
         "serving": serving,
     }

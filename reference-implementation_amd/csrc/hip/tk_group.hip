@@ -1,18 +1,21 @@
 // Persistent launch over many uneven FP32 products (gfx950).
 //
 // One grid of `nblocks` workgroups; each takes the next tile index from an
-// atomic counter, finds its product by binary search over the inclusive tile
-// prefix, runs the tile and goes back for another until the counter passes
+// atomic counter, finds its product through the tile -> product map (or a
+// binary search over the inclusive tile prefix when no map is given), runs
+// the tile and goes back for another until the counter passes
 // the total.  Every workgroup reaches that exit.  Products are sorted by cost
 // on the host, so long tiles start first.
 //
-//  thin tile  (S <= 16): 256 columns g, all s.  The S x 64 chunk of X and the
-//             64 x 256 chunk of Y go through LDS (Y loaded along whichever of
+//  thin tile  (S <= 16): 256 columns g, all s.  The S x 32 chunk of X and the
+//             32 x 256 chunk of Y go through LDS (Y loaded along whichever of
 //             k or g has unit stride, so global reads are coalesced); each lane
 //             then owns one column g and accumulates S outputs with FMA.  This
 //             is the HBM-streaming shape of Tucker mode products.
-//  mfma tile  64 x 64 x 16 per step, 4 waves each one 32 x 32 block of
-//             v_mfma_f32_32x32x2f32; operands through LDS with table loads.
+//  mfma tile  64 x 64 x 32 per step, 4 waves each one 32 x 32 block of
+//             v_mfma_f32_32x32x2f32; operands through LDS; row offsets in
+//             registers and K offsets staged in LDS per 256-wide chunk, so a
+//             step's loads are one memory round trip, issued a step ahead.
 #include "tk_device.hpp"
 
 namespace tk {
@@ -20,10 +23,35 @@ namespace tk {
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 constexpr int GT = 256;                   // thin tile columns
-constexpr int KC = 64;                    // thin K chunk
+constexpr int KC = 32;                    // thin K chunk
 constexpr int YP = GT + 1;                // LDS pitch of the Y chunk
 constexpr int SMAX = 16;
-constexpr int MT = 64, MK = 16, MP = MT + 4;
+constexpr int MT = 64, MK = 32, MP = MT + 4;
+constexpr int KO = 256;                   // K offsets staged in LDS per chunk (mfma tile)
+
+// Pointers read from a descriptor in memory are generic to the compiler, and
+// generic (FLAT) loads wait conservatively; the operands and tables all live
+// in global memory, so they are accessed through address-space-1 pointers.
+#define TK_GLOBAL __attribute__((address_space(1)))
+typedef const float TK_GLOBAL* gf;
+typedef float TK_GLOBAL* gfw;
+typedef const int64_t TK_GLOBAL* gi;
+
+struct GView {
+    int64_t S, G, K;
+    gf X, Y, C;
+    gfw D;
+    gi sX, kX, lX, gY, kY, lY, sD, gD, lD, sC, gC, lC;
+    float alpha, beta;
+    int beta_zero, round, round_C, y_kfast;
+    __device__ __forceinline__ explicit GView(const tk_prob& p)
+        : S(p.S), G(p.G), K(p.K), X((gf)p.X), Y((gf)p.Y), C((gf)p.C), D((gfw)p.D), sX((gi)p.sX), kX((gi)p.kX),
+          lX((gi)p.lX), gY((gi)p.gY), kY((gi)p.kY), lY((gi)p.lY), sD((gi)p.sD), gD((gi)p.gD), lD((gi)p.lD),
+          sC((gi)p.sC), gC((gi)p.gC), lC((gi)p.lC), alpha(p.alpha), beta(p.beta), beta_zero(p.beta_zero),
+          round(p.round), round_C(p.round_C), y_kfast(p.y_kfast)
+    {
+    }
+};
 
 __device__ __forceinline__ float rnd_in(float x, int mode)
 {
@@ -32,7 +60,7 @@ __device__ __forceinline__ float rnd_in(float x, int mode)
     return x;
 }
 
-__device__ __forceinline__ void group_epilogue(const tk_prob& p, int64_t l, int64_t s, int64_t g, float acc)
+__device__ __forceinline__ void group_epilogue(const GView& p, int64_t l, int64_t s, int64_t g, float acc)
 {
     float out = p.alpha * acc;
     if (!p.beta_zero) {
@@ -44,7 +72,7 @@ __device__ __forceinline__ void group_epilogue(const tk_prob& p, int64_t l, int6
 }
 
 template <int SB>
-__device__ __forceinline__ void thin_tile(const tk_prob& p, int64_t l, int64_t g0, float* lds)
+__device__ __forceinline__ void thin_tile(const GView& p, int64_t l, int64_t g0, float* lds, int64_t* goff)
 {
     float* Yl = lds;               // [KC][YP]
     float* Xl = lds + KC * YP;     // [SMAX][KC]
@@ -53,6 +81,12 @@ __device__ __forceinline__ void thin_tile(const tk_prob& p, int64_t l, int64_t g
 #pragma unroll
     for (int s = 0; s < SB; ++s) acc[s] = 0.0f;
     const int64_t lx = p.lX[l], ly = p.lY[l];
+    const int gn = (int)(p.G - g0 < GT ? p.G - g0 : GT);
+    // column offsets of this tile, once (the K loop reuses them)
+    __syncthreads();
+    goff[tid] = tid < gn ? p.gY[g0 + tid] : 0;
+    // dense Y (K consecutive, columns K apart): the tile is one contiguous block
+    const bool dense = p.y_kfast == 2;
     for (int64_t k0 = 0; k0 < p.K; k0 += KC) {
         const int kc = (int)(p.K - k0 < KC ? p.K - k0 : KC);
         __syncthreads();
@@ -62,18 +96,31 @@ __device__ __forceinline__ void thin_tile(const tk_prob& p, int64_t l, int64_t g
             if (s < p.S && k < kc) v = rnd_in(p.X[lx + p.sX[s] + p.kX[k0 + k]], p.round);
             Xl[s * KC + k] = v;
         }
-        if (p.y_kfast) {
+        if (dense) {
+            // element (k, gg) at ly + goff[0] + gg * K + k0 + k
+            gf base = p.Y + ly + goff[0] + k0;
+#pragma unroll 8
             for (int e = tid; e < KC * GT; e += 256) {
                 const int k = e % KC, gg = e / KC;
                 float v = 0.0f;
-                if (k < kc && g0 + gg < p.G) v = rnd_in(p.Y[ly + p.kY[k0 + k] + p.gY[g0 + gg]], p.round);
+                if (k < kc && gg < gn) v = rnd_in(base[(int64_t)gg * p.K + k], p.round);
+                Yl[k * YP + gg] = v;
+            }
+        } else if (p.y_kfast) {
+            const int k = tid % KC;
+            const int64_t ko = k < kc ? p.kY[k0 + k] : 0;
+#pragma unroll 8
+            for (int gg = tid / KC; gg < GT; gg += 256 / KC) {
+                float v = 0.0f;
+                if (k < kc && gg < gn) v = rnd_in(p.Y[ly + ko + goff[gg]], p.round);
                 Yl[k * YP + gg] = v;
             }
         } else {
-            for (int e = tid; e < KC * GT; e += 256) {
-                const int gg = e % GT, k = e / GT;
+            const int gg = tid;
+#pragma unroll 8
+            for (int k = 0; k < KC; ++k) {
                 float v = 0.0f;
-                if (k < kc && g0 + gg < p.G) v = rnd_in(p.Y[ly + p.kY[k0 + k] + p.gY[g0 + gg]], p.round);
+                if (k < kc && gg < gn) v = rnd_in(p.Y[ly + p.kY[k0 + k] + goff[gg]], p.round);
                 Yl[k * YP + gg] = v;
             }
         }
@@ -92,35 +139,77 @@ __device__ __forceinline__ void thin_tile(const tk_prob& p, int64_t l, int64_t g
     }
 }
 
-__device__ __forceinline__ void mfma_tile(const tk_prob& p, int64_t l, int64_t m0, int64_t n0, float* lds)
+__device__ __forceinline__ void mfma_tile(const GView& p, int64_t l, int64_t m0, int64_t n0, float* lds,
+                                          int64_t* kox, int64_t* koy)
 {
     float* As = lds;               // [MK][MP]
     float* Bs = lds + MK * MP;     // [MK][MP]
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave & 1, wn = wave >> 1, il = lane & 31, kl = lane >> 5;
     const int64_t lx = p.lX[l], ly = p.lY[l];
+    // fixed rows of this lane: A element (tid % 64, tid / 64 + 4 t); B element
+    // (n, k) chosen so that the unit-stride index runs across lanes
+    const int am = tid % MT;
+    const bool arow_ok = m0 + am < p.S;
+    const int64_t arow = arow_ok ? lx + p.sX[m0 + am] : 0;
+    const bool bk = p.y_kfast != 0;
+    const int bn = bk ? tid / MK : tid % MT;
+    int64_t bcol[(MT * MK) / 256];
+    bool bok[(MT * MK) / 256];          // offsets may be negative: validity kept apart
+#pragma unroll
+    for (int t = 0; t < (MT * MK) / 256; ++t) {
+        const int n = bk ? bn + (256 / MK) * t : bn;
+        bok[t] = n0 + n < p.G;
+        bcol[t] = bok[t] ? ly + p.gY[n0 + n] : 0;
+    }
+    float ra[(MT * MK) / 256], rb[(MT * MK) / 256];
+    // loads of one K step: offsets of K come from LDS (staged per KO-chunk)
+    auto fetch = [&](int64_t k0) {
+        const int64_t kb = k0 - (k0 / KO) * KO;
+#pragma unroll
+        for (int t = 0; t < (MT * MK) / 256; ++t) {
+            const int k = tid / MT + (256 / MT) * t;
+            ra[t] = (arow_ok && k0 + k < p.K) ? rnd_in(p.X[arow + kox[kb + k]], p.round) : 0.0f;
+        }
+#pragma unroll
+        for (int t = 0; t < (MT * MK) / 256; ++t) {
+            const int k = bk ? tid % MK : tid / MT + (256 / MT) * t;
+            rb[t] = (bok[t] && k0 + k < p.K) ? rnd_in(p.Y[bcol[t] + koy[kb + k]], p.round) : 0.0f;
+        }
+    };
+    auto stage = [&](int64_t k0) {      // K offsets for k0 .. k0 + KO
+        __syncthreads();
+        for (int k = tid; k < KO; k += 256) {
+            const bool in = k0 + k < p.K;
+            kox[k] = in ? p.kX[k0 + k] : 0;
+            koy[k] = in ? p.kY[k0 + k] : 0;
+        }
+        __syncthreads();
+    };
     f32x16 acc = {};
+    if (p.K > 0) {
+        stage(0);
+        fetch(0);
+    }
     for (int64_t k0 = 0; k0 < p.K; k0 += MK) {
         __syncthreads();
 #pragma unroll
-        for (int t = 0; t < (MT * MK) / 256; ++t) {
-            const int e = tid + 256 * t;
-            const int m = e % MT, k = e / MT;
-            float v = 0.0f;
-            if (m0 + m < p.S && k0 + k < p.K) v = rnd_in(p.X[lx + p.sX[m0 + m] + p.kX[k0 + k]], p.round);
-            As[k * MP + m] = v;
-        }
+        for (int t = 0; t < (MT * MK) / 256; ++t) As[(tid / MT + (256 / MT) * t) * MP + am] = ra[t];
 #pragma unroll
         for (int t = 0; t < (MT * MK) / 256; ++t) {
-            const int e = tid + 256 * t;
-            int n, k;
-            if (p.y_kfast) { k = e % MK; n = e / MK; }
-            else { n = e % MT; k = e / MT; }
-            float v = 0.0f;
-            if (n0 + n < p.G && k0 + k < p.K) v = rnd_in(p.Y[ly + p.gY[n0 + n] + p.kY[k0 + k]], p.round);
-            Bs[k * MP + n] = v;
+            const int k = bk ? tid % MK : tid / MT + (256 / MT) * t;
+            const int n = bk ? bn + (256 / MK) * t : bn;
+            Bs[k * MP + n] = rb[t];
         }
         __syncthreads();
+        const int64_t kn = k0 + MK;
+        if (kn < p.K) {
+            if (kn % KO == 0) {
+                // the offsets of the next chunk are needed before its first loads
+                stage(kn);
+            }
+            fetch(kn);                  // next step's loads in flight during the MFMAs
+        }
 #pragma unroll
         for (int kk = 0; kk < MK; kk += 2) {
             const float a = As[(kk + kl) * MP + wm * 32 + il];
@@ -138,10 +227,13 @@ __device__ __forceinline__ void mfma_tile(const tk_prob& p, int64_t l, int64_t m
     }
 }
 
-__global__ __launch_bounds__(256) void k_group(const tk_prob* __restrict__ probs, const int64_t* __restrict__ tile_end,
-                                               int nprobs, int64_t total, unsigned long long* counter)
+__global__ __launch_bounds__(256, 3) void k_group(const tk_prob* __restrict__ probs, const int64_t* __restrict__ tile_end,
+                                               const int* __restrict__ tile_prob, int nprobs, int64_t total,
+                                               unsigned long long* counter)
 {
     __shared__ __attribute__((aligned(16))) float lds[KC * YP + SMAX * KC];
+    __shared__ int64_t goff[GT];
+    __shared__ int64_t kox[KO], koy[KO];
     __shared__ int64_t next;
     for (;;) {
         if (threadIdx.x == 0) next = (int64_t)atomicAdd(counter, 1ull);
@@ -149,22 +241,32 @@ __global__ __launch_bounds__(256) void k_group(const tk_prob* __restrict__ probs
         const int64_t t = next;
         __syncthreads();
         if (t >= total) break;
-        int lo = 0, hi = nprobs - 1;                 // first problem with tile_end > t
-        while (lo < hi) {
-            const int mid = (lo + hi) >> 1;
-            if (tile_end[mid] > t) hi = mid; else lo = mid + 1;
-        }
-        const tk_prob& p = probs[lo];
-        const int64_t local = t - (lo ? tile_end[lo - 1] : 0);
-        const int64_t per_l = p.tiles_s * p.tiles_g;
-        const int64_t l = local / per_l, rem = local % per_l;
-        const int64_t ts = rem / p.tiles_g, tg = rem % p.tiles_g;
-        if (p.cls == TK_GROUP_THIN) {
-            if (p.S <= 4) thin_tile<4>(p, l, tg * GT, lds);
-            else if (p.S <= 8) thin_tile<8>(p, l, tg * GT, lds);
-            else thin_tile<16>(p, l, tg * GT, lds);
+        int lo;
+        if (tile_prob) {
+            lo = tile_prob[t];                       // direct map, one load
         } else {
-            mfma_tile(p, l, ts * MT, tg * MT, lds);
+            int a = 0, b = nprobs - 1;               // first problem with tile_end > t
+            while (a < b) {
+                const int mid = (a + b) >> 1;
+                if (tile_end[mid] > t) b = mid; else a = mid + 1;
+            }
+            lo = a;
+        }
+        const tk_prob& pd = probs[lo];
+        const GView p(pd);
+        const int64_t local = t - (lo ? tile_end[lo - 1] : 0);
+        const int64_t tiles_s = pd.tiles_s, tiles_g = pd.tiles_g, L = pd.L;
+        const int cls = pd.cls;
+        (void)L;
+        const int64_t per_l = tiles_s * tiles_g;
+        const int64_t l = local / per_l, rem = local % per_l;
+        const int64_t ts = rem / tiles_g, tg = rem % tiles_g;
+        if (cls == TK_GROUP_THIN) {
+            if (p.S <= 4) thin_tile<4>(p, l, tg * GT, lds, goff);
+            else if (p.S <= 8) thin_tile<8>(p, l, tg * GT, lds, goff);
+            else thin_tile<16>(p, l, tg * GT, lds, goff);
+        } else {
+            mfma_tile(p, l, ts * MT, tg * MT, lds, kox, koy);
         }
     }
 }
@@ -175,12 +277,13 @@ __global__ void k_zero_counter(unsigned long long* counter) { *counter = 0ull; }
 
 using namespace tk;
 
-extern "C" int tk_launch_group(const tk_prob* probs, const int64_t* tile_end, int nprobs, int64_t total,
-                               unsigned long long* counter, int nblocks, void* stream)
+extern "C" int tk_launch_group(const tk_prob* probs, const int64_t* tile_end, const int* tile_prob, int nprobs,
+                               int64_t total, unsigned long long* counter, int nblocks, void* stream)
 {
     hipStream_t st = (hipStream_t)stream;
     if (nprobs <= 0 || total <= 0) return 0;
     hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, st, counter);
-    hipLaunchKernelGGL(k_group, dim3((unsigned)nblocks), dim3(256), 0, st, probs, tile_end, nprobs, total, counter);
+    hipLaunchKernelGGL(k_group, dim3((unsigned)nblocks), dim3(256), 0, st, probs, tile_end, tile_prob, nprobs, total,
+                       counter);
     return (int)hipGetLastError();
 }

@@ -209,6 +209,14 @@ void tk_problem_host_k(struct tapp_plan* plan, const int64_t** kA, const int64_t
     *kB = c->host + c->off[T_kB];
 }
 
+/* Host copies of the M table of A and the N table of B. */
+void tk_problem_host_mn(struct tapp_plan* plan, const int64_t** mA, const int64_t** nB)
+{
+    struct tapp_plan_cache* c = plan->cache;
+    *mA = c->host + c->off[T_mA];
+    *nB = c->host + c->off[T_nB];
+}
+
 /* --- residency ------------------------------------------------------------ */
 
 static bool is_device_ptr(const void* p)

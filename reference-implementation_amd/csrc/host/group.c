@@ -12,6 +12,7 @@
  * conversion pre-passes, host operands) run one by one after it, through the
  * same path as TAPP_execute_product.
  */
+#include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -26,6 +27,7 @@ struct tapp_group {
     void* dmem;                /* tk_prob[n] | int64 tile_end[n] | counter */
     tk_prob* dprobs;
     int64_t* dtile_end;
+    int* dtile_prob;
     unsigned long long* dcounter;
     hipGraph_t graph;
     hipGraphExec_t gexec;
@@ -44,6 +46,19 @@ static bool device_ptr(const void* p)
     hipError_t e = hipPointerGetAttributes(&a, p);
     (void)hipGetLastError();
     return e == hipSuccess && (a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged);
+}
+
+/* How the persistent kernel reads Y: 0 = along columns (unit step across
+ * g), 1 = along K (unit step in k), 2 = dense (k[i] = i and g[j] = j * K:
+ * each tile of columns is one contiguous block). */
+static int y_access(const int64_t* kt, int64_t K, const int64_t* gt, int64_t G)
+{
+    if (K <= 1) return 0;
+    for (int64_t k = 0; k < K; k++)
+        if (kt[k] != kt[0] + k) return kt[1] - kt[0] == 1 ? 1 : 0;
+    for (int64_t g = 0; g < G; g++)
+        if (gt[g] != g * K) return 1;
+    return kt[0] == 0 ? 2 : 1;
 }
 
 typedef struct { tk_prob p; double cost; } ranked;
@@ -68,8 +83,9 @@ bool tk_group_describe(struct tapp_plan* plan, struct tapp_exec* ex, const tk_pr
     const int64_t *mA, *kA, *lA, *nB, *kB, *lB, *mD, *nD, *lD, *mC, *nC, *lC;
     if ((*rc = tk_problem_tables(plan, P, ex->device, &mA, &kA, &lA, &nB, &kB, &lB, &mD, &nD, &lD, &mC, &nC, &lC)))
         return false;
-    const int64_t* hkA; const int64_t* hkB;
+    const int64_t *hkA, *hkB, *hmA, *hnB;
     tk_problem_host_k(plan, &hkA, &hkB);
+    tk_problem_host_mn(plan, &hmA, &hnB);
     memset(out, 0, sizeof *out);
     const float* a = (const float*)A + P->pin[0];
     const float* b = (const float*)B + P->pin[1];
@@ -92,13 +108,13 @@ bool tk_group_describe(struct tapp_plan* plan, struct tapp_exec* ex, const tk_pr
         out->X = b; out->sX = nB; out->kX = kB; out->lX = lB;
         out->Y = a; out->gY = mA; out->kY = kA; out->lY = lA;
         out->sD = nD; out->gD = mD; out->sC = nC; out->gC = mC;
-        out->y_kfast = P->nK > 1 && hkA[1] - hkA[0] == 1;
+        out->y_kfast = y_access(hkA, P->nK, hmA, M);
     } else {
         out->S = M; out->G = N;
         out->X = a; out->sX = mA; out->kX = kA; out->lX = lA;
         out->Y = b; out->gY = nB; out->kY = kB; out->lY = lB;
         out->sD = mD; out->gD = nD; out->sC = mC; out->gC = nC;
-        out->y_kfast = P->nK > 1 && hkB[1] - hkB[0] == 1;
+        out->y_kfast = y_access(hkB, P->nK, hnB, N);
     }
     out->lD = lD; out->lC = lC;
     out->K = P->nK; out->L = P->nL;
@@ -180,7 +196,6 @@ static int create_group(TAPP_AMD_product_group* group, TAPP_executor exec, int c
     g->n = n;
     if (n > 0) {
         size_t pb = sizeof(tk_prob) * (size_t)n, tb = sizeof(int64_t) * (size_t)n;
-        size_t bytes = pb + tb + 256;
         tk_prob* hp = malloc(pb);
         int64_t* he = malloc(tb);
         if (!hp || !he) { free(hp); free(he); free(list); group_free(g); return TAPP_AMD_ERR_NOMEM; }
@@ -191,23 +206,34 @@ static int create_group(TAPP_AMD_product_group* group, TAPP_executor exec, int c
             he[i] = total;
         }
         g->total = total;
-        hipError_t e = hipMalloc(&g->dmem, bytes);
-        if (e != hipSuccess) { (void)hipGetLastError(); free(hp); free(he); free(list); group_free(g); return TAPP_AMD_ERR_NOMEM; }
+        /* tile -> product map (int32), so a workgroup finds its product in one load */
+        size_t mb = total <= INT32_MAX ? sizeof(int) * (size_t)total : 0;
+        int* hm = mb ? malloc(mb) : NULL;
+        if (hm)
+            for (int i = 0, t = 0; i < n; i++)
+                for (; t < he[i]; t++) hm[t] = i;
+        size_t o_end = (pb + 255) & ~(size_t)255, o_map = (o_end + tb + 255) & ~(size_t)255;
+        size_t o_cnt = (o_map + (hm ? mb : 0) + 255) & ~(size_t)255;
+        hipError_t e = hipMalloc(&g->dmem, o_cnt + 256);
+        if (e != hipSuccess) { (void)hipGetLastError(); free(hp); free(he); free(hm); free(list); group_free(g); return TAPP_AMD_ERR_NOMEM; }
         g->dprobs = (tk_prob*)g->dmem;
-        g->dtile_end = (int64_t*)((char*)g->dmem + pb);
-        g->dcounter = (unsigned long long*)((char*)g->dmem + ((pb + tb + 255) & ~(size_t)255));
+        g->dtile_end = (int64_t*)((char*)g->dmem + o_end);
+        g->dtile_prob = hm ? (int*)((char*)g->dmem + o_map) : NULL;
+        g->dcounter = (unsigned long long*)((char*)g->dmem + o_cnt);
         e = hipMemcpy(g->dprobs, hp, pb, hipMemcpyHostToDevice);
         if (e == hipSuccess) e = hipMemcpy(g->dtile_end, he, tb, hipMemcpyHostToDevice);
-        free(hp); free(he);
+        if (e == hipSuccess && hm) e = hipMemcpy(g->dtile_prob, hm, mb, hipMemcpyHostToDevice);
+        free(hp); free(he); free(hm);
         if (e != hipSuccess) { free(list); group_free(g); return TAPP_AMD_ERR_DEVICE; }
         int cus = 0;
         if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ex->device) != hipSuccess || cus <= 0)
             cus = 256;
-        int64_t nb = (int64_t)cus * 2;
+        int64_t nb = (int64_t)cus * 3;
         g->nblocks = (int)(total < nb ? total : nb);
         /* capture zero-counter + persistent kernel once; replays reuse it */
         if (capture && hipStreamBeginCapture(ex->stream, hipStreamCaptureModeThreadLocal) == hipSuccess) {
-            int lrc = tk_launch_group(g->dprobs, g->dtile_end, n, total, g->dcounter, g->nblocks, ex->stream);
+            int lrc = tk_launch_group(g->dprobs, g->dtile_end, g->dtile_prob, n, total, g->dcounter, g->nblocks,
+                                      ex->stream);
             hipError_t ce = hipStreamEndCapture(ex->stream, &g->graph);
             if (lrc == 0 && ce == hipSuccess && hipGraphInstantiate(&g->gexec, g->graph, NULL, NULL, 0) == hipSuccess)
                 g->have_graph = true;
@@ -261,7 +287,8 @@ TAPP_error TAPP_AMD_execute_product_group(TAPP_AMD_product_group group)
     if (g->n > 0) {
         if (g->have_graph) {
             if (hipGraphLaunch(g->gexec, ex->stream) != hipSuccess) return TAPP_AMD_ERR_DEVICE;
-        } else if (tk_launch_group(g->dprobs, g->dtile_end, g->n, g->total, g->dcounter, g->nblocks, ex->stream)) {
+        } else if (tk_launch_group(g->dprobs, g->dtile_end, g->dtile_prob, g->n, g->total, g->dcounter, g->nblocks,
+                                   ex->stream)) {
             return TAPP_AMD_ERR_DEVICE;
         }
     }

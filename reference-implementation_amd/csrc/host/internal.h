@@ -119,6 +119,7 @@ int tk_problem_tables(struct tapp_plan* plan, const tk_problem* P, int device, c
                       const int64_t** lB, const int64_t** mD, const int64_t** nD, const int64_t** lD,
                       const int64_t** mC, const int64_t** nC, const int64_t** lC);
 void tk_problem_host_k(struct tapp_plan* plan, const int64_t** kA, const int64_t** kB);
+void tk_problem_host_mn(struct tapp_plan* plan, const int64_t** mA, const int64_t** nB);
 bool tk_group_describe(struct tapp_plan* plan, struct tapp_exec* ex, const tk_problem* P, const void* A,
                        const void* B, const void* C, void* D, tk_prob* out, int* rc);
 

@@ -119,7 +119,7 @@ typedef struct tk_prob {
     float alpha, beta;
     int beta_zero;
     int cls;
-    int y_kfast;        /* Y's K step is 1: load Y tiles along K */
+    int y_kfast;        /* 0: load Y along g, 1: along K (unit K step), 2: dense K x G block */
     int round;          /* 0 or TK_MODE_BF16 / TK_MODE_F16: inputs rounded (FP32 accumulate) */
     int round_C;
     int pad;
@@ -131,9 +131,10 @@ int tk_launch_gett(const tk_gett* g, void* stream);
 int tk_launch_prep(const tk_prep* p, void* stream);
 int tk_launch_scatter(const tk_scatter* s, void* stream);
 /* Zeroes *counter, then runs the persistent group kernel: `nblocks`
- * workgroups take tiles from the atomic counter until `total` is reached. */
-int tk_launch_group(const tk_prob* probs, const int64_t* tile_end, int nprobs, int64_t total,
-                    unsigned long long* counter, int nblocks, void* stream);
+ * workgroups take tiles from the atomic counter until `total` is reached.
+ * tile_prob (optional, int[total]) maps a tile to its product. */
+int tk_launch_group(const tk_prob* probs, const int64_t* tile_end, const int* tile_prob, int nprobs,
+                    int64_t total, unsigned long long* counter, int nblocks, void* stream);
 
 /* Number of kernel variants compiled in, for diagnostics. */
 int tk_kernel_count(void);
