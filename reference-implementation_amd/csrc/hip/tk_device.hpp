@@ -101,9 +101,9 @@ template <> struct Elem<double, true> {
     }
 };
 
-// out = op_D(alpha * acc + beta * op_C(C[l, m, n])) written to D[l, m, n].
+// out = op_D(alpha * acc + beta * op_C(C at coff)) written to D at doff.
 template <typename R, bool CPLX>
-__device__ __forceinline__ void epilogue(const tk_gett& g, int64_t l, int64_t m, int64_t n, R acc_re, R acc_im)
+__device__ __forceinline__ void epilogue_at(const tk_gett& g, int64_t coff, int64_t doff, R acc_re, R acc_im)
 {
     const R ar = (R)g.alpha[0], ai = (R)g.alpha[1];
     R o_re, o_im = 0;
@@ -114,7 +114,7 @@ __device__ __forceinline__ void epilogue(const tk_gett& g, int64_t l, int64_t m,
         o_re = ar * acc_re;
     }
     if (!g.beta_zero) {
-        Val<R, CPLX> c = Elem<R, CPLX>::ld(g.C, g.lC[l] + g.mC[m] + g.nC[n]);
+        Val<R, CPLX> c = Elem<R, CPLX>::ld(g.C, coff);
         if (g.round_C) {
             c.re = round_mode(c.re, g.mode);
             if (CPLX) c.im = round_mode(c.im, g.mode);
@@ -132,7 +132,15 @@ __device__ __forceinline__ void epilogue(const tk_gett& g, int64_t l, int64_t m,
         if (g.conjD_via_f32) { o_re = (R)(float)o_re; o_im = (R)(float)o_im; }
         o_im = -o_im;
     }
-    Elem<R, CPLX>::st(g.D, g.lD[l] + g.mD[m] + g.nD[n], Val<R, CPLX>{o_re, o_im});
+    Elem<R, CPLX>::st(g.D, doff, Val<R, CPLX>{o_re, o_im});
+}
+
+// out = op_D(alpha * acc + beta * op_C(C[l, m, n])) written to D[l, m, n].
+template <typename R, bool CPLX>
+__device__ __forceinline__ void epilogue(const tk_gett& g, int64_t l, int64_t m, int64_t n, R acc_re, R acc_im)
+{
+    const int64_t coff = g.beta_zero ? 0 : g.lC[l] + g.mC[m] + g.nC[n];
+    epilogue_at<R, CPLX>(g, coff, g.lD[l] + g.mD[m] + g.nD[n], acc_re, acc_im);
 }
 
 // Workgroup id -> tile id so that consecutive tiles land on the same XCD

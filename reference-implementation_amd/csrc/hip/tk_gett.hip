@@ -320,18 +320,34 @@ __global__ __launch_bounds__(256) void k_mfma(tk_gett g)
             __syncthreads();
         }
 
+        // table offsets hoisted: per batch index, per row m (fixed per lane
+        // and fm), per column n (fixed per fn and register)
+        const int64_t lc = g.beta_zero ? 0 : g.lC[l], ld = g.lD[l];
+        int64_t nc[FN][MF::NR], nd[FN][MF::NR];
 #pragma unroll
         for (int fn = 0; fn < FN; ++fn)
 #pragma unroll
-            for (int fm = 0; fm < FM; ++fm) {
-                const int64_t m = m0 + wm * FM * S + fm * S + il;
+            for (int r = 0; r < MF::NR; ++r) {
+                const int64_t n = n0 + wn * FN * S + fn * S + MF::row(r, lane);
+                const bool in = n < g.N;
+                nd[fn][r] = in ? g.nD[n] : 0;
+                nc[fn][r] = (in && !g.beta_zero) ? g.nC[n] : 0;
+            }
+#pragma unroll
+        for (int fm = 0; fm < FM; ++fm) {
+            const int64_t m = m0 + wm * FM * S + fm * S + il;
+            if (m >= g.M) continue;
+            const int64_t mc = lc + (g.beta_zero ? 0 : g.mC[m]), md = ld + g.mD[m];
+#pragma unroll
+            for (int fn = 0; fn < FN; ++fn)
 #pragma unroll
                 for (int r = 0; r < MF::NR; ++r) {
                     const int64_t n = n0 + wn * FN * S + fn * S + MF::row(r, lane);
-                    if (m < g.M && n < g.N)
-                        epilogue<R, CPLX>(g, l, m, n, acc[fn][fm][r], CPLX ? acci[fn][fm][r] : (R)0);
+                    if (n < g.N)
+                        epilogue_at<R, CPLX>(g, mc + nc[fn][r], md + nd[fn][r], acc[fn][fm][r],
+                                             CPLX ? acci[fn][fm][r] : (R)0);
                 }
-            }
+        }
     }
 }
 
@@ -498,7 +514,11 @@ __global__ __launch_bounds__(256, 2) void k_gett_f32(tk_gett g)
     const int64_t tile = xcd_tile(blockIdx.x, ntm * ntn);
     const int64_t m0 = (tile / ntn) * BM, n0 = (tile % ntn) * BN;
     const bool interior = m0 + BM <= g.M && n0 + BN <= g.N;
-    const int64_t nk = (g.K + BK - 1) / BK, nk_full = g.K / BK;
+    // split-K: this workgroup covers K in [kb, ke)
+    const int64_t kb = g.ksplit > 1 ? (int64_t)blockIdx.z * g.kchunk : 0;
+    const int64_t ke = g.ksplit > 1 ? (kb + g.kchunk < g.K ? kb + g.kchunk : g.K) : g.K;
+    const int64_t klen = ke > kb ? ke - kb : 0;
+    const int64_t nk = (klen + BK - 1) / BK, nk_full = klen / BK;
 
     F32Tile<AL, BM, BK> la;
     F32Tile<BL, BN, BK> lb;
@@ -516,11 +536,11 @@ __global__ __launch_bounds__(256, 2) void k_gett_f32(tk_gett g)
 
         auto fetch = [&](int64_t kt) {
             if (interior && kt < nk_full) {
-                la.template load<false>(Ab, g.kA, m0, kt * BK, g.M, g.K);
-                lb.template load<false>(Bb, g.kB, n0, kt * BK, g.N, g.K);
+                la.template load<false>(Ab, g.kA, m0, kb + kt * BK, g.M, ke);
+                lb.template load<false>(Bb, g.kB, n0, kb + kt * BK, g.N, ke);
             } else {
-                la.template load<true>(Ab, g.kA, m0, kt * BK, g.M, g.K);
-                lb.template load<true>(Bb, g.kB, n0, kt * BK, g.N, g.K);
+                la.template load<true>(Ab, g.kA, m0, kb + kt * BK, g.M, ke);
+                lb.template load<true>(Bb, g.kB, n0, kb + kt * BK, g.N, ke);
             }
         };
         if (nk > 0) {
@@ -578,6 +598,23 @@ __global__ __launch_bounds__(256, 2) void k_gett_f32(tk_gett g)
             __syncthreads();
         }
 
+        if (g.ksplit > 1) {
+            // raw partial sums, [split][l][n][m] with m fastest (coalesced)
+            float* part = g.partial + (((int64_t)blockIdx.z * g.L + l) * g.N) * g.M;
+#pragma unroll
+            for (int fm = 0; fm < FM; ++fm) {
+                const int64_t m = m0 + wm * FM * S + fm * S + il;
+                if (m >= g.M) continue;
+#pragma unroll
+                for (int fn = 0; fn < FN; ++fn)
+#pragma unroll
+                    for (int r = 0; r < 16; ++r) {
+                        const int64_t n = n0 + wn * FN * S + fn * S + 8 * (r >> 2) + 4 * kl + (r & 3);
+                        if (n < g.N) part[n * g.M + m] = acc[fn][fm][r];
+                    }
+            }
+            continue;
+        }
         const bool plain = g.beta_zero && !g.round_C;
         const float alpha = (float)g.alpha[0];
         const int64_t dl = g.lD[l];
@@ -599,17 +636,36 @@ __global__ __launch_bounds__(256, 2) void k_gett_f32(tk_gett g)
     }
 }
 
+// Sums the split-K partials in split order (deterministic) and applies the
+// epilogue: one lane per output, m fastest.
+__global__ __launch_bounds__(256) void k_splitk_reduce(tk_gett g)
+{
+    const int64_t total = g.M * g.N * g.L;
+    for (int64_t o = (int64_t)blockIdx.x * 256 + threadIdx.x; o < total; o += (int64_t)gridDim.x * 256) {
+        const int64_t m = o % g.M, t = o / g.M, n = t % g.N, l = t / g.N;
+        float s = 0.0f;
+        for (int z = 0; z < g.ksplit; ++z) s += g.partial[((int64_t)z * g.L + l) * g.N * g.M + n * g.M + m];
+        epilogue<float, false>(g, l, m, n, s, 0.0f);
+    }
+}
+
 template <int AL, int BL>
 static hipError_t launch_gett_f32_modes(const tk_gett& g, hipStream_t st)
 {
     const int64_t tiles = ((g.M + 127) / 128) * ((g.N + 127) / 128);
     const int64_t ly = g.L < 65535 ? g.L : 65535;
-    const dim3 grid((unsigned)tiles, (unsigned)ly), block(256);
+    const dim3 grid((unsigned)tiles, (unsigned)ly, (unsigned)(g.ksplit > 1 ? g.ksplit : 1)), block(256);
     switch (g.mode) {
     case TK_MODE_PLAIN: hipLaunchKernelGGL((k_gett_f32<AL, BL, TK_MODE_PLAIN>), grid, block, 0, st, g); break;
     case TK_MODE_BF16: hipLaunchKernelGGL((k_gett_f32<AL, BL, TK_MODE_BF16>), grid, block, 0, st, g); break;
     case TK_MODE_F16: hipLaunchKernelGGL((k_gett_f32<AL, BL, TK_MODE_F16>), grid, block, 0, st, g); break;
     default: hipLaunchKernelGGL((k_gett_f32<AL, BL, TK_MODE_SPLIT_BF16>), grid, block, 0, st, g); break;
+    }
+    if (g.ksplit > 1) {
+        const int64_t total = g.M * g.N * g.L;
+        int64_t blocks = (total + 255) / 256;
+        if (blocks > 65536) blocks = 65536;
+        hipLaunchKernelGGL(k_splitk_reduce, dim3((unsigned)blocks), dim3(256), 0, st, g);
     }
     return hipGetLastError();
 }

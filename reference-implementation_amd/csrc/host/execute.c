@@ -58,6 +58,7 @@ enum {
 
 struct tapp_plan_cache {
     uint64_t key;
+    int ordA[3], ordB[3];    /* pre-pass order of (rows, K, L), fastest first */
     size_t off[T_COUNT];     /* element offset of each table in the arena */
     size_t total;
     int64_t* host;
@@ -116,6 +117,34 @@ static void fill_linear(int64_t* tab, int64_t n, int64_t step)
     for (int64_t i = 0; i < n; i++) tab[i] = i * step;
 }
 
+/* smallest |stride| of operand x over a group's modes (INT64_MAX if none) */
+static int64_t min_stride(const tk_group* g, int x)
+{
+    int64_t m = INT64_MAX;
+    for (int i = 0; i < g->n; i++) {
+        int64_t v = llabs(g->s[x][i]);
+        if (v < m) m = v;
+    }
+    return m;
+}
+
+/* Order in which the pre-pass walks (rows, K, L) of operand x: the group
+ * holding the operand's smallest stride first, so consecutive lanes read
+ * neighbouring elements; the dense buffer is written in the same order.
+ * dense_step receives each group's step in that buffer. */
+static void prep_order(const tk_group* rows, const tk_group* K, const tk_group* L, int x, const int64_t size[3],
+                       int ord[3], int64_t dense_step[3])
+{
+    int64_t key[3] = {min_stride(rows, x), min_stride(K, x), min_stride(L, x)};
+    ord[0] = 0; ord[1] = 1; ord[2] = 2;
+    for (int i = 1; i < 3; i++)
+        for (int j = i; j > 0 && key[ord[j]] < key[ord[j - 1]]; j--) {
+            int t = ord[j]; ord[j] = ord[j - 1]; ord[j - 1] = t;
+        }
+    int64_t step = 1;
+    for (int i = 0; i < 3; i++) { dense_step[ord[i]] = step; step *= size[ord[i]]; }
+}
+
 static int build_tables(struct tapp_plan* plan, const tk_problem* P, int device, struct tapp_plan_cache** out)
 {
     uint64_t key = problem_key(P);
@@ -155,12 +184,18 @@ static int build_tables(struct tapp_plan* plan, const tk_problem* P, int device,
     fill_table(h + c->off[T_lD], &P->L, 3, L);
     fill_table(h + c->off[T_uA], &P->UA, 0, P->nUA);
     fill_table(h + c->off[T_uB], &P->UB, 1, P->nUB);
-    fill_linear(h + c->off[T_dmA], M, 1);
-    fill_linear(h + c->off[T_dkA], K, M);
-    fill_linear(h + c->off[T_dlA], L, M * K);
-    fill_linear(h + c->off[T_dnB], N, 1);
-    fill_linear(h + c->off[T_dkB], K, N);
-    fill_linear(h + c->off[T_dlB], L, N * K);
+    {
+        const int64_t sa[3] = {M, K, L}, sb[3] = {N, K, L};
+        int64_t da[3], db[3];
+        prep_order(&P->M, &P->K, &P->L, 0, sa, c->ordA, da);
+        prep_order(&P->N, &P->K, &P->L, 1, sb, c->ordB, db);
+        fill_linear(h + c->off[T_dmA], M, da[0]);
+        fill_linear(h + c->off[T_dkA], K, da[1]);
+        fill_linear(h + c->off[T_dlA], L, da[2]);
+        fill_linear(h + c->off[T_dnB], N, db[0]);
+        fill_linear(h + c->off[T_dkB], K, db[1]);
+        fill_linear(h + c->off[T_dlB], L, db[2]);
+    }
     fill_linear(h + c->off[T_dmD], M, 1);
     fill_linear(h + c->off[T_dnD], N, M);
     fill_linear(h + c->off[T_dlD], L, M * N);
@@ -333,10 +368,43 @@ int tk_execute_on(struct tapp_plan* p, struct tapp_exec* ex, const void* alpha, 
     const size_t szC = needC ? (size_t)(M * N * L) * cs : 0;
     const size_t szD = needD ? (size_t)(M * N * L) * cs : 0;
 
-    size_t off_stage[4], off_tmp[4], total = 0;
+    /* kernel choice (symmetric in M and N, so it holds after the orientation
+     * swap below) */
+    const int64_t outs = M * N * L;
+    int kern = ex->force_kernel;
+    if (kern == TK_KERNEL_AUTO) {
+        if (K == 0) kern = TK_KERNEL_SIMPLE;
+        else if (outs <= 4096 && K >= 1024) kern = TK_KERNEL_SPLITK;
+        else if (M >= 16 && N >= 16 && K >= 4 && outs * K >= (1 << 16)) kern = TK_KERNEL_MFMA;
+        else kern = TK_KERNEL_SIMPLE;
+    }
+    if (kern == TK_KERNEL_SPLITK && outs > (int64_t)1 << 31) kern = TK_KERNEL_SIMPLE;
+
+    /* split-K for the FP32-data MFMA kernel when its 128 x 128 tiles cannot
+     * fill the GPU: workgroups take K slices (multiples of its 32-deep K
+     * tile), FP32 partial sums are added in a fixed order by a second kernel */
+    int64_t ks = 1, kchunk = 0;
+    size_t part_bytes = 0;
+    if (kern == TK_KERNEL_MFMA && ct == TK_F32 && K >= 64) {
+        const int64_t tiles = ((M + 127) / 128) * ((N + 127) / 128) * L;
+        const int64_t target = 2 * (int64_t)ex->cus;
+        if (tiles < target) {
+            ks = (target + tiles - 1) / tiles;
+            if (ks > (K + 31) / 32) ks = (K + 31) / 32;
+            if (ks > 64) ks = 64;
+            kchunk = ((K + ks - 1) / ks + 31) / 32 * 32;
+            ks = (K + kchunk - 1) / kchunk;
+            part_bytes = sizeof(float) * (size_t)(ks * outs);
+            if (ks < 2 || part_bytes > ((size_t)64 << 20)) { ks = 1; part_bytes = 0; }
+        }
+    }
+
+    size_t off_stage[4], off_tmp[4], off_part, total = 0;
     for (int x = 0; x < 4; x++) { off_stage[x] = total; total = ALIGN256(total + span_bytes[x]); }
     const size_t tmp_sz[4] = {szA, szB, szC, szD};
     for (int x = 0; x < 4; x++) { off_tmp[x] = total; total = ALIGN256(total + tmp_sz[x]); }
+    off_part = total;
+    total = ALIGN256(total + part_bytes);
     if (total && (rc = ensure_ws(ex, total))) return rc;
     char* ws = (char*)ex->ws;
     hipStream_t st = ex->stream;
@@ -383,15 +451,23 @@ int tk_execute_on(struct tapp_plan* p, struct tapp_exec* ex, const void* alpha, 
     const int64_t *hnB = HT(T_nB), *hkB = HT(T_kB), *hlB = HT(T_lB);
 
     if (needA && K > 0) {
+        const int64_t* tabs[3] = {DT(T_mA), DT(T_kA), DT(T_lA)};
+        const int64_t sizes[3] = {M, K, L};
+        const int* o = tc->ordA;
         tk_prep pr = {P.type[0], ct, P.op_A, (P.nUA > 1 && (P.mode == TK_MODE_F16 || P.mode == TK_MODE_BF16)) ? P.mode : 0,
-                      d[0].base, ws + off_tmp[0], M, K, L, P.nUA, DT(T_mA), DT(T_kA), DT(T_lA), DT(T_uA)};
+                      d[0].base, ws + off_tmp[0], sizes[o[0]], sizes[o[1]], sizes[o[2]], P.nUA,
+                      tabs[o[0]], tabs[o[1]], tabs[o[2]], DT(T_uA)};
         LAUNCHCHK(tk_launch_prep(&pr, st));
         g.A = ws + off_tmp[0]; g.mA = DT(T_dmA); g.kA = DT(T_dkA); g.lA = DT(T_dlA); g.conjA = 0;
         hmA = HT(T_dmA); hkA = HT(T_dkA); hlA = HT(T_dlA);
     }
     if (needB && K > 0) {
+        const int64_t* tabs[3] = {DT(T_nB), DT(T_kB), DT(T_lB)};
+        const int64_t sizes[3] = {N, K, L};
+        const int* o = tc->ordB;
         tk_prep pr = {P.type[1], ct, P.op_B, (P.nUB > 1 && (P.mode == TK_MODE_F16 || P.mode == TK_MODE_BF16)) ? P.mode : 0,
-                      d[1].base, ws + off_tmp[1], N, K, L, P.nUB, DT(T_nB), DT(T_kB), DT(T_lB), DT(T_uB)};
+                      d[1].base, ws + off_tmp[1], sizes[o[0]], sizes[o[1]], sizes[o[2]], P.nUB,
+                      tabs[o[0]], tabs[o[1]], tabs[o[2]], DT(T_uB)};
         LAUNCHCHK(tk_launch_prep(&pr, st));
         g.B = ws + off_tmp[1]; g.nB = DT(T_dnB); g.kB = DT(T_dkB); g.lB = DT(T_dlB); g.conjB = 0;
         hnB = HT(T_dnB); hkB = HT(T_dkB); hlB = HT(T_dlB);
@@ -426,16 +502,6 @@ int tk_execute_on(struct tapp_plan* p, struct tapp_exec* ex, const void* alpha, 
         }
     }
 
-    /* kernel choice */
-    const int64_t outs = g.M * g.N * g.L;
-    int kern = ex->force_kernel;
-    if (kern == TK_KERNEL_AUTO) {
-        if (g.K == 0) kern = TK_KERNEL_SIMPLE;
-        else if (outs <= 4096 && g.K >= 1024) kern = TK_KERNEL_SPLITK;
-        else if (g.M >= 16 && g.N >= 16 && g.K >= 4 && outs * g.K >= (1 << 16)) kern = TK_KERNEL_MFMA;
-        else kern = TK_KERNEL_SIMPLE;
-    }
-    if (kern == TK_KERNEL_SPLITK && outs > (int64_t)1 << 31) kern = TK_KERNEL_SIMPLE;
     g.kernel = kern;
 
     /* tile access patterns for the MFMA kernel */
@@ -449,6 +515,11 @@ int tk_execute_on(struct tapp_plan* p, struct tapp_exec* ex, const void* alpha, 
         else { g.b_load = TK_GATHER; g.b_vec = 0; }
     }
 
+    if (kern == TK_KERNEL_MFMA && ks > 1) {
+        g.ksplit = (int)ks;
+        g.kchunk = kchunk;
+        g.partial = (float*)(ws + off_part);
+    }
     LAUNCHCHK(tk_launch_gett(&g, st));
     ex->last_kernel = kern;
 

@@ -53,6 +53,8 @@ static int exec_init(struct tapp_exec* e)
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return TAPP_AMD_ERR_DEVICE;
     e->device = dev;
+    if (hipDeviceGetAttribute(&e->cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || e->cus <= 0)
+        e->cus = 256;
     if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) return TAPP_AMD_ERR_DEVICE;
     e->own_stream = true;
     if (hipEventCreate(&e->ev0) != hipSuccess || hipEventCreate(&e->ev1) != hipSuccess) return TAPP_AMD_ERR_DEVICE;

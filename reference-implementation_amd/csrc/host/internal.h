@@ -47,6 +47,7 @@ struct tapp_exec {
     int kind;                  /* 1 */
     uint32_t magic;
     int device;
+    int cus;                   /* compute units of the device */
     hipStream_t stream;
     bool own_stream;
     hipEvent_t ev0, ev1;

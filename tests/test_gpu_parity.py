@@ -151,6 +151,34 @@ def test_forced_kernels_on_gemm_shapes(lib, oracle, dtype, kernel):
     lib.destroy_executor(ex)
 
 
+@pytest.mark.parametrize("mode", ["plain", "bf16", "f16", "split_bf16"])
+def test_split_k_few_tiles(lib, oracle, mode):
+    """Few 128 x 128 output tiles and long K: the FP32 MFMA kernel splits K
+    over workgroups and a second kernel sums the partials.  Ragged K (not a
+    multiple of the 32-deep K tile), a Hadamard mode, beta != 0 and a
+    transposed operand are covered."""
+    prec = {"plain": tc.DEFAULT_PREC, "bf16": tc.BF16BF16_ACCUM_F32, "f16": tc.F16F16_ACCUM_F32,
+            "split_bf16": tc.DEFAULT_PREC}[mode]
+    _, ex = lib.create_executor()
+    if mode == "split_bf16":
+        assert lib.attr_set_int(ex, tc.ATTR_SPLIT_BF16, 1) == 0
+    for i, (M, N, K, L) in enumerate([(100, 90, 301, 3), (256, 256, 256, 1), (40, 200, 1000, 2)]):
+        A = T.make_operand(tc.F32, [M, K, L], "mkl", 40 + i, order=[1, 0, 2] if i == 1 else None)
+        B = T.make_operand(tc.F32, [K, N, L], "knl", 50 + i)
+        C = T.make_operand(tc.F32, [M, N, L], "mnl", 60 + i)
+        D = T.make_operand(tc.F32, [M, N, L], "mnl", 70 + i, pad=1)
+        c = T.Case(A, B, C, D, prec, 0.75, -0.5, name=f"splitk{M}x{N}x{K}x{L}_{mode}")
+        rc, got, want = run_both(lib, ex, oracle, c)
+        assert rc == 0
+        assert lib.attr_get_int(ex, tc.ATTR_LAST_KERNEL)[1] == tc.KERNEL_MFMA
+        if mode == "split_bf16":
+            _, exact = oracle.run(c, f64=True)
+            assert T.normwise_error(got, exact, T.written_mask(c.D)) < 1e-5
+        else:
+            assert_parity(got, want, c)
+    lib.destroy_executor(ex)
+
+
 def test_mixed_storage_types(lib, gpu_exec, oracle):
     combos = [(tc.F32, tc.F64, tc.F32, tc.F64), (tc.F32, tc.C32, tc.C32, tc.C32), (tc.C64, tc.F64, tc.C64, tc.C32),
               (tc.F64, tc.F64, tc.F32, tc.F32), (tc.F16, tc.F16, tc.F32, tc.F32), (tc.BF16, tc.F32, tc.F32, tc.BF16)]
