@@ -101,9 +101,28 @@ template <> struct Elem<double, true> {
     }
 };
 
-// out = op_D(alpha * acc + beta * op_C(C at coff)) written to D at doff.
+// op_C(C at coff), the value the epilogue scales by beta (zero when beta is
+// zero).  Kernels read all the C values of a lane before any of its D
+// stores: C and D may alias, so a load issued after a store has to wait for
+// it.
 template <typename R, bool CPLX>
-__device__ __forceinline__ void epilogue_at(const tk_gett& g, int64_t coff, int64_t doff, R acc_re, R acc_im)
+__device__ __forceinline__ Val<R, CPLX> epilogue_c(const tk_gett& g, int64_t coff)
+{
+    Val<R, CPLX> c{(R)0, (R)0};
+    if (!g.beta_zero) {
+        c = Elem<R, CPLX>::ld(g.C, coff);
+        if (g.round_C) {
+            c.re = round_mode(c.re, g.mode);
+            if (CPLX) c.im = round_mode(c.im, g.mode);
+        }
+        if (CPLX && g.conjC) c.im = -c.im;
+    }
+    return c;
+}
+
+// out = op_D(alpha * acc + beta * c) written to D at doff, c = op_C(C).
+template <typename R, bool CPLX>
+__device__ __forceinline__ void epilogue_d(const tk_gett& g, int64_t doff, R acc_re, R acc_im, Val<R, CPLX> c)
 {
     const R ar = (R)g.alpha[0], ai = (R)g.alpha[1];
     R o_re, o_im = 0;
@@ -114,12 +133,6 @@ __device__ __forceinline__ void epilogue_at(const tk_gett& g, int64_t coff, int6
         o_re = ar * acc_re;
     }
     if (!g.beta_zero) {
-        Val<R, CPLX> c = Elem<R, CPLX>::ld(g.C, coff);
-        if (g.round_C) {
-            c.re = round_mode(c.re, g.mode);
-            if (CPLX) c.im = round_mode(c.im, g.mode);
-        }
-        if (CPLX && g.conjC) c.im = -c.im;
         const R br = (R)g.beta[0], bi = (R)g.beta[1];
         if (CPLX) {
             o_re += br * c.re - bi * c.im;
@@ -133,6 +146,13 @@ __device__ __forceinline__ void epilogue_at(const tk_gett& g, int64_t coff, int6
         o_im = -o_im;
     }
     Elem<R, CPLX>::st(g.D, doff, Val<R, CPLX>{o_re, o_im});
+}
+
+// out = op_D(alpha * acc + beta * op_C(C at coff)) written to D at doff.
+template <typename R, bool CPLX>
+__device__ __forceinline__ void epilogue_at(const tk_gett& g, int64_t coff, int64_t doff, R acc_re, R acc_im)
+{
+    epilogue_d<R, CPLX>(g, doff, acc_re, acc_im, epilogue_c<R, CPLX>(g, coff));
 }
 
 // out = op_D(alpha * acc + beta * op_C(C[l, m, n])) written to D[l, m, n].

@@ -338,14 +338,22 @@ __global__ __launch_bounds__(256) void k_mfma(tk_gett g)
             const int64_t m = m0 + wm * FM * S + fm * S + il;
             if (m >= g.M) continue;
             const int64_t mc = lc + (g.beta_zero ? 0 : g.mC[m]), md = ld + g.mD[m];
+            Val<R, CPLX> cv[FN][MF::NR];
+#pragma unroll
+            for (int fn = 0; fn < FN; ++fn)
+#pragma unroll
+                for (int r = 0; r < MF::NR; ++r) {
+                    const int64_t n = n0 + wn * FN * S + fn * S + MF::row(r, lane);
+                    cv[fn][r] = n < g.N ? epilogue_c<R, CPLX>(g, mc + nc[fn][r]) : Val<R, CPLX>{(R)0, (R)0};
+                }
 #pragma unroll
             for (int fn = 0; fn < FN; ++fn)
 #pragma unroll
                 for (int r = 0; r < MF::NR; ++r) {
                     const int64_t n = n0 + wn * FN * S + fn * S + MF::row(r, lane);
                     if (n < g.N)
-                        epilogue_at<R, CPLX>(g, mc + nc[fn][r], md + nd[fn][r], acc[fn][fm][r],
-                                             CPLX ? acci[fn][fm][r] : (R)0);
+                        epilogue_d<R, CPLX>(g, md + nd[fn][r], acc[fn][fm][r], CPLX ? acci[fn][fm][r] : (R)0,
+                                            cv[fn][r]);
                 }
         }
     }
@@ -498,7 +506,7 @@ __device__ __forceinline__ void operand16(const float* t, int col, uint4& hi, ui
     }
 }
 
-template <int AL, int BL, int MODE>
+template <int AL, int BL, int MODE, bool SPLIT>
 __global__ __launch_bounds__(256, 2) void k_gett_f32(tk_gett g)
 {
     constexpr int S = 32, FM = 2, FN = 2, BM = 128, BN = 128, BK = 32;
@@ -514,9 +522,9 @@ __global__ __launch_bounds__(256, 2) void k_gett_f32(tk_gett g)
     const int64_t tile = xcd_tile(blockIdx.x, ntm * ntn);
     const int64_t m0 = (tile / ntn) * BM, n0 = (tile % ntn) * BN;
     const bool interior = m0 + BM <= g.M && n0 + BN <= g.N;
-    // split-K: this workgroup covers K in [kb, ke)
-    const int64_t kb = g.ksplit > 1 ? (int64_t)blockIdx.z * g.kchunk : 0;
-    const int64_t ke = g.ksplit > 1 ? (kb + g.kchunk < g.K ? kb + g.kchunk : g.K) : g.K;
+    // SPLIT: this workgroup covers K in [kb, ke) and writes raw partial sums
+    const int64_t kb = SPLIT ? (int64_t)blockIdx.z * g.kchunk : 0;
+    const int64_t ke = SPLIT ? (kb + g.kchunk < g.K ? kb + g.kchunk : g.K) : g.K;
     const int64_t klen = ke > kb ? ke - kb : 0;
     const int64_t nk = (klen + BK - 1) / BK, nk_full = klen / BK;
 
@@ -598,53 +606,77 @@ __global__ __launch_bounds__(256, 2) void k_gett_f32(tk_gett g)
             __syncthreads();
         }
 
-        if (g.ksplit > 1) {
-            // raw partial sums, [split][l][n][m] with m fastest (coalesced)
-            float* part = g.partial + (((int64_t)blockIdx.z * g.L + l) * g.N) * g.M;
+        if (SPLIT) {
+            // raw partial sums in accumulator order: per (split, l, tile,
+            // wave) 16 float4 slots x 64 lanes, one 1 KB store per slot
+            float4* part = reinterpret_cast<float4*>(g.partial) +
+                           ((((int64_t)blockIdx.z * g.L + l) * (ntm * ntn) + tile) * 4 + wave) * (16 * 64) + lane;
 #pragma unroll
-            for (int fm = 0; fm < FM; ++fm) {
-                const int64_t m = m0 + wm * FM * S + fm * S + il;
-                if (m >= g.M) continue;
+            for (int fn = 0; fn < FN; ++fn)
 #pragma unroll
-                for (int fn = 0; fn < FN; ++fn)
+                for (int fm = 0; fm < FM; ++fm)
 #pragma unroll
-                    for (int r = 0; r < 16; ++r) {
-                        const int64_t n = n0 + wn * FN * S + fn * S + 8 * (r >> 2) + 4 * kl + (r & 3);
-                        if (n < g.N) part[n * g.M + m] = acc[fn][fm][r];
-                    }
-            }
+                    for (int q = 0; q < 4; ++q)
+                        part[((fn * FM + fm) * 4 + q) * 64] =
+                            make_float4(acc[fn][fm][4 * q], acc[fn][fm][4 * q + 1], acc[fn][fm][4 * q + 2],
+                                        acc[fn][fm][4 * q + 3]);
             continue;
         }
-        const bool plain = g.beta_zero && !g.round_C;
         const float alpha = (float)g.alpha[0];
-        const int64_t dl = g.lD[l];
+        const int64_t dl = g.lD[l], cl = g.beta_zero ? 0 : g.lC[l];
 #pragma unroll
         for (int fm = 0; fm < FM; ++fm) {
             const int64_t m = m0 + wm * FM * S + fm * S + il;
             if (m >= g.M) continue;
             const int64_t dm = dl + g.mD[m];
+            if (g.beta_zero) {
 #pragma unroll
-            for (int fn = 0; fn < FN; ++fn)
+                for (int fn = 0; fn < FN; ++fn)
+#pragma unroll
+                    for (int r = 0; r < 16; ++r) {
+                        const int64_t n = n0 + wn * FN * S + fn * S + 8 * (r >> 2) + 4 * kl + (r & 3);
+                        if (n < g.N) ((float*)g.D)[dm + g.nD[n]] = alpha * acc[fn][fm][r];
+                    }
+                continue;
+            }
+            // all C reads of this row block before its D writes
+            const int64_t cm = cl + g.mC[m];
+#pragma unroll
+            for (int fn = 0; fn < FN; ++fn) {
+                float cv[16];
 #pragma unroll
                 for (int r = 0; r < 16; ++r) {
                     const int64_t n = n0 + wn * FN * S + fn * S + 8 * (r >> 2) + 4 * kl + (r & 3);
-                    if (n >= g.N) continue;
-                    if (plain) ((float*)g.D)[dm + g.nD[n]] = alpha * acc[fn][fm][r];
-                    else epilogue<float, false>(g, l, m, n, acc[fn][fm][r], 0.0f);
+                    cv[r] = n < g.N ? epilogue_c<float, false>(g, cm + g.nC[n]).re : 0.0f;
                 }
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    const int64_t n = n0 + wn * FN * S + fn * S + 8 * (r >> 2) + 4 * kl + (r & 3);
+                    if (n < g.N)
+                        epilogue_d<float, false>(g, dm + g.nD[n], acc[fn][fm][r], 0.0f, Val<float, false>{cv[r], 0.0f});
+                }
+            }
         }
     }
 }
 
-// Sums the split-K partials in split order (deterministic) and applies the
-// epilogue: one lane per output, m fastest.
+// Sums the split-K partials of k_gett_f32<..., true> in split order
+// (deterministic) and applies the epilogue.  Position p of a split's partial
+// block is (l, tile, wave, slot, lane, element) with element fastest, the
+// order the GETT kernel stored its accumulators in.
 __global__ __launch_bounds__(256) void k_splitk_reduce(tk_gett g)
 {
-    const int64_t total = g.M * g.N * g.L;
-    for (int64_t o = (int64_t)blockIdx.x * 256 + threadIdx.x; o < total; o += (int64_t)gridDim.x * 256) {
-        const int64_t m = o % g.M, t = o / g.M, n = t % g.N, l = t / g.N;
+    const int64_t ntm = (g.M + 127) / 128, ntn = (g.N + 127) / 128;
+    const int64_t per_split = g.L * ntm * ntn * 16384;
+    for (int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x; p < per_split; p += (int64_t)gridDim.x * 256) {
+        const int e = (int)(p & 3), lane = (int)((p >> 2) & 63), slot = (int)((p >> 8) & 15), wave = (int)((p >> 12) & 3);
+        const int64_t tl = p >> 14, tile = tl % (ntm * ntn), l = tl / (ntm * ntn);
+        const int fn = slot >> 3, fm = (slot >> 2) & 1, r = 4 * (slot & 3) + e;
+        const int64_t m = (tile / ntn) * 128 + (wave & 1) * 64 + fm * 32 + (lane & 31);
+        const int64_t n = (tile % ntn) * 128 + (wave >> 1) * 64 + fn * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+        if (m >= g.M || n >= g.N) continue;
         float s = 0.0f;
-        for (int z = 0; z < g.ksplit; ++z) s += g.partial[((int64_t)z * g.L + l) * g.N * g.M + n * g.M + m];
+        for (int z = 0; z < g.ksplit; ++z) s += g.partial[z * per_split + p];
         epilogue<float, false>(g, l, m, n, s, 0.0f);
     }
 }
@@ -655,14 +687,19 @@ static hipError_t launch_gett_f32_modes(const tk_gett& g, hipStream_t st)
     const int64_t tiles = ((g.M + 127) / 128) * ((g.N + 127) / 128);
     const int64_t ly = g.L < 65535 ? g.L : 65535;
     const dim3 grid((unsigned)tiles, (unsigned)ly, (unsigned)(g.ksplit > 1 ? g.ksplit : 1)), block(256);
+    const bool split = g.ksplit > 1;
+#define TK_F32_LAUNCH(MODE)                                                          \
+    if (split) hipLaunchKernelGGL((k_gett_f32<AL, BL, MODE, true>), grid, block, 0, st, g); \
+    else hipLaunchKernelGGL((k_gett_f32<AL, BL, MODE, false>), grid, block, 0, st, g)
     switch (g.mode) {
-    case TK_MODE_PLAIN: hipLaunchKernelGGL((k_gett_f32<AL, BL, TK_MODE_PLAIN>), grid, block, 0, st, g); break;
-    case TK_MODE_BF16: hipLaunchKernelGGL((k_gett_f32<AL, BL, TK_MODE_BF16>), grid, block, 0, st, g); break;
-    case TK_MODE_F16: hipLaunchKernelGGL((k_gett_f32<AL, BL, TK_MODE_F16>), grid, block, 0, st, g); break;
-    default: hipLaunchKernelGGL((k_gett_f32<AL, BL, TK_MODE_SPLIT_BF16>), grid, block, 0, st, g); break;
+    case TK_MODE_PLAIN: TK_F32_LAUNCH(TK_MODE_PLAIN); break;
+    case TK_MODE_BF16: TK_F32_LAUNCH(TK_MODE_BF16); break;
+    case TK_MODE_F16: TK_F32_LAUNCH(TK_MODE_F16); break;
+    default: TK_F32_LAUNCH(TK_MODE_SPLIT_BF16); break;
     }
+#undef TK_F32_LAUNCH
     if (g.ksplit > 1) {
-        const int64_t total = g.M * g.N * g.L;
+        const int64_t total = tiles * g.L * 16384;
         int64_t blocks = (total + 255) / 256;
         if (blocks > 65536) blocks = 65536;
         hipLaunchKernelGGL(k_splitk_reduce, dim3((unsigned)blocks), dim3(256), 0, st, g);
@@ -711,6 +748,13 @@ static hipError_t launch_splitk(const tk_gett& g, hipStream_t st)
 }
 
 template <typename R, bool CPLX, int S, int FM, int FN, int BK>
+static int64_t mfma_workgroups(const tk_gett& g)
+{
+    constexpr int BM = 2 * FM * S, BN = 2 * FN * S;
+    return ((g.M + BM - 1) / BM) * ((g.N + BN - 1) / BN) * g.L;
+}
+
+template <typename R, bool CPLX, int S, int FM, int FN, int BK>
 static hipError_t launch_mfma(const tk_gett& g, hipStream_t st)
 {
     constexpr int BM = 2 * FM * S, BN = 2 * FN * S;
@@ -738,15 +782,23 @@ extern "C" int tk_launch_gett(const tk_gett* gp, void* stream)
     case TK_F64:
         if (kern == TK_KERNEL_SIMPLE) return launch_simple<double, false>(g, st);
         if (kern == TK_KERNEL_SPLITK) return launch_splitk<double, false>(g, st);
-        return launch_mfma<double, false, 16, 4, 4, 8>(g, st);
+        // large tiles while they give every CU 4 workgroups, else 4x smaller
+        // ones: more workgroups and waves in flight for short-K problems
+        if (mfma_workgroups<double, false, 16, 4, 4, 8>(g) >= 4 * (int64_t)g.cus)
+            return launch_mfma<double, false, 16, 4, 4, 8>(g, st);
+        return launch_mfma<double, false, 16, 2, 2, 8>(g, st);
     case TK_C32:
         if (kern == TK_KERNEL_SIMPLE || g.mode != TK_MODE_PLAIN) return launch_simple<float, true>(g, st);
         if (kern == TK_KERNEL_SPLITK) return launch_splitk<float, true>(g, st);
-        return launch_mfma<float, true, 32, 2, 2, 8>(g, st);
+        if (mfma_workgroups<float, true, 32, 2, 2, 8>(g) >= 4 * (int64_t)g.cus)
+            return launch_mfma<float, true, 32, 2, 2, 8>(g, st);
+        return launch_mfma<float, true, 32, 1, 1, 8>(g, st);
     case TK_C64:
         if (kern == TK_KERNEL_SIMPLE) return launch_simple<double, true>(g, st);
         if (kern == TK_KERNEL_SPLITK) return launch_splitk<double, true>(g, st);
-        return launch_mfma<double, true, 16, 2, 2, 8>(g, st);
+        if (mfma_workgroups<double, true, 16, 2, 2, 8>(g) >= 4 * (int64_t)g.cus)
+            return launch_mfma<double, true, 16, 2, 2, 8>(g, st);
+        return launch_mfma<double, true, 16, 1, 1, 8>(g, st);
     default:
         return (int)hipErrorInvalidValue;
     }

@@ -60,14 +60,20 @@ __device__ __forceinline__ float rnd_in(float x, int mode)
     return x;
 }
 
-__device__ __forceinline__ void group_epilogue(const GView& p, int64_t l, int64_t s, int64_t g, float acc)
+// C value of output (l, s, g) as the epilogue scales it.  Tiles read all the
+// C values of a lane before any of its D stores: C and D may alias, so a
+// load issued after a store has to wait for it.
+__device__ __forceinline__ float group_c(const GView& p, int64_t l, int64_t s, int64_t g)
+{
+    if (p.beta_zero) return 0.0f;
+    float c = p.C[p.lC[l] + p.sC[s] + p.gC[g]];
+    return p.round_C ? rnd_in(c, p.round) : c;
+}
+
+__device__ __forceinline__ void group_store(const GView& p, int64_t l, int64_t s, int64_t g, float acc, float c)
 {
     float out = p.alpha * acc;
-    if (!p.beta_zero) {
-        float c = p.C[p.lC[l] + p.sC[s] + p.gC[g]];
-        if (p.round_C) c = rnd_in(c, p.round);
-        out += p.beta * c;
-    }
+    if (!p.beta_zero) out += p.beta * c;
     p.D[p.lD[l] + p.sD[s] + p.gD[g]] = out;
 }
 
@@ -133,9 +139,12 @@ __device__ __forceinline__ void thin_tile(const GView& p, int64_t l, int64_t g0,
     }
     const int64_t g = g0 + tid;
     if (g < p.G) {
+        float cv[SB];
+#pragma unroll
+        for (int s = 0; s < SB; ++s) cv[s] = s < p.S ? group_c(p, l, s, g) : 0.0f;
 #pragma unroll
         for (int s = 0; s < SB; ++s)
-            if (s < p.S) group_epilogue(p, l, s, g, acc[s]);
+            if (s < p.S) group_store(p, l, s, g, acc[s], cv[s]);
     }
 }
 
@@ -219,10 +228,16 @@ __device__ __forceinline__ void mfma_tile(const GView& p, int64_t l, int64_t m0,
     }
     const int64_t m = m0 + wm * 32 + il;
     if (m < p.S) {
+        float cv[16];
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
             const int64_t n = n0 + wn * 32 + 8 * (r >> 2) + 4 * kl + (r & 3);
-            if (n < p.G) group_epilogue(p, l, m, n, acc[r]);
+            cv[r] = n < p.G ? group_c(p, l, m, n) : 0.0f;
+        }
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int64_t n = n0 + wn * 32 + 8 * (r >> 2) + 4 * kl + (r & 3);
+            if (n < p.G) group_store(p, l, m, n, acc[r], cv[r]);
         }
     }
 }

@@ -20,14 +20,24 @@ __global__ __launch_bounds__(256) void k_prep(tk_prep p)
         const int64_t i0 = o % p.n0, t = o / p.n0, i1 = t % p.n1, i2 = t / p.n1;
         const int64_t base = p.t0[i0] + p.t1[i1] + p.t2[i2];
         R sr = 0, si = 0;
-        for (int64_t u = 0; u < p.nu; ++u) {
-            double re, im;
-            load_storage(p.src, p.src_type, base + p.tu[u], re, im);
-            R vr = (R)re, vi = (R)im;
-            if (p.round) { vr = round_mode(vr, p.round); vi = round_mode(vi, p.round); }
-            if (p.conj) vi = -vi;
-            sr += vr;
-            if (CPLX) si += vi;
+        // loads in batches of UB (all in flight together), added in u order
+        constexpr int UB = 8;
+        for (int64_t u0 = 0; u0 < p.nu; u0 += UB) {
+            double re[UB], im[UB];
+#pragma unroll
+            for (int j = 0; j < UB; ++j) {
+                re[j] = 0; im[j] = 0;
+                if (u0 + j < p.nu) load_storage(p.src, p.src_type, base + p.tu[u0 + j], re[j], im[j]);
+            }
+#pragma unroll
+            for (int j = 0; j < UB; ++j) {
+                if (u0 + j >= p.nu) break;
+                R vr = (R)re[j], vi = (R)im[j];
+                if (p.round) { vr = round_mode(vr, p.round); vi = round_mode(vi, p.round); }
+                if (p.conj) vi = -vi;
+                sr += vr;
+                if (CPLX) si += vi;
+            }
         }
         Elem<R, CPLX>::st(p.dst, o, Val<R, CPLX>{sr, si});
     }

@@ -382,7 +382,8 @@ int tk_execute_on(struct tapp_plan* p, struct tapp_exec* ex, const void* alpha, 
 
     /* split-K for the FP32-data MFMA kernel when its 128 x 128 tiles cannot
      * fill the GPU: workgroups take K slices (multiples of its 32-deep K
-     * tile), FP32 partial sums are added in a fixed order by a second kernel */
+     * tile) and store FP32 partial sums of whole tiles, which a second
+     * kernel adds in a fixed order */
     int64_t ks = 1, kchunk = 0;
     size_t part_bytes = 0;
     if (kern == TK_KERNEL_MFMA && ct == TK_F32 && K >= 64) {
@@ -394,7 +395,7 @@ int tk_execute_on(struct tapp_plan* p, struct tapp_exec* ex, const void* alpha, 
             if (ks > 64) ks = 64;
             kchunk = ((K + ks - 1) / ks + 31) / 32 * 32;
             ks = (K + kchunk - 1) / kchunk;
-            part_bytes = sizeof(float) * (size_t)(ks * outs);
+            part_bytes = sizeof(float) * (size_t)(ks * tiles * 128 * 128);   /* whole tiles */
             if (ks < 2 || part_bytes > ((size_t)64 << 20)) { ks = 1; part_bytes = 0; }
         }
     }
@@ -503,6 +504,7 @@ int tk_execute_on(struct tapp_plan* p, struct tapp_exec* ex, const void* alpha, 
     }
 
     g.kernel = kern;
+    g.cus = ex->cus;
 
     /* tile access patterns for the MFMA kernel */
     if (kern == TK_KERNEL_MFMA) {

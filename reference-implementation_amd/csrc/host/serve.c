@@ -327,6 +327,7 @@ TAPP_error TAPP_AMD_create_server(TAPP_AMD_server* server, TAPP_executor exec, s
         e->kind = 1;
         e->magic = TAPP_AMD_EXEC_MAGIC;
         e->device = base->device;
+        e->cus = base->cus;
         e->split_bf16 = base->split_bf16;
         e->force_kernel = base->force_kernel;
         if (base->stream) {

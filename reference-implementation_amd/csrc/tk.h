@@ -61,6 +61,7 @@ typedef struct tk_gett {
     int d_contig;            /* D (and C if c_contig) unit step along M inside each tile */
     int c_contig;
     int kernel;              /* chosen kernel variant, TK_KERNEL_* */
+    int cus;                 /* compute units of the device (tile-size choice) */
     int ksplit;              /* > 1: K split over gridDim.z; partial sums go to `partial` */
     int64_t kchunk;          /* K length per split (multiple of the kernel's K tile) */
     float* partial;          /* [ksplit][L][N][M] FP32 partial sums (real compute only) */

@@ -151,6 +151,24 @@ def test_forced_kernels_on_gemm_shapes(lib, oracle, dtype, kernel):
     lib.destroy_executor(ex)
 
 
+@pytest.mark.parametrize("dtype,M", [(tc.F64, 512), (tc.C32, 512), (tc.C64, 256)])
+def test_large_tile_variants(lib, gpu_exec, oracle, dtype, M):
+    """Batched products with at least 4 large-tile workgroups per CU (1024
+    of them on 256 CUs) run the large-tile MFMA kernels; the small-tile ones
+    are what the shapes of the other tests reach."""
+    L, K = 64, 12
+    A = T.make_operand(dtype, [M, K, L], "mkl", 81)
+    B = T.make_operand(dtype, [L, K, M], "lkn", 82, op=tc.CONJUGATE if dtype in T.COMPLEX_TYPES else 0)
+    C = T.make_operand(dtype, [M, M, L], "mnl", 83)
+    D = T.make_operand(dtype, [M, M, L], "mnl", 84)
+    alpha, beta = (0.5 - 0.25j, 0.75 + 0.5j) if dtype in T.COMPLEX_TYPES else (0.5, 0.75)
+    c = T.Case(A, B, C, D, tc.DEFAULT_PREC, alpha, beta, name=f"large_tiles_{dtype}")
+    rc, got, want = run_both(lib, gpu_exec, oracle, c)
+    assert rc == 0
+    assert lib.attr_get_int(gpu_exec, tc.ATTR_LAST_KERNEL)[1] == tc.KERNEL_MFMA
+    assert_parity(got, want, c)
+
+
 @pytest.mark.parametrize("mode", ["plain", "bf16", "f16", "split_bf16"])
 def test_split_k_few_tiles(lib, oracle, mode):
     """Few 128 x 128 output tiles and long K: the FP32 MFMA kernel splits K
