@@ -118,6 +118,25 @@ TAPP_EXPORT TAPP_error TAPP_AMD_server_drain(TAPP_AMD_server server);
 TAPP_EXPORT TAPP_error TAPP_AMD_server_stats(TAPP_AMD_server server, double stats[11], int reset);
 TAPP_EXPORT TAPP_error TAPP_AMD_destroy_server(TAPP_AMD_server server);
 
+/* --- resident tensors ----------------------------------------------------
+ * Handle-scoped device copies of host buffers (handle.h:9-24 leaves the
+ * handle's contents to the implementation).  After
+ * TAPP_AMD_make_resident(handle, host, bytes), every execute of a plan created
+ * with `handle` -- TAPP_execute_product, product groups, the serving loop --
+ * that passes, as A, B or C, a host pointer whose accessed bytes lie inside
+ * [host, host + bytes) reads the device copy instead of staging over PCIe.
+ * The copy is taken now and refreshed only by TAPP_AMD_update_resident; D is
+ * always written through the caller's pointer.  Ranges of one handle must not
+ * overlap.  Release (or destroy the handle) only after the requests and
+ * product groups that read a copy have completed or been destroyed.
+ * Errors: TAPP_AMD_ERR_INVALID (not this library's handle, overlapping or
+ * unknown range, device pointer), TAPP_AMD_ERR_NOMEM, TAPP_AMD_ERR_DEVICE. */
+TAPP_EXPORT TAPP_error TAPP_AMD_make_resident(TAPP_handle handle, const void* host, size_t bytes);
+TAPP_EXPORT TAPP_error TAPP_AMD_update_resident(TAPP_handle handle, const void* host);
+TAPP_EXPORT TAPP_error TAPP_AMD_release_resident(TAPP_handle handle, const void* host);
+/* info[0] resident buffers, info[1] their bytes. */
+TAPP_EXPORT TAPP_error TAPP_AMD_resident_info(TAPP_handle handle, int64_t info[2]);
+
 /* Text of the last device error of the calling thread (which HIP call
  * failed and why), with TAPP_explain_error's maxlen contract. */
 TAPP_EXPORT size_t TAPP_AMD_last_error_message(char* message, size_t maxlen);

@@ -254,15 +254,6 @@ void tk_problem_host_mn(struct tapp_plan* plan, const int64_t** mA, const int64_
 
 /* --- residency ------------------------------------------------------------ */
 
-static bool is_device_ptr(const void* p)
-{
-    if (!p) return false;
-    hipPointerAttribute_t a;
-    hipError_t e = hipPointerGetAttributes(&a, p);
-    (void)hipGetLastError();   /* an unregistered host pointer reports an error: not sticky */
-    if (e != hipSuccess) return false;
-    return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged;
-}
 
 static int ensure_ws(struct tapp_exec* ex, size_t bytes)
 {
@@ -351,13 +342,21 @@ int tk_execute_on(struct tapp_plan* p, struct tapp_exec* ex, const void* alpha, 
     const size_t cs = ctype_size(ct);
 
     /* what has to be staged or converted */
+    /* device operands are used in place, host ranges resident on the plan's
+     * handle are read from their copies (not D: it is written through the
+     * caller's pointer), other host operands are staged over their span */
     const void* src[4] = {A, B, P.beta_zero ? NULL : C, D};
     bool on_dev[4];
     size_t span_bytes[4] = {0, 0, 0, 0};
     for (int x = 0; x < 4; x++) {
-        on_dev[x] = src[x] ? is_device_ptr(src[x]) : true;
-        if (src[x] && !on_dev[x] && P.hi[x] >= P.lo[x])
-            span_bytes[x] = (size_t)(P.hi[x] - P.lo[x] + 1) * tk_type_size(P.type[x]);
+        on_dev[x] = true;
+        if (!src[x]) continue;
+        const int64_t es = (int64_t)tk_type_size(P.type[x]);
+        const void* r = x < 3 ? tk_resolve_operand(p->handle, src[x], P.lo[x] * es, (P.hi[x] + 1) * es)
+                              : (tk_is_device_ptr(src[x]) ? src[x] : NULL);
+        if (r) { src[x] = r; continue; }
+        on_dev[x] = false;
+        if (P.hi[x] >= P.lo[x]) span_bytes[x] = (size_t)(P.hi[x] - P.lo[x] + 1) * (size_t)es;
     }
     const bool needA = P.nUA > 1 || !compatible(P.type[0], ct) || (P.op_A && P.nUA > 1);
     const bool needB = P.nUB > 1 || !compatible(P.type[1], ct);

@@ -39,14 +39,6 @@ struct tapp_group {
     void** rest_D;
 };
 
-static bool device_ptr(const void* p)
-{
-    if (!p) return false;
-    hipPointerAttribute_t a;
-    hipError_t e = hipPointerGetAttributes(&a, p);
-    (void)hipGetLastError();
-    return e == hipSuccess && (a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged);
-}
 
 /* How the persistent kernel reads Y: 0 = along columns (unit step across
  * g), 1 = along K (unit step in k), 2 = dense (k[i] = i and g[j] = j * K:
@@ -79,7 +71,11 @@ bool tk_group_describe(struct tapp_plan* plan, struct tapp_exec* ex, const tk_pr
     if (P->nUA != 1 || P->nUB != 1) return false;
     for (int x = 0; x < 4; x++)
         if (P->type[x] != 0 && !(x == 2 && P->beta_zero)) return false;
-    if (!device_ptr(A) || !device_ptr(B) || !device_ptr(D) || (!P->beta_zero && !device_ptr(C))) return false;
+    /* device operands, or host ranges resident on the plan's handle (A, B, C) */
+    A = tk_resolve_operand(plan->handle, A, P->lo[0] * 4, (P->hi[0] + 1) * 4);
+    B = tk_resolve_operand(plan->handle, B, P->lo[1] * 4, (P->hi[1] + 1) * 4);
+    if (!P->beta_zero) C = tk_resolve_operand(plan->handle, C, P->lo[2] * 4, (P->hi[2] + 1) * 4);
+    if (!A || !B || !tk_is_device_ptr(D) || (!P->beta_zero && !C)) return false;
     const int64_t *mA, *kA, *lA, *nB, *kB, *lB, *mD, *nD, *lD, *mC, *nC, *lC;
     if ((*rc = tk_problem_tables(plan, P, ex->device, &mA, &kA, &lA, &nB, &kB, &lB, &mD, &nD, &lD, &mC, &nC, &lC)))
         return false;

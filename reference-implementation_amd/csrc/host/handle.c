@@ -1,7 +1,9 @@
 /*
  * Library handle (reference: handle.c:9-17, a bare malloc).  Here it records
- * the HIP device current at creation.  Plans keep the handle by value and
- * execution never requires it, as in the reference.
+ * the HIP device current at creation and owns the handle's resident tensors
+ * (resident.c).  Plans keep the handle by value; execution needs it only to
+ * find resident operands, so a plan whose handle is not one of this
+ * library's still executes, staging host operands as usual.
  */
 #include <stdlib.h>
 
@@ -15,13 +17,17 @@ TAPP_error TAPP_create_handle(TAPP_handle* handle)
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) dev = 0;
     h->device = dev;
+    tk_handle_register(h);
     *handle = (TAPP_handle)h;
     return 0;
 }
 
 TAPP_error TAPP_destroy_handle(TAPP_handle handle)
 {
-    free((void*)handle);
+    struct tapp_handle_obj* h = (struct tapp_handle_obj*)handle;
+    if (!h) return 0;
+    tk_handle_unregister(h);
+    free(h);
     return 0;
 }
 

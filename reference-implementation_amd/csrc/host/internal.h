@@ -62,9 +62,13 @@ struct tapp_exec {
     double last_device_us;
 };
 
+/* Handle: the device current at creation and the table of resident tensors
+ * (resident.c). */
 struct tapp_handle_obj {
     uint32_t magic;
     int device;
+    void* res;                 /* struct resident[cap] */
+    int nres, cap;
 };
 
 /* --- plan stage ---------------------------------------------------------- */
@@ -102,6 +106,15 @@ int tk_plan_problem(const struct tapp_plan* p, const struct tapp_exec* ex, const
 /* storage helpers */
 size_t tk_type_size(int type);
 bool tk_type_complex(int type);
+
+/* operand residency (resident.c) */
+bool tk_is_device_ptr(const void* p);
+/* Device address to read operand p at, whose accessed bytes are
+ * [p + lo_bytes, p + hi_bytes): p if it is device memory, the resident copy if
+ * the handle holds that range, NULL for host memory to be staged. */
+const void* tk_resolve_operand(TAPP_handle handle, const void* p, int64_t lo_bytes, int64_t hi_bytes);
+void tk_handle_register(struct tapp_handle_obj* h);
+void tk_handle_unregister(struct tapp_handle_obj* h);
 
 /* registry of live executors / handles */
 struct tapp_exec* tk_exec_lookup(TAPP_executor e);

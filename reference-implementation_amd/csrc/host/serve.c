@@ -94,14 +94,6 @@ static double now_us(void)
     return (double)t.tv_sec * 1e6 + (double)t.tv_nsec * 1e-3;
 }
 
-static bool on_device(const void* p)
-{
-    if (!p) return false;
-    hipPointerAttribute_t a;
-    hipError_t e = hipPointerGetAttributes(&a, p);
-    (void)hipGetLastError();
-    return e == hipSuccess && (a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged);
-}
 
 static void release(srv_request* r)
 {
@@ -365,9 +357,13 @@ static int footprint(struct tapp_plan* p, const void* alpha, const void* A, cons
     if (rc) return rc;
     size_t f = 0;
     const void* ops[4] = {A, B, P.beta_zero ? NULL : C, D};
-    for (int x = 0; x < 4; x++)
-        if (ops[x] && !on_device(ops[x]) && P.hi[x] >= P.lo[x])
-            f += (size_t)(P.hi[x] - P.lo[x] + 1) * tk_type_size(P.type[x]);
+    for (int x = 0; x < 4; x++) {
+        if (!ops[x] || P.hi[x] < P.lo[x]) continue;
+        const int64_t es = (int64_t)tk_type_size(P.type[x]);
+        const bool dev = x < 3 ? tk_resolve_operand(p->handle, ops[x], P.lo[x] * es, (P.hi[x] + 1) * es) != NULL
+                               : tk_is_device_ptr(ops[x]);
+        if (!dev) f += (size_t)(P.hi[x] - P.lo[x] + 1) * (size_t)es;
+    }
     size_t cs = (P.ctype == TK_F32) ? 4 : (P.ctype == TK_C64) ? 16 : 8;
     if (P.nUA > 1 || P.type[0] != (P.ctype == TK_F32 ? 0 : P.ctype == TK_F64 ? 1 : P.ctype == TK_C32 ? 2 : 3))
         f += (size_t)(P.nM * P.nK * P.nL) * cs;

@@ -27,6 +27,9 @@ ATTR_LAST_KERNEL = 0x1003
 ATTR_LAST_DEVICE_US = 0x1004
 ATTR_STREAM = 0x1005
 ATTR_TIMING = 0x1006
+# this library's error codes (tapp_amd.h)
+AMD_ERR_UNSUPPORTED, AMD_ERR_DEVICE, AMD_ERR_NOMEM, AMD_ERR_INVALID, AMD_ERR_KEY, AMD_ERR_BUDGET, AMD_ERR_BUSY = \
+    range(101, 108)
 KERNEL_AUTO, KERNEL_SIMPLE, KERNEL_MFMA, KERNEL_SPLITK, KERNEL_GROUP = 0, 1, 2, 3, 4
 
 TYPE_SIZE = {F32: 4, F64: 8, C32: 8, C64: 16, F16: 2, BF16: 2}
@@ -71,7 +74,8 @@ class TappLib:
         "TAPP_AMD_create_product_group", "TAPP_AMD_execute_product_group", "TAPP_AMD_destroy_product_group",
         "TAPP_AMD_group_info", "TAPP_AMD_create_server", "TAPP_AMD_server_submit", "TAPP_AMD_status_wait",
         "TAPP_AMD_server_cancel", "TAPP_AMD_server_set_paused", "TAPP_AMD_server_drain", "TAPP_AMD_server_stats",
-        "TAPP_AMD_destroy_server",
+        "TAPP_AMD_destroy_server", "TAPP_AMD_make_resident", "TAPP_AMD_update_resident",
+        "TAPP_AMD_release_resident", "TAPP_AMD_resident_info",
     )
 
     def __init__(self, path=None):
@@ -137,6 +141,10 @@ class TappLib:
             L.TAPP_AMD_server_drain.argtypes = [_H]
             L.TAPP_AMD_server_stats.argtypes = [_H, ctypes.POINTER(ctypes.c_double), ctypes.c_int]
             L.TAPP_AMD_destroy_server.argtypes = [_H]
+            L.TAPP_AMD_make_resident.argtypes = [_H, ctypes.c_void_p, ctypes.c_size_t]
+            L.TAPP_AMD_update_resident.argtypes = [_H, ctypes.c_void_p]
+            L.TAPP_AMD_release_resident.argtypes = [_H, ctypes.c_void_p]
+            L.TAPP_AMD_resident_info.argtypes = [_H, _i64p]
 
     # --- objects ---------------------------------------------------------
     def create_info(self, dtype, extents, strides):
@@ -272,6 +280,21 @@ class TappLib:
 
     def destroy_server(self, srv):
         return self.lib.TAPP_AMD_destroy_server(srv)
+
+    # resident tensors
+    def make_resident(self, handle, host_ptr, nbytes):
+        return self.lib.TAPP_AMD_make_resident(handle, host_ptr, nbytes)
+
+    def update_resident(self, handle, host_ptr):
+        return self.lib.TAPP_AMD_update_resident(handle, host_ptr)
+
+    def release_resident(self, handle, host_ptr):
+        return self.lib.TAPP_AMD_release_resident(handle, host_ptr)
+
+    def resident_info(self, handle):
+        info = (ctypes.c_int64 * 2)()
+        rc = self.lib.TAPP_AMD_resident_info(handle, info)
+        return rc, list(info)
 
     def plan_info(self, plan, exec_, beta_ptr, c_ptr):
         sizes = (ctypes.c_int64 * 6)()
