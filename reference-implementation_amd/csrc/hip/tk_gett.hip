@@ -7,13 +7,18 @@
 //             Hadamard-only products where an MFMA tile would be mostly empty.
 //  k_splitk   one workgroup per output element, K split over 256 lanes and
 //             reduced through LDS: very few outputs with a long K.
-//  k_gett_f32 FP32 data, load modes as template parameters; PLAIN runs
-//             v_mfma_f32_32x32x2f32, BF16 / F16 / SPLIT_BF16 convert in
-//             registers and run v_mfma_f32_32x32x16_{bf16,f16}.
+//  k_gett_f32 FP32, v_mfma_f32_32x32x2f32 on 128 x 128 x 32 tiles with the
+//             load modes as template parameters.
+//  k_gett_lp  FP32 data with 16-bit MFMA inputs (BF16, F16, split-BF16):
+//             converted once on the way into 16-bit K-major LDS tiles,
+//             v_mfma_f32_32x32x16_{bf16,f16}.
+//             Both run split-K (partial sums + k_splitk_reduce) when their
+//             128 x 128 tiles alone cannot fill the GPU.
 //  k_mfma     LDS-tiled MFMA GETT, 256 lanes = 2x2 waves, double-buffered
 //             LDS, register prefetch of the next K tile, XCD-aware tile order.
 //             FP64: v_mfma_f64_16x16x4f64; complex: four real MFMAs per step
-//             on split re/im LDS planes; also the generic FP32 fallback.
+//             on split re/im LDS planes.  Large or 4x smaller tiles by how
+//             many workgroups the problem gives.
 //             The MFMA's first operand is the B tile and its second the A
 //             tile, so output lanes run along M, the unit-stride direction of
 //             D when the host has oriented the problem that way.
@@ -483,32 +488,74 @@ struct F32Tile {
     }
 };
 
-// 8 consecutive k values of one row of an LDS tile ([k][row] layout, pitch LD)
-// as a 16-bit MFMA operand; SPLIT also returns the bfloat16 remainder.
-template <int MODE, int LD>
-__device__ __forceinline__ void operand16(const float* t, int col, uint4& hi, uint4& lo)
+// Epilogue of a 128 x 128 FP32-accumulator tile (2 x 2 waves of 2 x 2
+// v_mfma 32x32 blocks): SPLIT stores the raw partial sums for
+// k_splitk_reduce, otherwise D = alpha * acc + beta * C (op_C / rounding per
+// the plan) through the offset tables.
+template <bool SPLIT>
+__device__ __forceinline__ void tile_epilogue(const tk_gett& g, const f32x16 (&acc)[2][2], int64_t l, int64_t m0,
+                                              int64_t n0, int64_t tile, int64_t ntiles, int wave, int lane)
 {
-    if (MODE == TK_MODE_F16) {
-        f16x8 h;
+    constexpr int S = 32, FM = 2, FN = 2;
+    const int wm = wave & 1, wn = wave >> 1;
+    const int il = lane & 31, kl = lane >> 5;
+    if (SPLIT) {
+        // raw partial sums in accumulator order: per (split, l, tile,
+        // wave) 16 float4 slots x 64 lanes, one 1 KB store per slot
+        float4* part = reinterpret_cast<float4*>(g.partial) +
+                       ((((int64_t)blockIdx.z * g.L + l) * ntiles + tile) * 4 + wave) * (16 * 64) + lane;
 #pragma unroll
-        for (int e = 0; e < 8; ++e) h[e] = (_Float16)t[e * LD + col];
-        hi = __builtin_bit_cast(uint4, h);
-    } else {
-        bf16x8 h, l;
+        for (int fn = 0; fn < FN; ++fn)
 #pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            const float x = t[e * LD + col];
-            h[e] = (__bf16)x;                                  // v_cvt_pk_bf16_f32, round to nearest even
-            if (MODE == TK_MODE_SPLIT_BF16) l[e] = (__bf16)(x - (float)h[e]);
+            for (int fm = 0; fm < FM; ++fm)
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    part[((fn * FM + fm) * 4 + q) * 64] =
+                        make_float4(acc[fn][fm][4 * q], acc[fn][fm][4 * q + 1], acc[fn][fm][4 * q + 2],
+                                    acc[fn][fm][4 * q + 3]);
+        return;
+    }
+    const float alpha = (float)g.alpha[0];
+    const int64_t dl = g.lD[l], cl = g.beta_zero ? 0 : g.lC[l];
+#pragma unroll
+    for (int fm = 0; fm < FM; ++fm) {
+        const int64_t m = m0 + wm * FM * S + fm * S + il;
+        if (m >= g.M) continue;
+        const int64_t dm = dl + g.mD[m];
+        if (g.beta_zero) {
+#pragma unroll
+            for (int fn = 0; fn < FN; ++fn)
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    const int64_t n = n0 + wn * FN * S + fn * S + 8 * (r >> 2) + 4 * kl + (r & 3);
+                    if (n < g.N) ((float*)g.D)[dm + g.nD[n]] = alpha * acc[fn][fm][r];
+                }
+            continue;
         }
-        hi = __builtin_bit_cast(uint4, h);
-        if (MODE == TK_MODE_SPLIT_BF16) lo = __builtin_bit_cast(uint4, l);
+        // all C reads of this row block before its D writes
+        const int64_t cm = cl + g.mC[m];
+#pragma unroll
+        for (int fn = 0; fn < FN; ++fn) {
+            float cv[16];
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int64_t n = n0 + wn * FN * S + fn * S + 8 * (r >> 2) + 4 * kl + (r & 3);
+                cv[r] = n < g.N ? epilogue_c<float, false>(g, cm + g.nC[n]).re : 0.0f;
+            }
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int64_t n = n0 + wn * FN * S + fn * S + 8 * (r >> 2) + 4 * kl + (r & 3);
+                if (n < g.N)
+                    epilogue_d<float, false>(g, dm + g.nD[n], acc[fn][fm][r], 0.0f, Val<float, false>{cv[r], 0.0f});
+            }
+        }
     }
 }
 
 template <int AL, int BL, int MODE, bool SPLIT>
 __global__ __launch_bounds__(256, 2) void k_gett_f32(tk_gett g)
 {
+    static_assert(MODE == TK_MODE_PLAIN, "16-bit input modes run k_gett_lp");
     constexpr int S = 32, FM = 2, FN = 2, BM = 128, BN = 128, BK = 32;
     constexpr int LDA = BM + 4, LDB = BN + 4;
     __shared__ __attribute__((aligned(16))) float As[2][BK * LDA];
@@ -562,42 +609,18 @@ __global__ __launch_bounds__(256, 2) void k_gett_f32(tk_gett g)
             if (kt + 1 < nk) fetch(kt + 1);
             const float* as = As[cur];
             const float* bs = Bs[cur];
-            if (MODE == TK_MODE_PLAIN) {
 #pragma unroll
-                for (int kk = 0; kk < BK; kk += 2) {
-                    float a[FM], b[FN];
+            for (int kk = 0; kk < BK; kk += 2) {
+                float a[FM], b[FN];
 #pragma unroll
-                    for (int fm = 0; fm < FM; ++fm) a[fm] = as[(kk + kl) * LDA + wm * FM * S + fm * S + il];
+                for (int fm = 0; fm < FM; ++fm) a[fm] = as[(kk + kl) * LDA + wm * FM * S + fm * S + il];
 #pragma unroll
-                    for (int fn = 0; fn < FN; ++fn) b[fn] = bs[(kk + kl) * LDB + wn * FN * S + fn * S + il];
+                for (int fn = 0; fn < FN; ++fn) b[fn] = bs[(kk + kl) * LDB + wn * FN * S + fn * S + il];
 #pragma unroll
-                    for (int fn = 0; fn < FN; ++fn)
-#pragma unroll
-                        for (int fm = 0; fm < FM; ++fm)
-                            acc[fn][fm] = __builtin_amdgcn_mfma_f32_32x32x2f32(b[fn], a[fm], acc[fn][fm], 0, 0, 0);
-                }
-            } else {
-                // 16-bit MFMA: lane holds k = 8 * kl .. 8 * kl + 7 of its row
-#pragma unroll
-                for (int kk = 0; kk < BK; kk += 16) {
-                    uint4 ah[FM], al[FM], bh[FN], bl[FN];
+                for (int fn = 0; fn < FN; ++fn)
 #pragma unroll
                     for (int fm = 0; fm < FM; ++fm)
-                        operand16<MODE, LDA>(as + (kk + 8 * kl) * LDA, wm * FM * S + fm * S + il, ah[fm], al[fm]);
-#pragma unroll
-                    for (int fn = 0; fn < FN; ++fn)
-                        operand16<MODE, LDB>(bs + (kk + 8 * kl) * LDB, wn * FN * S + fn * S + il, bh[fn], bl[fn]);
-#pragma unroll
-                    for (int fn = 0; fn < FN; ++fn)
-#pragma unroll
-                        for (int fm = 0; fm < FM; ++fm) {
-                            acc[fn][fm] = mfma16<MODE>(bh[fn], ah[fm], acc[fn][fm]);
-                            if (MODE == TK_MODE_SPLIT_BF16) {
-                                acc[fn][fm] = mfma16<MODE>(bh[fn], al[fm], acc[fn][fm]);
-                                acc[fn][fm] = mfma16<MODE>(bl[fn], ah[fm], acc[fn][fm]);
-                            }
-                        }
-                }
+                        acc[fn][fm] = __builtin_amdgcn_mfma_f32_32x32x2f32(b[fn], a[fm], acc[fn][fm], 0, 0, 0);
             }
             if (kt + 1 < nk) {
                 la.template store<LDA>(As[cur ^ 1]);
@@ -606,57 +629,241 @@ __global__ __launch_bounds__(256, 2) void k_gett_f32(tk_gett g)
             __syncthreads();
         }
 
-        if (SPLIT) {
-            // raw partial sums in accumulator order: per (split, l, tile,
-            // wave) 16 float4 slots x 64 lanes, one 1 KB store per slot
-            float4* part = reinterpret_cast<float4*>(g.partial) +
-                           ((((int64_t)blockIdx.z * g.L + l) * (ntm * ntn) + tile) * 4 + wave) * (16 * 64) + lane;
+        tile_epilogue<SPLIT>(g, acc, l, m0, n0, tile, ntm * ntn, wave, lane);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_gett_lp: FP32 data computed with 16-bit MFMA inputs (BF16, F16, split-BF16)
+//
+// Each element is converted once, on its way into LDS, and LDS holds 16-bit
+// K-major tiles: row r of a 128 x 32 tile is 32 consecutive k at pitch LDK,
+// so an MFMA operand (8 consecutive k of one row) is one 16-byte read.  The
+// 4-k groups of a row are XOR-swizzled by (row / 16) % 4 (times 2, keeping
+// the two groups an operand reads adjacent): the 8-byte writes of 16 lanes
+// that start 16 rows apart then fall on distinct banks, and the 16-byte
+// reads of 8 consecutive rows are conflict-free with or without it.
+// Split-BF16 keeps a hi and a lo plane (x = hi + lo, hi = bf16(x),
+// lo = bf16(x - hi)) and runs hi*hi + hi*lo + lo*hi.
+// Each lane loads one 4-row x 4-k FP32 sub-block per operand per K tile:
+// float4 along rows (CONTIG_ROW), along k (CONTIG_K) or 16 table gathers.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int lp_swz(int row, int q) { return q ^ (2 * ((row >> 4) & 3)); }
+
+template <int MODE>
+__device__ __forceinline__ void lp_pack(const float (&x)[4], uint2& hi, uint2& lo)
+{
+    if (MODE == TK_MODE_F16) {
+        typedef _Float16 f16x4 __attribute__((ext_vector_type(4)));
+        f16x4 h;
 #pragma unroll
-            for (int fn = 0; fn < FN; ++fn)
+        for (int e = 0; e < 4; ++e) h[e] = (_Float16)x[e];      // round to nearest even
+        hi = __builtin_bit_cast(uint2, h);
+    } else {
+        typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+        bf16x4 h, l;
 #pragma unroll
-                for (int fm = 0; fm < FM; ++fm)
-#pragma unroll
-                    for (int q = 0; q < 4; ++q)
-                        part[((fn * FM + fm) * 4 + q) * 64] =
-                            make_float4(acc[fn][fm][4 * q], acc[fn][fm][4 * q + 1], acc[fn][fm][4 * q + 2],
-                                        acc[fn][fm][4 * q + 3]);
-            continue;
+        for (int e = 0; e < 4; ++e) {
+            h[e] = (__bf16)x[e];                                 // v_cvt_pk_bf16_f32, round to nearest even
+            if (MODE == TK_MODE_SPLIT_BF16) l[e] = (__bf16)(x[e] - (float)h[e]);
         }
-        const float alpha = (float)g.alpha[0];
-        const int64_t dl = g.lD[l], cl = g.beta_zero ? 0 : g.lC[l];
+        hi = __builtin_bit_cast(uint2, h);
+        if (MODE == TK_MODE_SPLIT_BF16) lo = __builtin_bit_cast(uint2, l);
+    }
+}
+
+// BK-deep tile of 128 rows; LDS pitch BK + 8 (16-byte aligned rows whose
+// 16-byte reads by 8 consecutive rows hit distinct banks).
+template <int LMODE, int BK>
+struct LpTile {
+    static constexpr int BR = 128, LDK = BK + 8;
+    static constexpr int NB = BR * BK / (256 * 16);  // 4 x 4 sub-blocks per lane
+    float v[NB][4][4];                               // [sub-block][row j][k e]
+    int64_t roff[LMODE == TK_CONTIG_ROW ? 1 : 4];    // the lane's 4 rows (shared by its sub-blocks)
+
+    // sub-block b of this lane: rows 4 * gi() .. +3, k 4 * gk(b) .. +3.
+    // Lanes adjacent in memory are adjacent in the lane index: along rows
+    // (CONTIG_ROW, GATHER) 32 lanes cover the 128 rows, along k (CONTIG_K)
+    // 8 lanes cover 32 consecutive k.
+    __device__ __forceinline__ static int gi() { return LMODE == TK_CONTIG_K ? threadIdx.x / 8 : threadIdx.x % 32; }
+    __device__ __forceinline__ static int gk(int b)
+    {
+        return LMODE == TK_CONTIG_K ? threadIdx.x % 8 + 8 * b : threadIdx.x / 32 + 8 * b;
+    }
+
+    __device__ __forceinline__ void rows(const int64_t* rt, int64_t r0, int64_t Rn)
+    {
+        const int64_t r = r0 + 4 * gi();
+        if (LMODE == TK_CONTIG_ROW) {
+            roff[0] = r < Rn ? rt[r] : 0;
+        } else {
 #pragma unroll
-        for (int fm = 0; fm < FM; ++fm) {
-            const int64_t m = m0 + wm * FM * S + fm * S + il;
-            if (m >= g.M) continue;
-            const int64_t dm = dl + g.mD[m];
-            if (g.beta_zero) {
+            for (int j = 0; j < 4; ++j) roff[j] = r + j < Rn ? rt[r + j] : 0;
+        }
+    }
+
+    template <bool CHECK>
+    __device__ __forceinline__ void load(const float* base, const int64_t* kt, int64_t r0, int64_t k0, int64_t Rn,
+                                         int64_t Kn)
+    {
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            const int64_t r = r0 + 4 * gi(), k = k0 + 4 * gk(b);
+            const int64_t* ro = roff;
+            if (LMODE == TK_CONTIG_ROW) {
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int64_t kk = k + e;
+                    if (!CHECK || (kk < Kn && r + 4 <= Rn)) {
+                        const float4 q = *reinterpret_cast<const float4*>(base + ro[0] + kt[kk]);
+                        v[b][0][e] = q.x; v[b][1][e] = q.y; v[b][2][e] = q.z; v[b][3][e] = q.w;
+                    } else {
+#pragma unroll
+                        for (int j = 0; j < 4; ++j)
+                            v[b][j][e] = (kk < Kn && r + j < Rn) ? base[ro[0] + kt[kk] + j] : 0.0f;
+                    }
+                }
+            } else if (LMODE == TK_CONTIG_K) {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    if (!CHECK || (r + j < Rn && k + 4 <= Kn)) {
+                        const float4 q = *reinterpret_cast<const float4*>(base + ro[j] + kt[k]);
+                        v[b][j][0] = q.x; v[b][j][1] = q.y; v[b][j][2] = q.z; v[b][j][3] = q.w;
+                    } else {
+#pragma unroll
+                        for (int e = 0; e < 4; ++e)
+                            v[b][j][e] = (r + j < Rn && k + e < Kn) ? base[ro[j] + kt[k + e]] : 0.0f;
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+#pragma unroll
+                    for (int e = 0; e < 4; ++e)
+                        v[b][j][e] = (!CHECK || (r + j < Rn && k + e < Kn)) ? base[ro[j] + kt[k + e]] : 0.0f;
+            }
+        }
+    }
+
+    // converted rows into the K-major 16-bit planes (hi; lo for split)
+    template <int MODE>
+    __device__ __forceinline__ void store(uint16_t* hi, uint16_t* lo) const
+    {
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int row = 4 * gi() + j;
+                const int off = row * LDK + 4 * lp_swz(row, gk(b));
+                uint2 h, l;
+                lp_pack<MODE>(v[b][j], h, l);
+                *reinterpret_cast<uint2*>(hi + off) = h;
+                if (MODE == TK_MODE_SPLIT_BF16) *reinterpret_cast<uint2*>(lo + off) = l;
+            }
+    }
+};
+
+// MFMA operand: 8 consecutive k (kk .. kk + 7, kk a multiple of 8) of one row
+template <int LDK>
+__device__ __forceinline__ uint4 lp_operand(const uint16_t* plane, int row, int kk)
+{
+    return *reinterpret_cast<const uint4*>(plane + row * LDK + 4 * lp_swz(row, kk >> 2));
+}
+
+template <int AL, int BL, int MODE, bool SPLIT>
+__global__ __launch_bounds__(256, 2) void k_gett_lp(tk_gett g)
+{
+    // K tile: 64 deep (twice the bytes in flight per step) where two
+    // workgroups per CU still fit in LDS; split-BF16's two planes need 32
+    constexpr int S = 32, FM = 2, FN = 2, BM = 128, BN = 128, BK = MODE == TK_MODE_SPLIT_BF16 ? 32 : 64;
+    constexpr int LDK = BK + 8;
+    constexpr int P = MODE == TK_MODE_SPLIT_BF16 ? 2 : 1;          // planes per operand
+    constexpr int PLANE = BM * LDK;                                 // 16-bit elements per plane
+    __shared__ __attribute__((aligned(16))) uint16_t lds[2][2 * P][PLANE];   // [buffer][A hi, (A lo), B hi, (B lo)]
+
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    const int il = lane & 31, kl = lane >> 5;
+
+    const int64_t ntm = (g.M + BM - 1) / BM, ntn = (g.N + BN - 1) / BN;
+    const int64_t tile = xcd_tile(blockIdx.x, ntm * ntn);
+    const int64_t m0 = (tile / ntn) * BM, n0 = (tile % ntn) * BN;
+    const bool interior = m0 + BM <= g.M && n0 + BN <= g.N;
+    // SPLIT: this workgroup covers K in [kb, ke) and writes raw partial sums
+    const int64_t kb = SPLIT ? (int64_t)blockIdx.z * g.kchunk : 0;
+    const int64_t ke = SPLIT ? (kb + g.kchunk < g.K ? kb + g.kchunk : g.K) : g.K;
+    const int64_t klen = ke > kb ? ke - kb : 0;
+    const int64_t nk = (klen + BK - 1) / BK, nk_full = klen / BK;
+
+    LpTile<AL, BK> la;
+    LpTile<BL, BK> lb;
+    la.rows(g.mA, m0, g.M);
+    lb.rows(g.nB, n0, g.N);
+
+    for (int64_t l = blockIdx.y; l < g.L; l += gridDim.y) {
+        const float* Ab = (const float*)g.A + g.lA[l];
+        const float* Bb = (const float*)g.B + g.lB[l];
+        f32x16 acc[FN][FM];
+#pragma unroll
+        for (int fn = 0; fn < FN; ++fn)
+#pragma unroll
+            for (int fm = 0; fm < FM; ++fm) acc[fn][fm] = f32x16{};
+
+        auto fetch = [&](int64_t kt) {
+            if (interior && kt < nk_full) {
+                la.template load<false>(Ab, g.kA, m0, kb + kt * BK, g.M, ke);
+                lb.template load<false>(Bb, g.kB, n0, kb + kt * BK, g.N, ke);
+            } else {
+                la.template load<true>(Ab, g.kA, m0, kb + kt * BK, g.M, ke);
+                lb.template load<true>(Bb, g.kB, n0, kb + kt * BK, g.N, ke);
+            }
+        };
+        auto stash = [&](int buf) {
+            la.template store<MODE>(lds[buf][0], lds[buf][P - 1]);
+            lb.template store<MODE>(lds[buf][P], lds[buf][2 * P - 1]);
+        };
+        if (nk > 0) {
+            fetch(0);
+            stash(0);
+        }
+        __syncthreads();
+        for (int64_t kt = 0; kt < nk; ++kt) {
+            const int cur = kt & 1;
+            if (kt + 1 < nk) fetch(kt + 1);
+            const uint16_t* ah = lds[cur][0];
+            const uint16_t* al = lds[cur][P - 1];
+            const uint16_t* bh = lds[cur][P];
+            const uint16_t* bl = lds[cur][2 * P - 1];
+#pragma unroll
+            for (int kk = 0; kk < BK; kk += 16) {
+                uint4 a_h[FM], a_l[FM], b_h[FN], b_l[FN];
+#pragma unroll
+                for (int fm = 0; fm < FM; ++fm) {
+                    const int row = wm * FM * S + fm * S + il;
+                    a_h[fm] = lp_operand<LDK>(ah, row, kk + 8 * kl);
+                    if (MODE == TK_MODE_SPLIT_BF16) a_l[fm] = lp_operand<LDK>(al, row, kk + 8 * kl);
+                }
+#pragma unroll
+                for (int fn = 0; fn < FN; ++fn) {
+                    const int row = wn * FN * S + fn * S + il;
+                    b_h[fn] = lp_operand<LDK>(bh, row, kk + 8 * kl);
+                    if (MODE == TK_MODE_SPLIT_BF16) b_l[fn] = lp_operand<LDK>(bl, row, kk + 8 * kl);
+                }
 #pragma unroll
                 for (int fn = 0; fn < FN; ++fn)
 #pragma unroll
-                    for (int r = 0; r < 16; ++r) {
-                        const int64_t n = n0 + wn * FN * S + fn * S + 8 * (r >> 2) + 4 * kl + (r & 3);
-                        if (n < g.N) ((float*)g.D)[dm + g.nD[n]] = alpha * acc[fn][fm][r];
+                    for (int fm = 0; fm < FM; ++fm) {
+                        acc[fn][fm] = mfma16<MODE>(b_h[fn], a_h[fm], acc[fn][fm]);
+                        if (MODE == TK_MODE_SPLIT_BF16) {
+                            acc[fn][fm] = mfma16<MODE>(b_h[fn], a_l[fm], acc[fn][fm]);
+                            acc[fn][fm] = mfma16<MODE>(b_l[fn], a_h[fm], acc[fn][fm]);
+                        }
                     }
-                continue;
             }
-            // all C reads of this row block before its D writes
-            const int64_t cm = cl + g.mC[m];
-#pragma unroll
-            for (int fn = 0; fn < FN; ++fn) {
-                float cv[16];
-#pragma unroll
-                for (int r = 0; r < 16; ++r) {
-                    const int64_t n = n0 + wn * FN * S + fn * S + 8 * (r >> 2) + 4 * kl + (r & 3);
-                    cv[r] = n < g.N ? epilogue_c<float, false>(g, cm + g.nC[n]).re : 0.0f;
-                }
-#pragma unroll
-                for (int r = 0; r < 16; ++r) {
-                    const int64_t n = n0 + wn * FN * S + fn * S + 8 * (r >> 2) + 4 * kl + (r & 3);
-                    if (n < g.N)
-                        epilogue_d<float, false>(g, dm + g.nD[n], acc[fn][fm][r], 0.0f, Val<float, false>{cv[r], 0.0f});
-                }
-            }
+            if (kt + 1 < nk) stash(cur ^ 1);
+            __syncthreads();
         }
+        tile_epilogue<SPLIT>(g, acc, l, m0, n0, tile, ntm * ntn, wave, lane);
     }
 }
 
@@ -688,16 +895,16 @@ static hipError_t launch_gett_f32_modes(const tk_gett& g, hipStream_t st)
     const int64_t ly = g.L < 65535 ? g.L : 65535;
     const dim3 grid((unsigned)tiles, (unsigned)ly, (unsigned)(g.ksplit > 1 ? g.ksplit : 1)), block(256);
     const bool split = g.ksplit > 1;
-#define TK_F32_LAUNCH(MODE)                                                          \
-    if (split) hipLaunchKernelGGL((k_gett_f32<AL, BL, MODE, true>), grid, block, 0, st, g); \
-    else hipLaunchKernelGGL((k_gett_f32<AL, BL, MODE, false>), grid, block, 0, st, g)
+#define TK_LAUNCH(KERNEL, MODE)                                                       \
+    if (split) hipLaunchKernelGGL((KERNEL<AL, BL, MODE, true>), grid, block, 0, st, g); \
+    else hipLaunchKernelGGL((KERNEL<AL, BL, MODE, false>), grid, block, 0, st, g)
     switch (g.mode) {
-    case TK_MODE_PLAIN: TK_F32_LAUNCH(TK_MODE_PLAIN); break;
-    case TK_MODE_BF16: TK_F32_LAUNCH(TK_MODE_BF16); break;
-    case TK_MODE_F16: TK_F32_LAUNCH(TK_MODE_F16); break;
-    default: TK_F32_LAUNCH(TK_MODE_SPLIT_BF16); break;
+    case TK_MODE_PLAIN: TK_LAUNCH(k_gett_f32, TK_MODE_PLAIN); break;
+    case TK_MODE_BF16: TK_LAUNCH(k_gett_lp, TK_MODE_BF16); break;
+    case TK_MODE_F16: TK_LAUNCH(k_gett_lp, TK_MODE_F16); break;
+    default: TK_LAUNCH(k_gett_lp, TK_MODE_SPLIT_BF16); break;
     }
-#undef TK_F32_LAUNCH
+#undef TK_LAUNCH
     if (g.ksplit > 1) {
         const int64_t total = tiles * g.L * 16384;
         int64_t blocks = (total + 255) / 256;

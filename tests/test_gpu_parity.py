@@ -151,6 +151,34 @@ def test_forced_kernels_on_gemm_shapes(lib, oracle, dtype, kernel):
     lib.destroy_executor(ex)
 
 
+@pytest.mark.parametrize("mode", ["bf16", "f16", "split_bf16"])
+def test_low_precision_kernel_load_modes(lib, oracle, mode):
+    """The 16-bit-input GETT on GEMM shapes whose layouts reach every pair of
+    load modes (rows contiguous, k contiguous, gathered), with ragged edges
+    and beta != 0: same input rounding as the oracle, so only the FP32
+    summation order differs; split-BF16 against the FP64 oracle."""
+    prec = {"bf16": tc.BF16BF16_ACCUM_F32, "f16": tc.F16F16_ACCUM_F32, "split_bf16": tc.DEFAULT_PREC}[mode]
+    _, ex = lib.create_executor()
+    assert lib.attr_set_int(ex, tc.ATTR_KERNEL, tc.KERNEL_MFMA) == 0
+    if mode == "split_bf16":
+        assert lib.attr_set_int(ex, tc.ATTR_SPLIT_BF16, 1) == 0
+    shapes = [(200, 130, 257), (129, 70, 37), (256, 384, 96), (64, 64, 64)]
+    for i, (M, N, K) in enumerate(shapes):
+        for tA, tB, pad in ((False, False, 0), (True, True, 0), (True, False, 0), (False, True, 1)):
+            c = _gemm_case(tc.F32, M, N, K, 100 * i + 7, tA, tB, ldpad=pad)
+            c.prec = prec
+            rc, got = T.run_lib(lib, c, ex)
+            assert rc == 0
+            assert lib.attr_get_int(ex, tc.ATTR_LAST_KERNEL)[1] == tc.KERNEL_MFMA
+            if mode == "split_bf16":
+                _, exact = oracle.run(c, f64=True)
+                assert T.normwise_error(got, exact, T.written_mask(c.D)) < 1e-5, (M, N, K, tA, tB, pad)
+            else:
+                _, want = oracle.run(c)
+                assert_parity(got, want, c, tol=1e-5)
+    lib.destroy_executor(ex)
+
+
 @pytest.mark.parametrize("dtype,M", [(tc.F64, 512), (tc.C32, 512), (tc.C64, 256)])
 def test_large_tile_variants(lib, gpu_exec, oracle, dtype, M):
     """Batched products with at least 4 large-tile workgroups per CU (1024
