@@ -46,7 +46,16 @@ enum {
      * external stream, get to read it.  Value: pointer to a void*. */
     TAPP_AMD_ATTR_STREAM = 0x1005,
     /* 1: time every execute with HIP events (needed for LAST_DEVICE_US). */
-    TAPP_AMD_ATTR_TIMING = 0x1006
+    TAPP_AMD_ATTR_TIMING = 0x1006,
+    /* int: 0 = built-in kernel rules (default); 1 = autotune: measure the
+     * kernel configurations of a shape class the first time it runs (single
+     * products: kernel family, MFMA tile size, split-K; product groups: thin
+     * FMA or MFMA tiles) and keep the fastest in an on-disk cache
+     * ($TAPP_AMD_TUNE_CACHE, else ~/.cache/tapp_amd/tune-<arch>.txt);
+     * 2 = use cached choices, never measure.  Measuring runs the product
+     * several times, so it is skipped when C and D overlap with beta != 0
+     * and for requests of the serving loop. */
+    TAPP_AMD_ATTR_AUTOTUNE = 0x1007
 };
 
 /* Kernel family identifiers reported by TAPP_AMD_ATTR_LAST_KERNEL. */

@@ -991,19 +991,19 @@ extern "C" int tk_launch_gett(const tk_gett* gp, void* stream)
         if (kern == TK_KERNEL_SPLITK) return launch_splitk<double, false>(g, st);
         // large tiles while they give every CU 4 workgroups, else 4x smaller
         // ones: more workgroups and waves in flight for short-K problems
-        if (mfma_workgroups<double, false, 16, 4, 4, 8>(g) >= 4 * (int64_t)g.cus)
+        if (g.tile_variant == 1 || (g.tile_variant == 0 && mfma_workgroups<double, false, 16, 4, 4, 8>(g) >= 4 * (int64_t)g.cus))
             return launch_mfma<double, false, 16, 4, 4, 8>(g, st);
         return launch_mfma<double, false, 16, 2, 2, 8>(g, st);
     case TK_C32:
         if (kern == TK_KERNEL_SIMPLE || g.mode != TK_MODE_PLAIN) return launch_simple<float, true>(g, st);
         if (kern == TK_KERNEL_SPLITK) return launch_splitk<float, true>(g, st);
-        if (mfma_workgroups<float, true, 32, 2, 2, 8>(g) >= 4 * (int64_t)g.cus)
+        if (g.tile_variant == 1 || (g.tile_variant == 0 && mfma_workgroups<float, true, 32, 2, 2, 8>(g) >= 4 * (int64_t)g.cus))
             return launch_mfma<float, true, 32, 2, 2, 8>(g, st);
         return launch_mfma<float, true, 32, 1, 1, 8>(g, st);
     case TK_C64:
         if (kern == TK_KERNEL_SIMPLE) return launch_simple<double, true>(g, st);
         if (kern == TK_KERNEL_SPLITK) return launch_splitk<double, true>(g, st);
-        if (mfma_workgroups<double, true, 16, 2, 2, 8>(g) >= 4 * (int64_t)g.cus)
+        if (g.tile_variant == 1 || (g.tile_variant == 0 && mfma_workgroups<double, true, 16, 2, 2, 8>(g) >= 4 * (int64_t)g.cus))
             return launch_mfma<double, true, 16, 2, 2, 8>(g, st);
         return launch_mfma<double, true, 16, 1, 1, 8>(g, st);
     default:

@@ -270,6 +270,73 @@ static int ensure_ws(struct tapp_exec* ex, size_t bytes)
     return 0;
 }
 
+/* --- autotuning of single products (tune.c holds the cache) -------------
+ * A configuration packs the kernel family (bits 0-3), the F64/C32/C64 MFMA
+ * tile size (bits 4-7: 0 by workgroup count, 1 large, 2 small) and whether a
+ * split-K launch is turned off (bit 8). */
+static void apply_cfg(tk_gett* g, int cfg, int64_t ks, int64_t kchunk, float* partial)
+{
+    g->kernel = cfg & 15;
+    g->tile_variant = (cfg >> 4) & 15;
+    const bool split = g->kernel == TK_KERNEL_MFMA && g->ctype == TK_F32 && ks > 1 && !((cfg >> 8) & 1);
+    g->ksplit = split ? (int)ks : 0;
+    g->kchunk = split ? kchunk : 0;
+    g->partial = split ? partial : NULL;
+}
+
+typedef struct {
+    tk_gett* g;
+    hipStream_t st;
+} gett_ctx;
+
+static int launch_gett_ctx(void* c)
+{
+    gett_ctx* x = (gett_ctx*)c;
+    return tk_launch_gett(x->g, x->st);
+}
+
+/* Candidates in a fixed order, each timed over 3 launches after a warm-up;
+ * the fastest wins (ties keep the earlier one). */
+static int tune_gett(struct tapp_exec* ex, tk_gett* g, int64_t ks, int64_t kchunk, float* partial, int64_t outs,
+                     int* best)
+{
+    int cand[6], nc = 0;
+    const double macs = (double)outs * (double)g->K;
+    if (macs <= (double)(1 << 30)) cand[nc++] = TK_KERNEL_SIMPLE;
+    if (g->ctype == TK_F32) {
+        cand[nc++] = TK_KERNEL_MFMA;
+        if (ks > 1) cand[nc++] = TK_KERNEL_MFMA | (1 << 8);
+    } else {
+        cand[nc++] = TK_KERNEL_MFMA | (1 << 4);
+        cand[nc++] = TK_KERNEL_MFMA | (2 << 4);
+    }
+    if (outs <= 65536 && g->K >= 64) cand[nc++] = TK_KERNEL_SPLITK;
+    tk_gett t = *g;
+    gett_ctx ctx = {&t, ex->stream};
+    double best_us = -1.0;
+    *best = g->kernel;
+    for (int i = 0; i < nc; i++) {
+        apply_cfg(&t, cand[i], ks, kchunk, partial);
+        const double us = tk_tune_time(ex, launch_gett_ctx, &ctx, 3);
+        if (us < 0) return device_error("autotune launch", (int)hipGetLastError(), __LINE__);
+        if (best_us < 0 || us < best_us) { best_us = us; *best = cand[i]; }
+    }
+    return 0;
+}
+
+/* Measuring reruns the product: only when rerunning gives the same D, i.e.
+ * C is not read or is not the memory D is written to. */
+static bool tune_safe(const tk_problem* P, const void* const* src, bool needC)
+{
+    if (P->beta_zero || needC || !src[2]) return true;
+    const int64_t ec = (int64_t)tk_type_size(P->type[2]), ed = (int64_t)tk_type_size(P->type[3]);
+    const char* c0 = (const char*)src[2] + P->lo[2] * ec;
+    const char* c1 = (const char*)src[2] + (P->hi[2] + 1) * ec;
+    const char* d0 = (const char*)src[3] + P->lo[3] * ed;
+    const char* d1 = (const char*)src[3] + (P->hi[3] + 1) * ed;
+    return c1 <= d0 || d1 <= c0;
+}
+
 static size_t ctype_size(int ct)
 {
     switch (ct) {
@@ -383,9 +450,10 @@ int tk_execute_on(struct tapp_plan* p, struct tapp_exec* ex, const void* alpha, 
      * fill the GPU: workgroups take K slices (multiples of its 32-deep K
      * tile) and store FP32 partial sums of whole tiles, which a second
      * kernel adds in a fixed order */
+    const bool may_tune = ex->autotune && ex->force_kernel == TK_KERNEL_AUTO && K > 0;
     int64_t ks = 1, kchunk = 0;
     size_t part_bytes = 0;
-    if (kern == TK_KERNEL_MFMA && ct == TK_F32 && K >= 64) {
+    if ((kern == TK_KERNEL_MFMA || may_tune) && ct == TK_F32 && K >= 64) {
         const int64_t tiles = ((M + 127) / 128) * ((N + 127) / 128) * L;
         const int64_t target = 2 * (int64_t)ex->cus;
         if (tiles < target) {
@@ -506,7 +574,7 @@ int tk_execute_on(struct tapp_plan* p, struct tapp_exec* ex, const void* alpha, 
     g.cus = ex->cus;
 
     /* tile access patterns for the MFMA kernel */
-    if (kern == TK_KERNEL_MFMA) {
+    if (kern == TK_KERNEL_MFMA || may_tune) {
         const int ve = (int)(16 / cs) > 0 ? (int)(16 / cs) : 1;
         if (vec_rows_ok(hmA, g.M, hkA, g.K, hlA, g.L, ve, g.A)) { g.a_load = TK_CONTIG_ROW; g.a_vec = 1; }
         else if (vec_rows_ok(hkA, g.K, hmA, g.M, hlA, g.L, ve, g.A)) { g.a_load = TK_CONTIG_K; g.a_vec = 1; }
@@ -520,6 +588,30 @@ int tk_execute_on(struct tapp_plan* p, struct tapp_exec* ex, const void* alpha, 
         g.ksplit = (int)ks;
         g.kchunk = kchunk;
         g.partial = (float*)(ws + off_part);
+    }
+
+    /* autotuned shape class (opt-in): a cached choice replaces the rules; an
+     * unknown class is measured here when that is safe (synchronous call; C
+     * not overlapping D unless beta is zero or C was converted) */
+    if (may_tune) {
+        const int64_t f[] = {1, ct, P.mode, tk_tune_bucket(g.M), tk_tune_bucket(g.N), tk_tune_bucket(K),
+                             tk_tune_bucket(L), g.a_load, g.b_load, P.beta_zero, needD};
+        const uint64_t key = tk_tune_key(f, (int)(sizeof f / sizeof f[0]));
+        int cfg = 0;
+        bool have = tk_tune_get(ex->device, key, &cfg);
+        if (!have && ex->autotune == 1 && sync && tune_safe(&P, src, needC)) {
+            if ((rc = tune_gett(ex, &g, ks, kchunk, (float*)(ws + off_part), outs, &cfg))) return rc;
+            char desc[160];
+            snprintf(desc, sizeof desc, "gett ct=%d mode=%d M=%lld N=%lld K=%lld L=%lld load=%d,%d beta0=%d",
+                     ct, P.mode, (long long)g.M, (long long)g.N, (long long)K, (long long)L, g.a_load, g.b_load,
+                     (int)P.beta_zero);
+            tk_tune_put(ex->device, key, cfg, desc);
+            have = true;
+        }
+        if (have) {
+            apply_cfg(&g, cfg, ks, kchunk, (float*)(ws + off_part));
+            kern = g.kernel;
+        }
     }
     LAUNCHCHK(tk_launch_gett(&g, st));
     ex->last_kernel = kern;

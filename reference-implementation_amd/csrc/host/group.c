@@ -13,6 +13,7 @@
  * same path as TAPP_execute_product.
  */
 #include <stdint.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -53,7 +54,7 @@ static int y_access(const int64_t* kt, int64_t K, const int64_t* gt, int64_t G)
     return kt[0] == 0 ? 2 : 1;
 }
 
-typedef struct { tk_prob p; double cost; } ranked;
+typedef struct { tk_prob p; double cost; int idx; } ranked;
 
 static int by_cost(const void* x, const void* y)
 {
@@ -64,7 +65,7 @@ static int by_cost(const void* x, const void* y)
 /* Descriptor of one eligible product; returns false when the product needs
  * the general path. */
 bool tk_group_describe(struct tapp_plan* plan, struct tapp_exec* ex, const tk_problem* P, const void* A,
-                       const void* B, const void* C, void* D, tk_prob* out, int* rc)
+                       const void* B, const void* C, void* D, int force_cls, tk_prob* out, int* rc)
 {
     *rc = 0;
     if (P->ctype != TK_F32 || P->mode == TK_MODE_SPLIT_BF16) return false;
@@ -92,7 +93,8 @@ bool tk_group_describe(struct tapp_plan* plan, struct tapp_exec* ex, const tk_pr
     for (int i = 0; i < P->M.n; i++) if (llabs(P->M.s[3][i]) < dm) dm = llabs(P->M.s[3][i]);
     for (int i = 0; i < P->N.n; i++) if (llabs(P->N.s[3][i]) < dn) dn = llabs(P->N.s[3][i]);
     bool x_is_b;
-    if (N <= 16 || M <= 16) {
+    const bool thin_ok = N <= 16 || M <= 16;
+    if (thin_ok && force_cls != TK_GROUP_MFMA) {
         out->cls = TK_GROUP_THIN;
         x_is_b = N <= M;            /* the small side is X */
     } else {
@@ -123,6 +125,145 @@ bool tk_group_describe(struct tapp_plan* plan, struct tapp_exec* ex, const tk_pr
     if (out->cls == TK_GROUP_THIN) { out->tiles_s = 1; out->tiles_g = (out->G + 255) / 256; }
     else { out->tiles_s = (out->S + 63) / 64; out->tiles_g = (out->G + 63) / 64; }
     return true;
+}
+
+static double tile_cost(const tk_prob* d)
+{
+    const double per_tile = d->cls == TK_GROUP_THIN ? 256.0 * (double)d->K * (double)(d->S + 1)
+                                                    : 64.0 * 64.0 * (double)d->K;
+    return per_tile + 1.0;
+}
+
+/* --- autotuning: thin FMA tiles or MFMA tiles per shape class ----------- */
+
+typedef struct {
+    tk_prob* probs;
+    int64_t* tile_end;
+    unsigned long long* counter;
+    int n;
+    int64_t total;
+    int nblocks;
+    hipStream_t st;
+} group_ctx;
+
+static int launch_group_ctx(void* c)
+{
+    group_ctx* x = (group_ctx*)c;
+    return tk_launch_group(x->probs, x->tile_end, NULL, x->n, x->total, x->counter, x->nblocks, x->st);
+}
+
+/* Device time of one persistent launch over `n` descriptors. */
+static double time_descriptors(struct tapp_exec* ex, const tk_prob* h, int n)
+{
+    int64_t* he = malloc(sizeof(int64_t) * (size_t)n);
+    if (!he) return -1.0;
+    int64_t total = 0;
+    for (int i = 0; i < n; i++) { total += h[i].L * h[i].tiles_s * h[i].tiles_g; he[i] = total; }
+    const size_t pb = sizeof(tk_prob) * (size_t)n, o_end = (pb + 255) & ~(size_t)255;
+    const size_t o_cnt = (o_end + sizeof(int64_t) * (size_t)n + 255) & ~(size_t)255;
+    void* dm = NULL;
+    double us = -1.0;
+    if (hipMalloc(&dm, o_cnt + 256) == hipSuccess &&
+        hipMemcpy(dm, h, pb, hipMemcpyHostToDevice) == hipSuccess &&
+        hipMemcpy((char*)dm + o_end, he, sizeof(int64_t) * (size_t)n, hipMemcpyHostToDevice) == hipSuccess) {
+        const int64_t nb = 3 * (int64_t)ex->cus;
+        group_ctx ctx = {(tk_prob*)dm, (int64_t*)((char*)dm + o_end), (unsigned long long*)((char*)dm + o_cnt), n,
+                         total, (int)(total < nb ? total : nb), ex->stream};
+        us = tk_tune_time(ex, launch_group_ctx, &ctx, 3);
+    }
+    (void)hipGetLastError();
+    if (dm) { (void)hipStreamSynchronize(ex->stream); (void)hipFree(dm); }
+    free(he);
+    return us;
+}
+
+static bool group_tune_safe(const tk_prob* d)
+{
+    if (d->beta_zero || !d->C) return true;
+    return d->C != (const float*)d->D;
+}
+
+/* Products whose smaller side allows thin tiles are grouped by shape class
+ * (power-of-two buckets of the small side, the large side, K and L, and how Y
+ * is read).  A class missing from the cache is measured on up to 64 of its
+ * products, once with thin and once with MFMA tiles; where MFMA wins, the
+ * class's products are described again with MFMA tiles. */
+static int tune_group(struct tapp_exec* ex, ranked* list, int n, const TAPP_tensor_product* plans,
+                      const void* const* alpha, const void* const* A, const void* const* B,
+                      const void* const* beta, const void* const* C, void* const* D)
+{
+    enum { SAMPLE = 64 };
+    uint64_t* keys = calloc((size_t)n, sizeof(uint64_t));
+    char* done = calloc((size_t)n, 1);
+    tk_prob* thin = malloc(sizeof(tk_prob) * SAMPLE);
+    tk_prob* mfma = malloc(sizeof(tk_prob) * SAMPLE);
+    int* members = malloc(sizeof(int) * (size_t)n);
+    int rc = 0;
+    if (!keys || !done || !thin || !mfma || !members) { rc = TAPP_AMD_ERR_NOMEM; goto out; }
+    for (int i = 0; i < n; i++) {
+        const tk_prob* d = &list[i].p;
+        if (d->cls != TK_GROUP_THIN) { done[i] = 1; continue; }
+        const int64_t f[] = {2, tk_tune_bucket(d->S), tk_tune_bucket(d->G), tk_tune_bucket(d->K),
+                             tk_tune_bucket(d->L), d->y_kfast, d->round};
+        keys[i] = tk_tune_key(f, (int)(sizeof f / sizeof f[0]));
+    }
+    for (int i = 0; i < n && !rc; i++) {
+        if (done[i]) continue;
+        int nm = 0;
+        for (int j = i; j < n; j++)
+            if (!done[j] && keys[j] == keys[i]) { members[nm++] = j; done[j] = 1; }
+        int choice = TK_GROUP_THIN;
+        if (!tk_tune_get(ex->device, keys[i], &choice)) {
+            if (ex->autotune != 1) continue;
+            /* measure on a sample of the class */
+            int ns = 0;
+            for (int m = 0; m < nm && ns < SAMPLE; m++) {
+                const ranked* r = &list[members[m]];
+                if (!group_tune_safe(&r->p)) continue;
+                const int o = r->idx;
+                struct tapp_plan* p = (struct tapp_plan*)plans[o];
+                tk_problem P;
+                const void* c = C ? C[o] : NULL;
+                int prc = tk_plan_problem(p, ex, alpha[o], beta[o], c, &P);
+                if (prc) { rc = prc; break; }
+                if (!tk_group_describe(p, ex, &P, A[o], B[o], c, D[o], TK_GROUP_MFMA, &mfma[ns], &prc)) {
+                    if (prc) { rc = prc; break; }
+                    continue;
+                }
+                thin[ns++] = r->p;
+            }
+            if (rc) break;
+            if (ns == 0) continue;
+            const double t_thin = time_descriptors(ex, thin, ns);
+            const double t_mfma = time_descriptors(ex, mfma, ns);
+            if (t_thin < 0 || t_mfma < 0) { rc = TAPP_AMD_ERR_DEVICE; break; }
+            choice = t_mfma < t_thin ? TK_GROUP_MFMA : TK_GROUP_THIN;
+            const tk_prob* d = &list[i].p;
+            char desc[160];
+            snprintf(desc, sizeof desc, "group S=%lld G=%lld K=%lld L=%lld y=%d thin=%.1fus mfma=%.1fus n=%d",
+                     (long long)d->S, (long long)d->G, (long long)d->K, (long long)d->L, d->y_kfast, t_thin, t_mfma,
+                     ns);
+            tk_tune_put(ex->device, keys[i], choice, desc);
+        }
+        if (choice != TK_GROUP_MFMA) continue;
+        for (int m = 0; m < nm; m++) {
+            ranked* r = &list[members[m]];
+            const int o = r->idx;
+            struct tapp_plan* p = (struct tapp_plan*)plans[o];
+            tk_problem P;
+            const void* c = C ? C[o] : NULL;
+            int prc = tk_plan_problem(p, ex, alpha[o], beta[o], c, &P);
+            tk_prob d;
+            if (!prc && tk_group_describe(p, ex, &P, A[o], B[o], c, D[o], TK_GROUP_MFMA, &d, &prc)) {
+                r->p = d;
+                r->cost = tile_cost(&d);
+            }
+            if (prc) { rc = prc; break; }
+        }
+    }
+out:
+    free(keys); free(done); free(thin); free(mfma); free(members);
+    return rc;
 }
 
 static void group_free(struct tapp_group* g)
@@ -175,11 +316,10 @@ static int create_group(TAPP_AMD_product_group* group, TAPP_executor exec, int c
         if (rc) { free(list); group_free(g); return rc; }
         if (P.empty) continue;
         tk_prob d;
-        if (tk_group_describe(p, ex, &P, A[i], B[i], c, D[i], &d, &rc)) {
-            double per_tile = d.cls == TK_GROUP_THIN ? 256.0 * (double)d.K * (double)(d.S + 1)
-                                                     : 64.0 * 64.0 * (double)d.K;
+        if (tk_group_describe(p, ex, &P, A[i], B[i], c, D[i], -1, &d, &rc)) {
             list[n].p = d;
-            list[n].cost = per_tile + 1.0;
+            list[n].cost = tile_cost(&d);
+            list[n].idx = i;
             n++;
         } else {
             if (rc) { free(list); group_free(g); return rc; }
@@ -187,6 +327,10 @@ static int create_group(TAPP_AMD_product_group* group, TAPP_executor exec, int c
             g->rest_plan[r] = p; g->rest_alpha[r] = alpha[i]; g->rest_A[r] = A[i]; g->rest_B[r] = B[i];
             g->rest_beta[r] = beta[i]; g->rest_C[r] = c; g->rest_D[r] = D[i];
         }
+    }
+    if (ex->autotune && n > 0) {
+        int trc = tune_group(ex, list, n, plans, alpha, A, B, beta, C, D);
+        if (trc) { free(list); group_free(g); return trc; }
     }
     qsort(list, (size_t)n, sizeof(ranked), by_cost);
     g->n = n;

@@ -57,6 +57,7 @@ struct tapp_exec {
     /* attributes */
     int split_bf16;
     int force_kernel;
+    int autotune;              /* 0 rules, 1 measure unknown shape classes, 2 cache only */
     int timing;
     int last_kernel;
     double last_device_us;
@@ -116,6 +117,13 @@ const void* tk_resolve_operand(TAPP_handle handle, const void* p, int64_t lo_byt
 void tk_handle_register(struct tapp_handle_obj* h);
 void tk_handle_unregister(struct tapp_handle_obj* h);
 
+/* cached autotuner (tune.c) */
+uint64_t tk_tune_key(const int64_t* fields, int n);
+int64_t tk_tune_bucket(int64_t x);
+bool tk_tune_get(int device, uint64_t key, int* value);
+void tk_tune_put(int device, uint64_t key, int value, const char* desc);
+double tk_tune_time(struct tapp_exec* ex, int (*launch)(void* ctx), void* ctx, int reps);
+
 /* registry of live executors / handles */
 struct tapp_exec* tk_exec_lookup(TAPP_executor e);
 struct tapp_exec* tk_default_exec(void);
@@ -134,7 +142,9 @@ int tk_problem_tables(struct tapp_plan* plan, const tk_problem* P, int device, c
                       const int64_t** mC, const int64_t** nC, const int64_t** lC);
 void tk_problem_host_k(struct tapp_plan* plan, const int64_t** kA, const int64_t** kB);
 void tk_problem_host_mn(struct tapp_plan* plan, const int64_t** mA, const int64_t** nB);
+/* force_cls: -1 by rule (thin tiles when the smaller side is <= 16), or
+ * TK_GROUP_THIN / TK_GROUP_MFMA (thin only where the rule allows it) */
 bool tk_group_describe(struct tapp_plan* plan, struct tapp_exec* ex, const tk_problem* P, const void* A,
-                       const void* B, const void* C, void* D, tk_prob* out, int* rc);
+                       const void* B, const void* C, void* D, int force_cls, tk_prob* out, int* rc);
 
 #endif /* TAPP_AMD_INTERNAL_H_ */

@@ -150,7 +150,7 @@ static void launch_batch(struct tapp_server* sv, srv_slot* s)
         sv->flop_done += (double)product_flop(&P);
         tk_prob d;
         if (product_flop(&P) <= sv->large_flop &&
-            tk_group_describe(r->plan, ex, &P, r->A, r->B, r->C, r->D, &d, &rc)) {
+            tk_group_describe(r->plan, ex, &P, r->A, r->B, r->C, r->D, -1, &d, &rc)) {
             s->hprobs[n] = d;
             total += d.L * d.tiles_s * d.tiles_g;
             s->hend[n] = total;

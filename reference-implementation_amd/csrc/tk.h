@@ -62,6 +62,7 @@ typedef struct tk_gett {
     int c_contig;
     int kernel;              /* chosen kernel variant, TK_KERNEL_* */
     int cus;                 /* compute units of the device (tile-size choice) */
+    int tile_variant;        /* F64/C32/C64 MFMA tiles: 0 by workgroup count, 1 large, 2 small */
     int ksplit;              /* > 1: K split over gridDim.z; partial sums go to `partial` */
     int64_t kchunk;          /* K length per split (multiple of the kernel's K tile) */
     float* partial;          /* [ksplit][L][N][M] FP32 partial sums (real compute only) */
