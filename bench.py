@@ -20,6 +20,7 @@ import ctypes
 import json
 import os
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -88,7 +89,7 @@ class Product:
 def time_form(lib, torch, prec, split, A, B, D, steps, warmup):
     rc, ex = lib.create_executor()
     assert rc == 0
-    lib.attr_set_int(ex, tc.ATTR_SPLIT_BF16, 1 if split else 0)
+    lib.attr_set_int(ex, tc.ATTR_SPLIT_BF16, int(split))
     lib.attr_set_int(ex, tc.ATTR_TIMING, 1)
     p = Product(lib, prec)
     for _ in range(warmup):
@@ -121,31 +122,50 @@ def fp64_reference(lib, torch, A, B):
 
 
 def cpu_baseline(A, B, D_gpu, sample_j=2):
-    """The compiled reference on the D-slice i = 0, j < sample_j at full K."""
+    """The compiled reference (single-threaded, reentrant) on D-slices at full
+    K: one thread on the slice i = 0, j < sample_j, then one thread per core,
+    each with its own plan on the disjoint slice i = t, j < sample_j."""
     path = os.path.join(ROOT, "oracle", "_ref", "libtapp-reference.so")
     kind = "reference"
     if not os.path.exists(path):
         return None
+    import threading
     ref = tc.TappLib(path)
     a_h = A.cpu().numpy()
     b_h = B.cpu().numpy()
     d_h = np.zeros((O, O, V, V), dtype=np.float32)
-    p = Product(ref, tc.DEFAULT_PREC, ext_ij=(1, sample_j))
-    _, ex = ref.create_executor()
-    t0 = time.perf_counter()
-    rc = p.run(ex, a_h.ctypes.data, b_h.ctypes.data, d_h.ctypes.data)
-    dt = time.perf_counter() - t0
+    cores = max(1, min(16, len(os.sched_getaffinity(0)), O))
+
+    def run_slice(i, out):
+        p = Product(ref, tc.DEFAULT_PREC, ext_ij=(1, sample_j))
+        _, ex = ref.create_executor()
+        t0 = time.perf_counter()
+        rc = p.run(ex, a_h.ctypes.data, b_h.ctypes.data + 4 * V * i, d_h.ctypes.data + 4 * V * V * i)
+        out[i] = (rc, time.perf_counter() - t0)
+        ref.destroy_executor(ex)
+        p.close()
+
+    res = {}
+    run_slice(0, res)
+    rc, dt1 = res[0]
     assert rc == 0
-    ref.destroy_executor(ex)
-    p.close()
+    threads = [threading.Thread(target=run_slice, args=(i, res)) for i in range(cores)]
+    t0 = time.perf_counter()
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    dtn = time.perf_counter() - t0
+    assert all(res[i][0] == 0 for i in range(cores))
     flops = 2.0 * V * V * 1 * sample_j * V * V
-    got = D_gpu.cpu().numpy()[:sample_j, :1]
-    want = d_h[:sample_j, :1]
+    got = D_gpu.cpu().numpy()[:sample_j, :cores]
+    want = d_h[:sample_j, :cores]
     err = float(np.abs(got - want).max() / np.abs(want).max())
-    return {"value": flops / dt / 1e9, "unit": "GFLOP/s", "cores": 1, "kind": kind,
-            "sample": f"D-slice i=0, j<{sample_j} (1/{O * O // sample_j} of D) at full K=16384, "
-                      f"{flops:.3e} flop in {dt:.1f} s",
-            "gpu_vs_reference_slice_normwise_err": err}
+    return {"value": cores * flops / dtn / 1e9, "unit": "GFLOP/s", "cores": cores, "kind": kind,
+            "sample": f"{cores} threads, thread t on the D-slice i=t, j<{sample_j} (1/{O * O // sample_j} of D each) "
+                      f"at full K=16384: {cores * flops:.3e} flop in {dtn:.1f} s",
+            "one_core_value": flops / dt1 / 1e9,
+            "gpu_vs_reference_slices_normwise_err": err}
 
 
 def batching_problems(seed=0, count=8192):
@@ -244,13 +264,40 @@ def batching_bench(lib, torch, steps, prec=tc.DEFAULT_PREC):
         dev_us.append(lib.attr_get_double(ex, tc.ATTR_LAST_DEVICE_US)[1])
     torch.cuda.synchronize()
     group_s = (time.perf_counter() - t0) / steps
-    err = 0.0
-    for p, sz, dl in zip(ptrs, sizes, D_loop):
-        dg = pool[(p[3] - base) // 4:(p[3] - base) // 4 + sz[3]]
-        den = dl.abs().max().item()
-        if den > 0:
-            err = max(err, (dg - dl).abs().max().item() / den)
+    def group_err():
+        e = 0.0
+        for p, sz, dl in zip(ptrs, sizes, D_loop):
+            dg = pool[(p[3] - base) // 4:(p[3] - base) // 4 + sz[3]]
+            den = dl.abs().max().item()
+            if den > 0:
+                e = max(e, (dg - dl).abs().max().item() / den)
+        return e
+
+    err = group_err()
     lib.destroy_group(grp)
+    # the same group with the autotuner: thin vs MFMA tiles measured per shape
+    # class at creation (fresh cache file per bench run)
+    _, tex = lib.create_executor()
+    lib.attr_set_int(tex, tc.ATTR_TIMING, 1)
+    lib.attr_set_int(tex, tc.ATTR_AUTOTUNE, 1)
+    t0 = time.perf_counter()
+    rc, tgrp = lib.create_group(tex, plans, [sc[0].ctypes.data for sc in scal], [p[0] for p in ptrs],
+                                [p[1] for p in ptrs], [sc[1].ctypes.data for sc in scal], [p[2] for p in ptrs],
+                                [p[3] for p in ptrs])
+    tuned_create_s = time.perf_counter() - t0
+    assert rc == 0, lib.explain(rc)
+    assert lib.execute_group(tgrp) == 0
+    torch.cuda.synchronize()
+    tuned_us = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        assert lib.execute_group(tgrp) == 0
+        tuned_us.append(lib.attr_get_double(tex, tc.ATTR_LAST_DEVICE_US)[1])
+    torch.cuda.synchronize()
+    tuned_s = (time.perf_counter() - t0) / steps
+    tuned_err = group_err()
+    lib.destroy_group(tgrp)
+    lib.destroy_executor(tex)
     lib.destroy_executor(ex)
     for plan, hs in zip(plans, infos):
         lib.destroy_plan(plan)
@@ -266,7 +313,10 @@ def batching_bench(lib, torch, steps, prec=tc.DEFAULT_PREC):
             "speedup_vs_loop": loop_s / group_s, "create_ms": create_s * 1e3,
             "persistent_products": info[0], "one_by_one_products": info[1], "tiles": info[2],
             "roofline_frac": ideal / kernel_s, "ideal_ms": ideal * 1e3,
-            "max_rel_err_group_vs_loop": err}
+            "max_rel_err_group_vs_loop": err,
+            "autotuned": {"group_ms": tuned_s * 1e3, "group_kernel_ms": float(np.mean(tuned_us)) * 1e-3,
+                          "group_gflops": flops / tuned_s / 1e9, "speedup_vs_loop": loop_s / tuned_s,
+                          "create_ms_with_tuning": tuned_create_s * 1e3, "max_rel_err_group_vs_loop": tuned_err}}
 
 
 def serving_bench(lib, torch, rates, nreq, split=False, seed=3):
@@ -467,6 +517,8 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (calibrates the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
+    # autotuned measurements start from an empty cache on every run
+    os.environ.setdefault("TAPP_AMD_TUNE_CACHE", os.path.join(tempfile.mkdtemp(prefix="tapp_bench_tune_"), "tune.txt"))
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (products shard over D tiles; "
               "see the sharding design).  Run with --gpus 1.", file=sys.stderr)
@@ -501,8 +553,8 @@ def main():
         den = D64.abs().max().item()
         forms["fp32"] = {"gflops": value, "kernel_us": dev_us,
                          "max_rel_err_vs_fp64": (D_fp32.double() - D64).abs().max().item() / den}
-        for name, prec, split in (("split_bf16x3", tc.DEFAULT_PREC, True), ("bf16", tc.BF16BF16_ACCUM_F32, False),
-                                  ("f16", tc.F16F16_ACCUM_F32, False)):
+        for name, prec, split in (("split_bf16x3", tc.DEFAULT_PREC, 1), ("split_bf16x6", tc.DEFAULT_PREC, 2),
+                                  ("bf16", tc.BF16BF16_ACCUM_F32, 0), ("f16", tc.F16F16_ACCUM_F32, 0)):
             log(f"bench: form {name}")
             w, du, k = time_form(lib, torch, prec, split, A, B, D, max(1, args.steps // 4), 1)
             err = (D.double() - D64).abs().max().item() / den

@@ -30,8 +30,11 @@ enum {
  * executor (pass the TAPP_executor as the TAPP_attr).  Values are passed by
  * pointer to an int unless stated. */
 enum {
-    /* FP32 products with TAPP_DEFAULT_PREC or TAPP_F32F32_ACCUM_F32 run as
-     * split-BF16 (x = hi + lo, three BF16 MFMA products, FP32 accumulate).
+    /* FP32 products with TAPP_DEFAULT_PREC or TAPP_F32F32_ACCUM_F32 run on
+     * BF16 MFMA: 1 = split-BF16 (x = hi + lo, three products, FP32
+     * accumulate: FP32-level accuracy at BF16 speed); 2 = three-piece
+     * split-BF16 (x = h + m + l, six products, per-K-tile FP32 sums added
+     * with compensation: below FP32's own error, at about FP32 speed).
      * Opt-in; default 0. */
     TAPP_AMD_ATTR_SPLIT_BF16 = 0x1001,
     /* Force a kernel family: 0 auto, 1 one-lane-per-output FMA, 2 MFMA,

@@ -34,7 +34,8 @@ __device__ __forceinline__ float round_f16(float x) { return (float)(_Float16)x;
 
 template <typename R> __device__ __forceinline__ R round_mode(R x, int mode)
 {
-    if (mode == TK_MODE_BF16 || mode == TK_MODE_SPLIT_BF16) return (R)round_bf16((float)x);
+    if (mode == TK_MODE_BF16 || mode == TK_MODE_SPLIT_BF16 || mode == TK_MODE_SPLIT6_BF16)
+        return (R)round_bf16((float)x);
     if (mode == TK_MODE_F16) return (R)round_f16((float)x);
     return x;
 }

@@ -51,6 +51,28 @@ __device__ __forceinline__ void mac(int mode, Val<R, CPLX> a, Val<R, CPLX> b, R&
         sr += rr - ii; si += ri + ir;
         return;
     }
+    if (mode == TK_MODE_SPLIT6_BF16) {
+        // x = h + m + l, each a bfloat16 of the remainder; six products
+        auto split = [](R x, R& h, R& m, R& l) {
+            h = (R)round_bf16((float)x);
+            m = (R)round_bf16((float)(x - h));
+            l = (R)round_bf16((float)(x - h - m));
+        };
+        auto prod = [](R ah, R am, R al, R bh, R bm, R bl) {
+            return ah * bh + ((ah * bm + am * bh) + ((ah * bl + al * bh) + am * bm));
+        };
+        R ah, am, al, bh, bm, bl;
+        split(a.re, ah, am, al); split(b.re, bh, bm, bl);
+        const R rr = prod(ah, am, al, bh, bm, bl);
+        if (!CPLX) { sr += rr; return; }
+        R aih, aim, ail, bih, bim, bil;
+        split(a.im, aih, aim, ail); split(b.im, bih, bim, bil);
+        const R ii = prod(aih, aim, ail, bih, bim, bil);
+        const R ri = prod(ah, am, al, bih, bim, bil);
+        const R ir = prod(aih, aim, ail, bh, bm, bl);
+        sr += rr - ii; si += ri + ir;
+        return;
+    }
     if (mode == TK_MODE_BF16 || mode == TK_MODE_F16) {
         a.re = round_mode(a.re, mode); b.re = round_mode(b.re, mode);
         if (CPLX) { a.im = round_mode(a.im, mode); b.im = round_mode(b.im, mode); }
@@ -651,7 +673,7 @@ __global__ __launch_bounds__(256, 2) void k_gett_f32(tk_gett g)
 __device__ __forceinline__ int lp_swz(int row, int q) { return q ^ (2 * ((row >> 4) & 3)); }
 
 template <int MODE>
-__device__ __forceinline__ void lp_pack(const float (&x)[4], uint2& hi, uint2& lo)
+__device__ __forceinline__ void lp_pack(const float (&x)[4], uint2& hi, uint2& lo, uint2& lo2)
 {
     if (MODE == TK_MODE_F16) {
         typedef _Float16 f16x4 __attribute__((ext_vector_type(4)));
@@ -661,14 +683,19 @@ __device__ __forceinline__ void lp_pack(const float (&x)[4], uint2& hi, uint2& l
         hi = __builtin_bit_cast(uint2, h);
     } else {
         typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
-        bf16x4 h, l;
+        bf16x4 h, l, l2;
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
             h[e] = (__bf16)x[e];                                 // v_cvt_pk_bf16_f32, round to nearest even
-            if (MODE == TK_MODE_SPLIT_BF16) l[e] = (__bf16)(x[e] - (float)h[e]);
+            if (MODE == TK_MODE_SPLIT_BF16 || MODE == TK_MODE_SPLIT6_BF16) {
+                const float r = x[e] - (float)h[e];              // exact
+                l[e] = (__bf16)r;
+                if (MODE == TK_MODE_SPLIT6_BF16) l2[e] = (__bf16)(r - (float)l[e]);
+            }
         }
         hi = __builtin_bit_cast(uint2, h);
-        if (MODE == TK_MODE_SPLIT_BF16) lo = __builtin_bit_cast(uint2, l);
+        if (MODE == TK_MODE_SPLIT_BF16 || MODE == TK_MODE_SPLIT6_BF16) lo = __builtin_bit_cast(uint2, l);
+        if (MODE == TK_MODE_SPLIT6_BF16) lo2 = __builtin_bit_cast(uint2, l2);
     }
 }
 
@@ -745,9 +772,10 @@ struct LpTile {
         }
     }
 
-    // converted rows into the K-major 16-bit planes (hi; lo for split)
+    // converted rows into the K-major 16-bit planes (hi; lo for split; the
+    // third piece for the three-piece split)
     template <int MODE>
-    __device__ __forceinline__ void store(uint16_t* hi, uint16_t* lo) const
+    __device__ __forceinline__ void store(uint16_t* hi, uint16_t* lo, uint16_t* lo2) const
     {
 #pragma unroll
         for (int b = 0; b < NB; ++b)
@@ -755,10 +783,11 @@ struct LpTile {
             for (int j = 0; j < 4; ++j) {
                 const int row = 4 * gi() + j;
                 const int off = row * LDK + 4 * lp_swz(row, gk(b));
-                uint2 h, l;
-                lp_pack<MODE>(v[b][j], h, l);
+                uint2 h, l, l2;
+                lp_pack<MODE>(v[b][j], h, l, l2);
                 *reinterpret_cast<uint2*>(hi + off) = h;
-                if (MODE == TK_MODE_SPLIT_BF16) *reinterpret_cast<uint2*>(lo + off) = l;
+                if (MODE == TK_MODE_SPLIT_BF16 || MODE == TK_MODE_SPLIT6_BF16) *reinterpret_cast<uint2*>(lo + off) = l;
+                if (MODE == TK_MODE_SPLIT6_BF16) *reinterpret_cast<uint2*>(lo2 + off) = l2;
             }
     }
 };
@@ -771,15 +800,16 @@ __device__ __forceinline__ uint4 lp_operand(const uint16_t* plane, int row, int 
 }
 
 template <int AL, int BL, int MODE, bool SPLIT>
-__global__ __launch_bounds__(256, 2) void k_gett_lp(tk_gett g)
+__global__ __launch_bounds__(256, MODE == TK_MODE_SPLIT6_BF16 ? 1 : 2) void k_gett_lp(tk_gett g)
 {
     // K tile: 64 deep (twice the bytes in flight per step) where two
-    // workgroups per CU still fit in LDS; split-BF16's two planes need 32
-    constexpr int S = 32, FM = 2, FN = 2, BM = 128, BN = 128, BK = MODE == TK_MODE_SPLIT_BF16 ? 32 : 64;
+    // workgroups per CU still fit in LDS; the split forms' planes need 32
+    constexpr bool S3 = MODE == TK_MODE_SPLIT_BF16, S6 = MODE == TK_MODE_SPLIT6_BF16;
+    constexpr int S = 32, FM = 2, FN = 2, BM = 128, BN = 128, BK = (S3 || S6) ? 32 : 64;
     constexpr int LDK = BK + 8;
-    constexpr int P = MODE == TK_MODE_SPLIT_BF16 ? 2 : 1;          // planes per operand
+    constexpr int P = S6 ? 3 : S3 ? 2 : 1;                         // planes per operand
     constexpr int PLANE = BM * LDK;                                 // 16-bit elements per plane
-    __shared__ __attribute__((aligned(16))) uint16_t lds[2][2 * P][PLANE];   // [buffer][A hi, (A lo), B hi, (B lo)]
+    __shared__ __attribute__((aligned(16))) uint16_t lds[2][2 * P][PLANE];   // [buffer][A planes, B planes]
 
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wm = wave & 1, wn = wave >> 1;
@@ -803,11 +833,14 @@ __global__ __launch_bounds__(256, 2) void k_gett_lp(tk_gett g)
     for (int64_t l = blockIdx.y; l < g.L; l += gridDim.y) {
         const float* Ab = (const float*)g.A + g.lA[l];
         const float* Bb = (const float*)g.B + g.lB[l];
-        f32x16 acc[FN][FM];
+        f32x16 acc[FN][FM], sum[FN][FM], cmp[FN][FM];
 #pragma unroll
         for (int fn = 0; fn < FN; ++fn)
 #pragma unroll
-            for (int fm = 0; fm < FM; ++fm) acc[fn][fm] = f32x16{};
+            for (int fm = 0; fm < FM; ++fm) {
+                acc[fn][fm] = f32x16{};
+                if (S6) { sum[fn][fm] = f32x16{}; cmp[fn][fm] = f32x16{}; }
+            }
 
         auto fetch = [&](int64_t kt) {
             if (interior && kt < nk_full) {
@@ -819,8 +852,8 @@ __global__ __launch_bounds__(256, 2) void k_gett_lp(tk_gett g)
             }
         };
         auto stash = [&](int buf) {
-            la.template store<MODE>(lds[buf][0], lds[buf][P - 1]);
-            lb.template store<MODE>(lds[buf][P], lds[buf][2 * P - 1]);
+            la.template store<MODE>(lds[buf][0], lds[buf][P > 1 ? 1 : 0], lds[buf][P > 2 ? 2 : 0]);
+            lb.template store<MODE>(lds[buf][P], lds[buf][P > 1 ? P + 1 : P], lds[buf][P > 2 ? P + 2 : P]);
         };
         if (nk > 0) {
             fetch(0);
@@ -830,38 +863,60 @@ __global__ __launch_bounds__(256, 2) void k_gett_lp(tk_gett g)
         for (int64_t kt = 0; kt < nk; ++kt) {
             const int cur = kt & 1;
             if (kt + 1 < nk) fetch(kt + 1);
-            const uint16_t* ah = lds[cur][0];
-            const uint16_t* al = lds[cur][P - 1];
-            const uint16_t* bh = lds[cur][P];
-            const uint16_t* bl = lds[cur][2 * P - 1];
 #pragma unroll
             for (int kk = 0; kk < BK; kk += 16) {
-                uint4 a_h[FM], a_l[FM], b_h[FN], b_l[FN];
+                uint4 a[P][FM], b[P][FN];
 #pragma unroll
-                for (int fm = 0; fm < FM; ++fm) {
-                    const int row = wm * FM * S + fm * S + il;
-                    a_h[fm] = lp_operand<LDK>(ah, row, kk + 8 * kl);
-                    if (MODE == TK_MODE_SPLIT_BF16) a_l[fm] = lp_operand<LDK>(al, row, kk + 8 * kl);
-                }
+                for (int q = 0; q < P; ++q) {
 #pragma unroll
-                for (int fn = 0; fn < FN; ++fn) {
-                    const int row = wn * FN * S + fn * S + il;
-                    b_h[fn] = lp_operand<LDK>(bh, row, kk + 8 * kl);
-                    if (MODE == TK_MODE_SPLIT_BF16) b_l[fn] = lp_operand<LDK>(bl, row, kk + 8 * kl);
+                    for (int fm = 0; fm < FM; ++fm)
+                        a[q][fm] = lp_operand<LDK>(lds[cur][q], wm * FM * S + fm * S + il, kk + 8 * kl);
+#pragma unroll
+                    for (int fn = 0; fn < FN; ++fn)
+                        b[q][fn] = lp_operand<LDK>(lds[cur][P + q], wn * FN * S + fn * S + il, kk + 8 * kl);
                 }
 #pragma unroll
                 for (int fn = 0; fn < FN; ++fn)
 #pragma unroll
                     for (int fm = 0; fm < FM; ++fm) {
-                        acc[fn][fm] = mfma16<MODE>(b_h[fn], a_h[fm], acc[fn][fm]);
-                        if (MODE == TK_MODE_SPLIT_BF16) {
-                            acc[fn][fm] = mfma16<MODE>(b_h[fn], a_l[fm], acc[fn][fm]);
-                            acc[fn][fm] = mfma16<MODE>(b_l[fn], a_h[fm], acc[fn][fm]);
+                        // smallest terms first
+                        if (S6) {
+                            acc[fn][fm] = mfma16<MODE>(b[1][fn], a[1][fm], acc[fn][fm]);   // m*m
+                            acc[fn][fm] = mfma16<MODE>(b[2][fn], a[0][fm], acc[fn][fm]);   // h*l
+                            acc[fn][fm] = mfma16<MODE>(b[0][fn], a[2][fm], acc[fn][fm]);   // l*h
                         }
+                        if (S3 || S6) {
+                            acc[fn][fm] = mfma16<MODE>(b[1][fn], a[0][fm], acc[fn][fm]);   // h*m (h*lo)
+                            acc[fn][fm] = mfma16<MODE>(b[0][fn], a[1][fm], acc[fn][fm]);   // m*h (lo*h)
+                        }
+                        acc[fn][fm] = mfma16<MODE>(b[0][fn], a[0][fm], acc[fn][fm]);
                     }
+            }
+            if (S6) {
+                // this K tile's FP32 sum into the compensated total
+#pragma unroll
+                for (int fn = 0; fn < FN; ++fn)
+#pragma unroll
+                    for (int fm = 0; fm < FM; ++fm)
+#pragma unroll
+                        for (int r = 0; r < 16; ++r) {
+                            const float y = acc[fn][fm][r] - cmp[fn][fm][r];
+                            const float t = sum[fn][fm][r] + y;
+                            cmp[fn][fm][r] = (t - sum[fn][fm][r]) - y;
+                            sum[fn][fm][r] = t;
+                            acc[fn][fm][r] = 0.0f;
+                        }
             }
             if (kt + 1 < nk) stash(cur ^ 1);
             __syncthreads();
+        }
+        if (S6) {
+#pragma unroll
+            for (int fn = 0; fn < FN; ++fn)
+#pragma unroll
+                for (int fm = 0; fm < FM; ++fm)
+#pragma unroll
+                    for (int r = 0; r < 16; ++r) acc[fn][fm][r] = sum[fn][fm][r] - cmp[fn][fm][r];
         }
         tile_epilogue<SPLIT>(g, acc, l, m0, n0, tile, ntm * ntn, wave, lane);
     }
@@ -902,7 +957,8 @@ static hipError_t launch_gett_f32_modes(const tk_gett& g, hipStream_t st)
     case TK_MODE_PLAIN: TK_LAUNCH(k_gett_f32, TK_MODE_PLAIN); break;
     case TK_MODE_BF16: TK_LAUNCH(k_gett_lp, TK_MODE_BF16); break;
     case TK_MODE_F16: TK_LAUNCH(k_gett_lp, TK_MODE_F16); break;
-    default: TK_LAUNCH(k_gett_lp, TK_MODE_SPLIT_BF16); break;
+    case TK_MODE_SPLIT_BF16: TK_LAUNCH(k_gett_lp, TK_MODE_SPLIT_BF16); break;
+    default: TK_LAUNCH(k_gett_lp, TK_MODE_SPLIT6_BF16); break;
     }
 #undef TK_LAUNCH
     if (g.ksplit > 1) {

@@ -15,7 +15,12 @@ TAPP_error TAPP_attr_set(TAPP_attr attr, TAPP_key key, void* value)
     struct tapp_exec* e = exec_of(attr);
     if (!e || !value) return TAPP_AMD_ERR_INVALID;
     switch (key) {
-    case TAPP_AMD_ATTR_SPLIT_BF16: e->split_bf16 = *(int*)value != 0; return 0;
+    case TAPP_AMD_ATTR_SPLIT_BF16: {
+        int v = *(int*)value;
+        if (v < 0 || v > 2) return TAPP_AMD_ERR_INVALID;
+        e->split_bf16 = v;
+        return 0;
+    }
     case TAPP_AMD_ATTR_KERNEL: {
         int k = *(int*)value;
         if (k < 0 || k > 3) return TAPP_AMD_ERR_INVALID;

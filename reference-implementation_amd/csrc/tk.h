@@ -29,7 +29,9 @@ enum {
     TK_MODE_PLAIN = 0,     /* FP32 or FP64 arithmetic as stored */
     TK_MODE_F16 = 1,       /* operands rounded to binary16, FP16 MFMA, FP32 accumulate */
     TK_MODE_BF16 = 2,      /* operands rounded to bfloat16, BF16 MFMA, FP32 accumulate */
-    TK_MODE_SPLIT_BF16 = 3 /* x = hi + lo in bfloat16, hi*hi + hi*lo + lo*hi, FP32 accumulate */
+    TK_MODE_SPLIT_BF16 = 3, /* x = hi + lo in bfloat16, hi*hi + hi*lo + lo*hi, FP32 accumulate */
+    TK_MODE_SPLIT6_BF16 = 4 /* x = h + m + l in bfloat16, six products (hh, hm, mh, hl, lh, mm),
+                               FP32 sums per K tile added with compensation (Kahan) */
 };
 
 /* how a kernel may read an operand tile */

@@ -151,17 +151,18 @@ def test_forced_kernels_on_gemm_shapes(lib, oracle, dtype, kernel):
     lib.destroy_executor(ex)
 
 
-@pytest.mark.parametrize("mode", ["bf16", "f16", "split_bf16"])
+@pytest.mark.parametrize("mode", ["bf16", "f16", "split_bf16", "split6_bf16"])
 def test_low_precision_kernel_load_modes(lib, oracle, mode):
     """The 16-bit-input GETT on GEMM shapes whose layouts reach every pair of
     load modes (rows contiguous, k contiguous, gathered), with ragged edges
     and beta != 0: same input rounding as the oracle, so only the FP32
     summation order differs; split-BF16 against the FP64 oracle."""
-    prec = {"bf16": tc.BF16BF16_ACCUM_F32, "f16": tc.F16F16_ACCUM_F32, "split_bf16": tc.DEFAULT_PREC}[mode]
+    prec = {"bf16": tc.BF16BF16_ACCUM_F32, "f16": tc.F16F16_ACCUM_F32, "split_bf16": tc.DEFAULT_PREC,
+            "split6_bf16": tc.DEFAULT_PREC}[mode]
     _, ex = lib.create_executor()
     assert lib.attr_set_int(ex, tc.ATTR_KERNEL, tc.KERNEL_MFMA) == 0
-    if mode == "split_bf16":
-        assert lib.attr_set_int(ex, tc.ATTR_SPLIT_BF16, 1) == 0
+    if mode.startswith("split"):
+        assert lib.attr_set_int(ex, tc.ATTR_SPLIT_BF16, 2 if mode == "split6_bf16" else 1) == 0
     shapes = [(200, 130, 257), (129, 70, 37), (256, 384, 96), (64, 64, 64)]
     for i, (M, N, K) in enumerate(shapes):
         for tA, tB, pad in ((False, False, 0), (True, True, 0), (True, False, 0), (False, True, 1)):
@@ -170,13 +171,42 @@ def test_low_precision_kernel_load_modes(lib, oracle, mode):
             rc, got = T.run_lib(lib, c, ex)
             assert rc == 0
             assert lib.attr_get_int(ex, tc.ATTR_LAST_KERNEL)[1] == tc.KERNEL_MFMA
-            if mode == "split_bf16":
+            if mode.startswith("split"):
                 _, exact = oracle.run(c, f64=True)
-                assert T.normwise_error(got, exact, T.written_mask(c.D)) < 1e-5, (M, N, K, tA, tB, pad)
+                tol = 1e-6 if mode == "split6_bf16" else 1e-5
+                assert T.normwise_error(got, exact, T.written_mask(c.D)) < tol, (M, N, K, tA, tB, pad)
             else:
                 _, want = oracle.run(c)
                 assert_parity(got, want, c, tol=1e-5)
     lib.destroy_executor(ex)
+
+
+def test_split6_bf16_below_fp32_error_at_long_k(lib, oracle):
+    """Three-piece split-BF16 (six products, compensated sums of the FP32
+    K-tile sums) against FP64 at K = 16384, where FP32 accumulation itself
+    errs by about 1e-5: below 1e-6, and below the plain FP32 kernel's error.
+    Also through the one-lane-per-output kernel on a thin shape."""
+    _, ex6 = lib.create_executor()
+    assert lib.attr_set_int(ex6, tc.ATTR_SPLIT_BF16, 2) == 0
+    assert lib.attr_get_int(ex6, tc.ATTR_SPLIT_BF16) == (0, 2)
+    _, ex32 = lib.create_executor()
+    c = _gemm_case(tc.F32, 160, 136, 16384, 910, beta=0.0)
+    _, exact = oracle.run(c, f64=True)
+    mask = T.written_mask(c.D)
+    rc, got6 = T.run_lib(lib, c, ex6)
+    assert rc == 0 and lib.attr_get_int(ex6, tc.ATTR_LAST_KERNEL)[1] == tc.KERNEL_MFMA
+    rc, got32 = T.run_lib(lib, c, ex32)
+    assert rc == 0
+    err6, err32 = T.normwise_error(got6, exact, mask), T.normwise_error(got32, exact, mask)
+    assert err6 < 1e-6, (err6, err32)
+    assert err6 < err32, (err6, err32)
+    thin = _gemm_case(tc.F32, 3, 700, 96, 911, beta=0.25)
+    rc, got = T.run_lib(lib, thin, ex6)
+    assert rc == 0 and lib.attr_get_int(ex6, tc.ATTR_LAST_KERNEL)[1] == tc.KERNEL_SIMPLE
+    _, exact = oracle.run(thin, f64=True)
+    assert T.normwise_error(got, exact, T.written_mask(thin.D)) < 1e-6
+    lib.destroy_executor(ex6)
+    lib.destroy_executor(ex32)
 
 
 @pytest.mark.parametrize("dtype,M", [(tc.F64, 512), (tc.C32, 512), (tc.C64, 256)])
@@ -197,17 +227,17 @@ def test_large_tile_variants(lib, gpu_exec, oracle, dtype, M):
     assert_parity(got, want, c)
 
 
-@pytest.mark.parametrize("mode", ["plain", "bf16", "f16", "split_bf16"])
+@pytest.mark.parametrize("mode", ["plain", "bf16", "f16", "split_bf16", "split6_bf16"])
 def test_split_k_few_tiles(lib, oracle, mode):
     """Few 128 x 128 output tiles and long K: the FP32 MFMA kernel splits K
     over workgroups and a second kernel sums the partials.  Ragged K (not a
     multiple of the 32-deep K tile), a Hadamard mode, beta != 0 and a
     transposed operand are covered."""
     prec = {"plain": tc.DEFAULT_PREC, "bf16": tc.BF16BF16_ACCUM_F32, "f16": tc.F16F16_ACCUM_F32,
-            "split_bf16": tc.DEFAULT_PREC}[mode]
+            "split_bf16": tc.DEFAULT_PREC, "split6_bf16": tc.DEFAULT_PREC}[mode]
     _, ex = lib.create_executor()
-    if mode == "split_bf16":
-        assert lib.attr_set_int(ex, tc.ATTR_SPLIT_BF16, 1) == 0
+    if mode.startswith("split"):
+        assert lib.attr_set_int(ex, tc.ATTR_SPLIT_BF16, 2 if mode == "split6_bf16" else 1) == 0
     for i, (M, N, K, L) in enumerate([(100, 90, 301, 3), (256, 256, 256, 1), (40, 200, 1000, 2)]):
         A = T.make_operand(tc.F32, [M, K, L], "mkl", 40 + i, order=[1, 0, 2] if i == 1 else None)
         B = T.make_operand(tc.F32, [K, N, L], "knl", 50 + i)
@@ -217,9 +247,9 @@ def test_split_k_few_tiles(lib, oracle, mode):
         rc, got, want = run_both(lib, ex, oracle, c)
         assert rc == 0
         assert lib.attr_get_int(ex, tc.ATTR_LAST_KERNEL)[1] == tc.KERNEL_MFMA
-        if mode == "split_bf16":
+        if mode.startswith("split"):
             _, exact = oracle.run(c, f64=True)
-            assert T.normwise_error(got, exact, T.written_mask(c.D)) < 1e-5
+            assert T.normwise_error(got, exact, T.written_mask(c.D)) < (1e-6 if mode == "split6_bf16" else 1e-5)
         else:
             assert_parity(got, want, c)
     lib.destroy_executor(ex)
