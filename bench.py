@@ -114,6 +114,7 @@ def fp64_reference(lib, torch, A, B):
     rc, ex = lib.create_executor()
     p = Product(lib, tc.F64F64_ACCUM_F64, d_type=tc.F64)
     D64 = torch.zeros((O, O, V, V), device="cuda", dtype=torch.float64)
+    torch.cuda.synchronize()   # torch's fill before the library's stream writes D64
     rc = p.run(ex, A.data_ptr(), B.data_ptr(), D64.data_ptr())
     assert rc == 0, lib.explain(rc)
     p.close()
@@ -213,6 +214,7 @@ def batching_bench(lib, torch, steps, prec=tc.DEFAULT_PREC):
     g = torch.Generator(device="cuda")
     g.manual_seed(7)
     pool = torch.rand(total, generator=g, device="cuda", dtype=torch.float32).mul_(2.0).sub_(1.0)
+    torch.cuda.synchronize()   # operands written on torch's stream before the library reads them
     base = pool.data_ptr()
     off = 0
     plans, infos, ptrs, scal = [], [], [], []
@@ -247,6 +249,7 @@ def batching_bench(lib, torch, steps, prec=tc.DEFAULT_PREC):
     torch.cuda.synchronize()
     loop_s = time.perf_counter() - t0
     D_loop = [pool[(p[3] - base) // 4:(p[3] - base) // 4 + sz[3]].clone() for p, sz in zip(ptrs, sizes)]
+    torch.cuda.synchronize()   # the copies run on torch's stream; the library's launches use their own
     # one persistent group
     t0 = time.perf_counter()
     rc, grp = lib.create_group(ex, plans, [sc[0].ctypes.data for sc in scal], [p[0] for p in ptrs],
@@ -274,6 +277,8 @@ def batching_bench(lib, torch, steps, prec=tc.DEFAULT_PREC):
         return e
 
     err = group_err()
+    D_group = [pool[(p[3] - base) // 4:(p[3] - base) // 4 + sz[3]].clone() for p, sz in zip(ptrs, sizes)]
+    torch.cuda.synchronize()
     lib.destroy_group(grp)
     # the same group with the autotuner: thin vs MFMA tiles measured per shape
     # class at creation (fresh cache file per bench run)
@@ -296,6 +301,10 @@ def batching_bench(lib, torch, steps, prec=tc.DEFAULT_PREC):
     torch.cuda.synchronize()
     tuned_s = (time.perf_counter() - t0) / steps
     tuned_err = group_err()
+    # the same products through one persistent launch give the same bits
+    # whatever the run (tile order on the GPU varies, each output's sum does not)
+    repeatable = all(torch.equal(pool[(p[3] - base) // 4:(p[3] - base) // 4 + sz[3]], dg)
+                     for p, sz, dg in zip(ptrs, sizes, D_group))
     lib.destroy_group(tgrp)
     lib.destroy_executor(tex)
     lib.destroy_executor(ex)
@@ -313,7 +322,7 @@ def batching_bench(lib, torch, steps, prec=tc.DEFAULT_PREC):
             "speedup_vs_loop": loop_s / group_s, "create_ms": create_s * 1e3,
             "persistent_products": info[0], "one_by_one_products": info[1], "tiles": info[2],
             "roofline_frac": ideal / kernel_s, "ideal_ms": ideal * 1e3,
-            "max_rel_err_group_vs_loop": err,
+            "max_rel_err_group_vs_loop": err, "tuned_group_bitwise_equal_to_default": repeatable,
             "autotuned": {"group_ms": tuned_s * 1e3, "group_kernel_ms": float(np.mean(tuned_us)) * 1e-3,
                           "group_gflops": flops / tuned_s / 1e9, "speedup_vs_loop": loop_s / tuned_s,
                           "create_ms_with_tuning": tuned_create_s * 1e3, "max_rel_err_group_vs_loop": tuned_err}}
@@ -446,6 +455,7 @@ def config_cases(lib, torch, steps):
             bufs.append(t)
             ptrs.append(t.data_ptr() + off * t.element_size())
         ops = [0, 1 if name == "configs1" else 0, 0, 0]
+        torch.cuda.synchronize()   # operands written on torch's stream before the library reads them
         _, plan = lib.create_plan(ops, infos, [L(x[2]) for x in specs], prec)
         npd = np.complex128 if ctype == tc.C64 else np.float32
         al = np.array([alpha], dtype=npd)
@@ -543,6 +553,7 @@ def main():
     log(f"bench: FP32 plain, warmup {args.warmup}, steps {args.steps}")
     wall, dev_us, kern = time_form(lib, torch, tc.DEFAULT_PREC, False, A, B, D, args.steps, args.warmup)
     D_fp32 = D.clone()
+    torch.cuda.synchronize()   # before the next products overwrite D from the library's stream
     achieved = flops / (dev_us * 1e-6) / 1e12
     value = flops / wall / 1e9
 

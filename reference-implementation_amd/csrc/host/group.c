@@ -152,9 +152,11 @@ static int launch_group_ctx(void* c)
     return tk_launch_group(x->probs, x->tile_end, NULL, x->n, x->total, x->counter, x->nblocks, x->st);
 }
 
-/* Device time of one persistent launch over `n` descriptors. */
+/* Device time of one persistent launch over `n` descriptors (in the order
+ * given; the caller sorts them as the group will run them). */
 static double time_descriptors(struct tapp_exec* ex, const tk_prob* h, int n)
 {
+    const int64_t nb = 3 * (int64_t)ex->cus;
     int64_t* he = malloc(sizeof(int64_t) * (size_t)n);
     if (!he) return -1.0;
     int64_t total = 0;
@@ -166,7 +168,6 @@ static double time_descriptors(struct tapp_exec* ex, const tk_prob* h, int n)
     if (hipMalloc(&dm, o_cnt + 256) == hipSuccess &&
         hipMemcpy(dm, h, pb, hipMemcpyHostToDevice) == hipSuccess &&
         hipMemcpy((char*)dm + o_end, he, sizeof(int64_t) * (size_t)n, hipMemcpyHostToDevice) == hipSuccess) {
-        const int64_t nb = 3 * (int64_t)ex->cus;
         group_ctx ctx = {(tk_prob*)dm, (int64_t*)((char*)dm + o_end), (unsigned long long*)((char*)dm + o_cnt), n,
                          total, (int)(total < nb ? total : nb), ex->stream};
         us = tk_tune_time(ex, launch_group_ctx, &ctx, 3);
@@ -177,92 +178,120 @@ static double time_descriptors(struct tapp_exec* ex, const tk_prob* h, int n)
     return us;
 }
 
+/* Measuring runs every product again: only without C that is D. */
 static bool group_tune_safe(const tk_prob* d)
 {
     if (d->beta_zero || !d->C) return true;
     return d->C != (const float*)d->D;
 }
 
-/* Products whose smaller side allows thin tiles are grouped by shape class
- * (power-of-two buckets of the small side, the large side, K and L, and how Y
- * is read).  A class missing from the cache is measured on up to 64 of its
- * products, once with thin and once with MFMA tiles; where MFMA wins, the
- * class's products are described again with MFMA tiles. */
+/* The group's launch with the products whose small-side bucket is in
+ * `mask` on MFMA tiles (their MFMA descriptors in alt[]), sorted by cost as
+ * create_group will order them. */
+static double time_mask(struct tapp_exec* ex, const ranked* list, const tk_prob* alt, const int* bucket, int n,
+                        unsigned mask, ranked* work, tk_prob* flat)
+{
+    for (int i = 0; i < n; i++) {
+        work[i] = list[i];
+        if (bucket[i] >= 0 && (mask >> bucket[i]) & 1u) {
+            work[i].p = alt[i];
+            work[i].cost = tile_cost(&alt[i]);
+        }
+    }
+    qsort(work, (size_t)n, sizeof(ranked), by_cost);
+    for (int i = 0; i < n; i++) flat[i] = work[i].p;
+    return time_descriptors(ex, flat, n);
+}
+
+/* Thin FMA tiles or MFMA tiles for products whose smaller side allows both,
+ * chosen per shape class -- the power-of-two bucket of the smaller side --
+ * in the context of the whole group: starting from the rule (thin tiles),
+ * each bucket in turn is moved to MFMA tiles and kept there if the whole
+ * launch gets at least 3 % faster.  The result is cached under the group's
+ * mix of shape classes (each product's small side, large side, K and L
+ * buckets), so a group with the same mix is not measured again. */
 static int tune_group(struct tapp_exec* ex, ranked* list, int n, const TAPP_tensor_product* plans,
                       const void* const* alpha, const void* const* A, const void* const* B,
                       const void* const* beta, const void* const* C, void* const* D)
 {
-    enum { SAMPLE = 64 };
-    uint64_t* keys = calloc((size_t)n, sizeof(uint64_t));
-    char* done = calloc((size_t)n, 1);
-    tk_prob* thin = malloc(sizeof(tk_prob) * SAMPLE);
-    tk_prob* mfma = malloc(sizeof(tk_prob) * SAMPLE);
-    int* members = malloc(sizeof(int) * (size_t)n);
-    int rc = 0;
-    if (!keys || !done || !thin || !mfma || !members) { rc = TAPP_AMD_ERR_NOMEM; goto out; }
+    int* bucket = malloc(sizeof(int) * (size_t)n);
+    uint64_t* cls = malloc(sizeof(uint64_t) * (size_t)n);
+    if (!bucket || !cls) { free(bucket); free(cls); return TAPP_AMD_ERR_NOMEM; }
+    int nthin = 0;
+    bool safe = true;
+    unsigned present = 0;
     for (int i = 0; i < n; i++) {
         const tk_prob* d = &list[i].p;
-        if (d->cls != TK_GROUP_THIN) { done[i] = 1; continue; }
-        const int64_t f[] = {2, tk_tune_bucket(d->S), tk_tune_bucket(d->G), tk_tune_bucket(d->K),
-                             tk_tune_bucket(d->L), d->y_kfast, d->round};
-        keys[i] = tk_tune_key(f, (int)(sizeof f / sizeof f[0]));
+        bucket[i] = -1;
+        if (!group_tune_safe(d)) safe = false;
+        if (d->cls != TK_GROUP_THIN) continue;
+        bucket[i] = (int)tk_tune_bucket(d->S);
+        present |= 1u << bucket[i];
+        const int64_t f[] = {tk_tune_bucket(d->S), tk_tune_bucket(d->G), tk_tune_bucket(d->K), tk_tune_bucket(d->L),
+                             d->round};
+        cls[nthin++] = tk_tune_key(f, (int)(sizeof f / sizeof f[0]));
     }
-    for (int i = 0; i < n && !rc; i++) {
-        if (done[i]) continue;
-        int nm = 0;
-        for (int j = i; j < n; j++)
-            if (!done[j] && keys[j] == keys[i]) { members[nm++] = j; done[j] = 1; }
-        int choice = TK_GROUP_THIN;
-        if (!tk_tune_get(ex->device, keys[i], &choice)) {
-            if (ex->autotune != 1) continue;
-            /* measure on a sample of the class */
-            int ns = 0;
-            for (int m = 0; m < nm && ns < SAMPLE; m++) {
-                const ranked* r = &list[members[m]];
-                if (!group_tune_safe(&r->p)) continue;
-                const int o = r->idx;
-                struct tapp_plan* p = (struct tapp_plan*)plans[o];
-                tk_problem P;
-                const void* c = C ? C[o] : NULL;
-                int prc = tk_plan_problem(p, ex, alpha[o], beta[o], c, &P);
-                if (prc) { rc = prc; break; }
-                if (!tk_group_describe(p, ex, &P, A[o], B[o], c, D[o], TK_GROUP_MFMA, &mfma[ns], &prc)) {
-                    if (prc) { rc = prc; break; }
-                    continue;
-                }
-                thin[ns++] = r->p;
-            }
-            if (rc) break;
-            if (ns == 0) continue;
-            const double t_thin = time_descriptors(ex, thin, ns);
-            const double t_mfma = time_descriptors(ex, mfma, ns);
-            if (t_thin < 0 || t_mfma < 0) { rc = TAPP_AMD_ERR_DEVICE; break; }
-            choice = t_mfma < t_thin ? TK_GROUP_MFMA : TK_GROUP_THIN;
-            const tk_prob* d = &list[i].p;
-            char desc[160];
-            snprintf(desc, sizeof desc, "group S=%lld G=%lld K=%lld L=%lld y=%d thin=%.1fus mfma=%.1fus n=%d",
-                     (long long)d->S, (long long)d->G, (long long)d->K, (long long)d->L, d->y_kfast, t_thin, t_mfma,
-                     ns);
-            tk_tune_put(ex->device, keys[i], choice, desc);
-        }
-        if (choice != TK_GROUP_MFMA) continue;
-        for (int m = 0; m < nm; m++) {
-            ranked* r = &list[members[m]];
-            const int o = r->idx;
+    int rc = 0;
+    if (nthin == 0) goto out;
+    /* key: the sorted multiset of thin-capable shape classes */
+    for (int i = 1; i < nthin; i++)
+        for (int j = i; j > 0 && cls[j] < cls[j - 1]; j--) { uint64_t t = cls[j]; cls[j] = cls[j - 1]; cls[j - 1] = t; }
+    uint64_t mix = 1469598103934665603ull;
+    for (int i = 0; i < nthin; i++) { mix ^= cls[i]; mix *= 1099511628211ull; }
+    const int64_t kf[] = {3, (int64_t)mix, nthin};
+    const uint64_t key = tk_tune_key(kf, 3);
+    int mask = 0;
+    if (!tk_tune_get(ex->device, key, &mask)) {
+        if (ex->autotune != 1 || !safe) goto out;
+        tk_prob* alt = malloc(sizeof(tk_prob) * (size_t)n);
+        ranked* work = malloc(sizeof(ranked) * (size_t)n);
+        tk_prob* flat = malloc(sizeof(tk_prob) * (size_t)n);
+        if (!alt || !work || !flat) { free(alt); free(work); free(flat); rc = TAPP_AMD_ERR_NOMEM; goto out; }
+        for (int i = 0; i < n && !rc; i++) {
+            if (bucket[i] < 0) continue;
+            const int o = list[i].idx;
             struct tapp_plan* p = (struct tapp_plan*)plans[o];
-            tk_problem P;
             const void* c = C ? C[o] : NULL;
+            tk_problem P;
             int prc = tk_plan_problem(p, ex, alpha[o], beta[o], c, &P);
-            tk_prob d;
-            if (!prc && tk_group_describe(p, ex, &P, A[o], B[o], c, D[o], TK_GROUP_MFMA, &d, &prc)) {
-                r->p = d;
-                r->cost = tile_cost(&d);
-            }
-            if (prc) { rc = prc; break; }
+            if (!prc && !tk_group_describe(p, ex, &P, A[o], B[o], c, D[o], TK_GROUP_MFMA, &alt[i], &prc) && !prc)
+                bucket[i] = -1;
+            rc = prc;
         }
+        double best = rc ? -1.0 : time_mask(ex, list, alt, bucket, n, 0u, work, flat);
+        if (!rc && best < 0) rc = TAPP_AMD_ERR_DEVICE;
+        double t0 = best;
+        for (int b = 31; b >= 0 && !rc; b--) {
+            if (!((present >> b) & 1u)) continue;
+            const double t = time_mask(ex, list, alt, bucket, n, (unsigned)mask | (1u << b), work, flat);
+            if (t < 0) { rc = TAPP_AMD_ERR_DEVICE; break; }
+            if (t < 0.97 * best) { best = t; mask |= 1 << b; }
+        }
+        free(alt); free(work); free(flat);
+        if (rc) goto out;
+        char desc[160];
+        snprintf(desc, sizeof desc, "group mix=%016llx thin-capable=%d rule=%.1fus tuned=%.1fus mfma-buckets=0x%x",
+                 (unsigned long long)mix, nthin, t0, best, (unsigned)mask);
+        tk_tune_put(ex->device, key, mask, desc);
+    }
+    /* products of the chosen buckets on MFMA tiles */
+    for (int i = 0; i < n && !rc && mask; i++) {
+        if (bucket[i] < 0 || !(((unsigned)mask >> bucket[i]) & 1u)) continue;
+        const int o = list[i].idx;
+        struct tapp_plan* p = (struct tapp_plan*)plans[o];
+        const void* c = C ? C[o] : NULL;
+        tk_problem P;
+        tk_prob d;
+        int prc = tk_plan_problem(p, ex, alpha[o], beta[o], c, &P);
+        if (!prc && tk_group_describe(p, ex, &P, A[o], B[o], c, D[o], TK_GROUP_MFMA, &d, &prc)) {
+            list[i].p = d;
+            list[i].cost = tile_cost(&d);
+        }
+        rc = prc;
     }
 out:
-    free(keys); free(done); free(thin); free(mfma); free(members);
+    free(bucket);
+    free(cls);
     return rc;
 }
 

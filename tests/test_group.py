@@ -37,6 +37,7 @@ class DevCase:
     def reset_d(self):
         import torch
         self.t["D"].copy_(torch.from_numpy(self.case.D.buf.view(np.uint8).copy()).cuda())
+        torch.cuda.synchronize()   # torch's stream; the library launches on its own
 
     def close(self):
         self.lib.destroy_plan(self.plan)
@@ -99,6 +100,7 @@ def test_group_graph_replays(lib, gpu_exec, oracle):
             new = T.fill(d.case.A.type, len(d.case.A.buf), 1000 * replay + len(d.case.A.buf))
             d.case.A.buf = new
             d.t["A"].copy_(torch.from_numpy(new.view(np.uint8).copy()).cuda())
+            torch.cuda.synchronize()
         x = torch.ones(1 << 20, device="cuda")
         x.mul_(2.0).sum().item()
         torch.cuda.synchronize()

@@ -150,9 +150,11 @@ def test_resident_operands_in_group_and_server(lib, gpu_exec, oracle):
     smallest = min(_nbytes(hp.case.A) for hp in hps)
     rc, srv = lib.create_server(gpu_exec, smallest // 2, slots=2)
     assert rc == 0
+    for dt in dts:
+        dt.zero_()
+    torch.cuda.synchronize()   # the fills run on torch's stream, the server on its own
     sts = []
     for hp, dt, d in zip(hps, dts, dptr):
-        dt.zero_()
         a = hp.args()
         rc, st = lib.submit(srv, hp.plan, a[0], a[1], a[2], a[3], a[4], d)
         assert rc == 0, lib.explain(rc)
