@@ -7,16 +7,17 @@
 //   number the reference's sum_A / sum_B holds when the compute type is the
 //   reference's sum type.  One lane per output; consecutive lanes take
 //   consecutive i0, which is the unit-stride direction the host picks.
+// k_prep2: A's and B's pre-passes in one launch.
 // k_scatter: compute-type dense buffer -> D of another storage type.
 #include "tk_device.hpp"
 
 namespace tk {
 
 template <typename R, bool CPLX>
-__global__ __launch_bounds__(256) void k_prep(tk_prep p)
+__device__ __forceinline__ void prep_body(const tk_prep& p, int64_t block, int64_t nblocks)
 {
     const int64_t total = p.n0 * p.n1 * p.n2;
-    for (int64_t o = (int64_t)blockIdx.x * 256 + threadIdx.x; o < total; o += (int64_t)gridDim.x * 256) {
+    for (int64_t o = block * 256 + threadIdx.x; o < total; o += nblocks * 256) {
         const int64_t i0 = o % p.n0, t = o / p.n0, i1 = t % p.n1, i2 = t / p.n1;
         const int64_t base = p.t0[i0] + p.t1[i1] + p.t2[i2];
         R sr = 0, si = 0;
@@ -41,6 +42,21 @@ __global__ __launch_bounds__(256) void k_prep(tk_prep p)
         }
         Elem<R, CPLX>::st(p.dst, o, Val<R, CPLX>{sr, si});
     }
+}
+
+template <typename R, bool CPLX>
+__global__ __launch_bounds__(256) void k_prep(tk_prep p)
+{
+    prep_body<R, CPLX>(p, blockIdx.x, gridDim.x);
+}
+
+// Two independent pre-passes (A's and B's) in one launch: workgroups
+// [0, ga) take the first, the rest the second.
+template <typename R, bool CPLX>
+__global__ __launch_bounds__(256) void k_prep2(tk_prep a, tk_prep b, int ga)
+{
+    if ((int)blockIdx.x < ga) prep_body<R, CPLX>(a, blockIdx.x, ga);
+    else prep_body<R, CPLX>(b, blockIdx.x - ga, gridDim.x - ga);
 }
 
 template <typename R, bool CPLX>
@@ -78,6 +94,28 @@ extern "C" int tk_launch_prep(const tk_prep* pp, void* stream)
     case TK_F64: hipLaunchKernelGGL((k_prep<double, false>), grid, block, 0, st, p); break;
     case TK_C32: hipLaunchKernelGGL((k_prep<float, true>), grid, block, 0, st, p); break;
     case TK_C64: hipLaunchKernelGGL((k_prep<double, true>), grid, block, 0, st, p); break;
+    default: return (int)hipErrorInvalidValue;
+    }
+    return (int)hipGetLastError();
+}
+
+extern "C" int tk_launch_prep2(const tk_prep* pa, const tk_prep* pb, void* stream)
+{
+    const tk_prep &a = *pa, &b = *pb;
+    hipStream_t st = (hipStream_t)stream;
+    const int64_t ta = a.n0 * a.n1 * a.n2, tb = b.n0 * b.n1 * b.n2;
+    if (ta <= 0) return tk_launch_prep(pb, stream);
+    if (tb <= 0 || a.ctype != b.ctype) {
+        int rc = tk_launch_prep(pa, stream);
+        return rc ? rc : tk_launch_prep(pb, stream);
+    }
+    const unsigned ga = grid_for(ta), gb = grid_for(tb);
+    const dim3 grid(ga + gb), block(256);
+    switch (a.ctype) {
+    case TK_F32: hipLaunchKernelGGL((k_prep2<float, false>), grid, block, 0, st, a, b, (int)ga); break;
+    case TK_F64: hipLaunchKernelGGL((k_prep2<double, false>), grid, block, 0, st, a, b, (int)ga); break;
+    case TK_C32: hipLaunchKernelGGL((k_prep2<float, true>), grid, block, 0, st, a, b, (int)ga); break;
+    case TK_C64: hipLaunchKernelGGL((k_prep2<double, true>), grid, block, 0, st, a, b, (int)ga); break;
     default: return (int)hipErrorInvalidValue;
     }
     return (int)hipGetLastError();

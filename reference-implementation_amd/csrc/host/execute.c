@@ -518,14 +518,15 @@ int tk_execute_on(struct tapp_plan* p, struct tapp_exec* ex, const void* alpha, 
     const int64_t *hmA = HT(T_mA), *hkA = HT(T_kA), *hlA = HT(T_lA);
     const int64_t *hnB = HT(T_nB), *hkB = HT(T_kB), *hlB = HT(T_lB);
 
+    /* A's and B's pre-passes (one launch when both are needed) */
+    tk_prep prA, prB;
     if (needA && K > 0) {
         const int64_t* tabs[3] = {DT(T_mA), DT(T_kA), DT(T_lA)};
         const int64_t sizes[3] = {M, K, L};
         const int* o = tc->ordA;
-        tk_prep pr = {P.type[0], ct, P.op_A, (P.nUA > 1 && (P.mode == TK_MODE_F16 || P.mode == TK_MODE_BF16)) ? P.mode : 0,
-                      d[0].base, ws + off_tmp[0], sizes[o[0]], sizes[o[1]], sizes[o[2]], P.nUA,
-                      tabs[o[0]], tabs[o[1]], tabs[o[2]], DT(T_uA)};
-        LAUNCHCHK(tk_launch_prep(&pr, st));
+        prA = (tk_prep){P.type[0], ct, P.op_A, (P.nUA > 1 && (P.mode == TK_MODE_F16 || P.mode == TK_MODE_BF16)) ? P.mode : 0,
+                        d[0].base, ws + off_tmp[0], sizes[o[0]], sizes[o[1]], sizes[o[2]], P.nUA,
+                        tabs[o[0]], tabs[o[1]], tabs[o[2]], DT(T_uA)};
         g.A = ws + off_tmp[0]; g.mA = DT(T_dmA); g.kA = DT(T_dkA); g.lA = DT(T_dlA); g.conjA = 0;
         hmA = HT(T_dmA); hkA = HT(T_dkA); hlA = HT(T_dlA);
     }
@@ -533,13 +534,15 @@ int tk_execute_on(struct tapp_plan* p, struct tapp_exec* ex, const void* alpha, 
         const int64_t* tabs[3] = {DT(T_nB), DT(T_kB), DT(T_lB)};
         const int64_t sizes[3] = {N, K, L};
         const int* o = tc->ordB;
-        tk_prep pr = {P.type[1], ct, P.op_B, (P.nUB > 1 && (P.mode == TK_MODE_F16 || P.mode == TK_MODE_BF16)) ? P.mode : 0,
-                      d[1].base, ws + off_tmp[1], sizes[o[0]], sizes[o[1]], sizes[o[2]], P.nUB,
-                      tabs[o[0]], tabs[o[1]], tabs[o[2]], DT(T_uB)};
-        LAUNCHCHK(tk_launch_prep(&pr, st));
+        prB = (tk_prep){P.type[1], ct, P.op_B, (P.nUB > 1 && (P.mode == TK_MODE_F16 || P.mode == TK_MODE_BF16)) ? P.mode : 0,
+                        d[1].base, ws + off_tmp[1], sizes[o[0]], sizes[o[1]], sizes[o[2]], P.nUB,
+                        tabs[o[0]], tabs[o[1]], tabs[o[2]], DT(T_uB)};
         g.B = ws + off_tmp[1]; g.nB = DT(T_dnB); g.kB = DT(T_dkB); g.lB = DT(T_dlB); g.conjB = 0;
         hnB = HT(T_dnB); hkB = HT(T_dkB); hlB = HT(T_dlB);
     }
+    if (needA && needB && K > 0) LAUNCHCHK(tk_launch_prep2(&prA, &prB, st));
+    else if (needA && K > 0) LAUNCHCHK(tk_launch_prep(&prA, st));
+    else if (needB && K > 0) LAUNCHCHK(tk_launch_prep(&prB, st));
     if (needC) {
         tk_prep pr = {P.type[2], ct, 0, 0, d[2].base, ws + off_tmp[2], M, N, L, 1,
                       DT(T_mC), DT(T_nC), DT(T_lC), DT(T_zero)};

@@ -136,6 +136,8 @@ typedef struct tk_prob {
  * or a hipError_t value. */
 int tk_launch_gett(const tk_gett* g, void* stream);
 int tk_launch_prep(const tk_prep* p, void* stream);
+/* A's and B's pre-passes (same compute type) as one launch */
+int tk_launch_prep2(const tk_prep* a, const tk_prep* b, void* stream);
 int tk_launch_scatter(const tk_scatter* s, void* stream);
 /* Zeroes *counter, then runs the persistent group kernel: `nblocks`
  * workgroups take tiles from the atomic counter until `total` is reached.
