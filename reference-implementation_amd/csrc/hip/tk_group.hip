@@ -3,9 +3,10 @@
 // One grid of `nblocks` workgroups; each takes the next tile index from an
 // atomic counter, finds its product through the tile -> product map (or a
 // binary search over the inclusive tile prefix when no map is given), runs
-// the tile and goes back for another until the counter passes
-// the total.  Every workgroup reaches that exit.  Products are sorted by cost
-// on the host, so long tiles start first.
+// the tile and goes back for another until its queue is drained, then
+// drains the other queues.  Every workgroup reaches that exit.  Products are
+// sorted by cost on the host, so long tiles start first; product groups keep
+// one queue per XCD (a product's tiles on one XCD's L2).
 //
 //  thin tile  (S <= 16): 256 columns g, all s.  The S x 32 chunk of X and the
 //             32 x 256 chunk of Y go through LDS (Y loaded along whichever of
@@ -242,63 +243,87 @@ __device__ __forceinline__ void mfma_tile(const GView& p, int64_t l, int64_t m0,
     }
 }
 
+// Tiles come from `nq` queues: queue q holds tiles [qbeg[q], qbeg[q + 1])
+// (one queue over [0, total) when qbeg is null) and has its own counter,
+// counter[32 q].  A workgroup starts with queue blockIdx.x % nq -- with one
+// queue per XCD that is its own XCD's, since workgroups are dispatched to the
+// XCDs round-robin, so a product's tiles share that XCD's L2 -- and then
+// takes what is left in the others, in order, so every workgroup ends after
+// every queue is empty.
 __global__ __launch_bounds__(256, 3) void k_group(const tk_prob* __restrict__ probs, const int64_t* __restrict__ tile_end,
                                                const int* __restrict__ tile_prob, int nprobs, int64_t total,
+                                               const int64_t* __restrict__ qbeg, int nq,
                                                unsigned long long* counter)
 {
     __shared__ __attribute__((aligned(16))) float lds[KC * YP + SMAX * KC];
     __shared__ int64_t goff[GT];
     __shared__ int64_t kox[KO], koy[KO];
     __shared__ int64_t next;
-    for (;;) {
-        if (threadIdx.x == 0) next = (int64_t)atomicAdd(counter, 1ull);
-        __syncthreads();
-        const int64_t t = next;
-        __syncthreads();
-        if (t >= total) break;
-        int lo;
-        if (tile_prob) {
-            lo = tile_prob[t];                       // direct map, one load
-        } else {
-            int a = 0, b = nprobs - 1;               // first problem with tile_end > t
-            while (a < b) {
-                const int mid = (a + b) >> 1;
-                if (tile_end[mid] > t) b = mid; else a = mid + 1;
+    const int home = (int)(blockIdx.x % (unsigned)nq);
+    for (int qi = 0; qi < nq; ++qi) {
+        const int q = (home + qi) % nq;
+        const int64_t qb = qbeg ? qbeg[q] : 0, qe = qbeg ? qbeg[q + 1] : total;
+        for (;;) {
+            if (threadIdx.x == 0) next = qb + (int64_t)atomicAdd(counter + 32 * q, 1ull);
+            __syncthreads();
+            const int64_t t = next;
+            __syncthreads();
+            if (t >= qe) break;
+            int lo;
+            if (tile_prob) {
+                lo = tile_prob[t];                       // direct map, one load
+            } else {
+                int a = 0, b = nprobs - 1;               // first problem with tile_end > t
+                while (a < b) {
+                    const int mid = (a + b) >> 1;
+                    if (tile_end[mid] > t) b = mid; else a = mid + 1;
+                }
+                lo = a;
             }
-            lo = a;
-        }
-        const tk_prob& pd = probs[lo];
-        const GView p(pd);
-        const int64_t local = t - (lo ? tile_end[lo - 1] : 0);
-        const int64_t tiles_s = pd.tiles_s, tiles_g = pd.tiles_g, L = pd.L;
-        const int cls = pd.cls;
-        (void)L;
-        const int64_t per_l = tiles_s * tiles_g;
-        const int64_t l = local / per_l, rem = local % per_l;
-        const int64_t ts = rem / tiles_g, tg = rem % tiles_g;
-        if (cls == TK_GROUP_THIN) {
-            if (p.S <= 4) thin_tile<4>(p, l, tg * GT, lds, goff);
-            else if (p.S <= 8) thin_tile<8>(p, l, tg * GT, lds, goff);
-            else thin_tile<16>(p, l, tg * GT, lds, goff);
-        } else {
-            mfma_tile(p, l, ts * MT, tg * MT, lds, kox, koy);
+            const tk_prob& pd = probs[lo];
+            const GView p(pd);
+            const int64_t local = t - (lo ? tile_end[lo - 1] : 0);
+            const int64_t tiles_s = pd.tiles_s, tiles_g = pd.tiles_g;
+            const int cls = pd.cls;
+            const int64_t per_l = tiles_s * tiles_g;
+            const int64_t l = local / per_l, rem = local % per_l;
+            const int64_t ts = rem / tiles_g, tg = rem % tiles_g;
+            if (cls == TK_GROUP_THIN) {
+                if (p.S <= 4) thin_tile<4>(p, l, tg * GT, lds, goff);
+                else if (p.S <= 8) thin_tile<8>(p, l, tg * GT, lds, goff);
+                else thin_tile<16>(p, l, tg * GT, lds, goff);
+            } else {
+                mfma_tile(p, l, ts * MT, tg * MT, lds, kox, koy);
+            }
         }
     }
 }
 
-__global__ void k_zero_counter(unsigned long long* counter) { *counter = 0ull; }
+// Fill kernel (graph-capturable, unlike a memset node): counters 0 .. nq-1.
+__global__ void k_zero_counters(unsigned long long* counter, int nq)
+{
+    if ((int)threadIdx.x < nq) counter[32 * threadIdx.x] = 0ull;
+}
 
 }  // namespace tk
 
 using namespace tk;
 
-extern "C" int tk_launch_group(const tk_prob* probs, const int64_t* tile_end, const int* tile_prob, int nprobs,
-                               int64_t total, unsigned long long* counter, int nblocks, void* stream)
+extern "C" int tk_launch_group_q(const tk_prob* probs, const int64_t* tile_end, const int* tile_prob, int nprobs,
+                                 int64_t total, const int64_t* qbeg, int nq, unsigned long long* counter, int nblocks,
+                                 void* stream)
 {
     hipStream_t st = (hipStream_t)stream;
     if (nprobs <= 0 || total <= 0) return 0;
-    hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, st, counter);
+    if (nq < 1 || nq > 64 || (nq > 1 && !qbeg)) return (int)hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, st, counter, nq);
     hipLaunchKernelGGL(k_group, dim3((unsigned)nblocks), dim3(256), 0, st, probs, tile_end, tile_prob, nprobs, total,
-                       counter);
+                       nq > 1 ? qbeg : (const int64_t*)nullptr, nq, counter);
     return (int)hipGetLastError();
+}
+
+extern "C" int tk_launch_group(const tk_prob* probs, const int64_t* tile_end, const int* tile_prob, int nprobs,
+                               int64_t total, unsigned long long* counter, int nblocks, void* stream)
+{
+    return tk_launch_group_q(probs, tile_end, tile_prob, nprobs, total, nullptr, 1, counter, nblocks, stream);
 }

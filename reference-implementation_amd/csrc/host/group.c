@@ -19,6 +19,9 @@
 
 #include "internal.h"
 
+/* tile queues of a product group: one per XCD of an MI355X */
+#define GROUP_QUEUES 8
+
 struct tapp_group {
     uint32_t magic;
     struct tapp_exec* ex;
@@ -29,7 +32,8 @@ struct tapp_group {
     tk_prob* dprobs;
     int64_t* dtile_end;
     int* dtile_prob;
-    unsigned long long* dcounter;
+    int64_t* dqbeg;            /* first tile of each queue, GROUP_QUEUES + 1 entries */
+    unsigned long long* dcounter;   /* one counter per queue, 256 bytes apart */
     hipGraph_t graph;
     hipGraphExec_t gexec;
     bool have_graph;
@@ -367,13 +371,30 @@ static int create_group(TAPP_AMD_product_group* group, TAPP_executor exec, int c
         size_t pb = sizeof(tk_prob) * (size_t)n, tb = sizeof(int64_t) * (size_t)n;
         tk_prob* hp = malloc(pb);
         int64_t* he = malloc(tb);
-        if (!hp || !he) { free(hp); free(he); free(list); group_free(g); return TAPP_AMD_ERR_NOMEM; }
-        int64_t total = 0;
+        int* bin = malloc(sizeof(int) * (size_t)n);
+        if (!hp || !he || !bin) { free(hp); free(he); free(bin); free(list); group_free(g); return TAPP_AMD_ERR_NOMEM; }
+        /* one tile queue per XCD: each product goes whole to the queue with
+         * the least work so far (products in cost order), so its tiles share
+         * one L2; queues are concatenated, each in cost order */
+        double work[GROUP_QUEUES] = {0};
         for (int i = 0; i < n; i++) {
-            hp[i] = list[i].p;
-            total += hp[i].L * hp[i].tiles_s * hp[i].tiles_g;
-            he[i] = total;
+            int q = 0;
+            for (int j = 1; j < GROUP_QUEUES; j++) if (work[j] < work[q]) q = j;
+            bin[i] = q;
+            work[q] += list[i].cost * (double)(list[i].p.L * list[i].p.tiles_s * list[i].p.tiles_g);
         }
+        int64_t total = 0, qbeg[GROUP_QUEUES + 1];
+        for (int q = 0, k = 0; q < GROUP_QUEUES; q++) {
+            qbeg[q] = total;
+            for (int i = 0; i < n; i++) {
+                if (bin[i] != q) continue;
+                hp[k] = list[i].p;
+                total += hp[k].L * hp[k].tiles_s * hp[k].tiles_g;
+                he[k++] = total;
+            }
+        }
+        qbeg[GROUP_QUEUES] = total;
+        free(bin);
         g->total = total;
         /* tile -> product map (int32), so a workgroup finds its product in one load */
         size_t mb = total <= INT32_MAX ? sizeof(int) * (size_t)total : 0;
@@ -382,16 +403,19 @@ static int create_group(TAPP_AMD_product_group* group, TAPP_executor exec, int c
             for (int i = 0, t = 0; i < n; i++)
                 for (; t < he[i]; t++) hm[t] = i;
         size_t o_end = (pb + 255) & ~(size_t)255, o_map = (o_end + tb + 255) & ~(size_t)255;
-        size_t o_cnt = (o_map + (hm ? mb : 0) + 255) & ~(size_t)255;
-        hipError_t e = hipMalloc(&g->dmem, o_cnt + 256);
+        size_t o_q = (o_map + (hm ? mb : 0) + 255) & ~(size_t)255;
+        size_t o_cnt = (o_q + sizeof qbeg + 255) & ~(size_t)255;
+        hipError_t e = hipMalloc(&g->dmem, o_cnt + 256 * GROUP_QUEUES);
         if (e != hipSuccess) { (void)hipGetLastError(); free(hp); free(he); free(hm); free(list); group_free(g); return TAPP_AMD_ERR_NOMEM; }
         g->dprobs = (tk_prob*)g->dmem;
         g->dtile_end = (int64_t*)((char*)g->dmem + o_end);
         g->dtile_prob = hm ? (int*)((char*)g->dmem + o_map) : NULL;
+        g->dqbeg = (int64_t*)((char*)g->dmem + o_q);
         g->dcounter = (unsigned long long*)((char*)g->dmem + o_cnt);
         e = hipMemcpy(g->dprobs, hp, pb, hipMemcpyHostToDevice);
         if (e == hipSuccess) e = hipMemcpy(g->dtile_end, he, tb, hipMemcpyHostToDevice);
         if (e == hipSuccess && hm) e = hipMemcpy(g->dtile_prob, hm, mb, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(g->dqbeg, qbeg, sizeof qbeg, hipMemcpyHostToDevice);
         free(hp); free(he); free(hm);
         if (e != hipSuccess) { free(list); group_free(g); return TAPP_AMD_ERR_DEVICE; }
         int cus = 0;
@@ -401,8 +425,8 @@ static int create_group(TAPP_AMD_product_group* group, TAPP_executor exec, int c
         g->nblocks = (int)(total < nb ? total : nb);
         /* capture zero-counter + persistent kernel once; replays reuse it */
         if (capture && hipStreamBeginCapture(ex->stream, hipStreamCaptureModeThreadLocal) == hipSuccess) {
-            int lrc = tk_launch_group(g->dprobs, g->dtile_end, g->dtile_prob, n, total, g->dcounter, g->nblocks,
-                                      ex->stream);
+            int lrc = tk_launch_group_q(g->dprobs, g->dtile_end, g->dtile_prob, n, total, g->dqbeg, GROUP_QUEUES,
+                                        g->dcounter, g->nblocks, ex->stream);
             hipError_t ce = hipStreamEndCapture(ex->stream, &g->graph);
             if (lrc == 0 && ce == hipSuccess && hipGraphInstantiate(&g->gexec, g->graph, NULL, NULL, 0) == hipSuccess)
                 g->have_graph = true;
@@ -456,8 +480,8 @@ TAPP_error TAPP_AMD_execute_product_group(TAPP_AMD_product_group group)
     if (g->n > 0) {
         if (g->have_graph) {
             if (hipGraphLaunch(g->gexec, ex->stream) != hipSuccess) return TAPP_AMD_ERR_DEVICE;
-        } else if (tk_launch_group(g->dprobs, g->dtile_end, g->dtile_prob, g->n, g->total, g->dcounter, g->nblocks,
-                                   ex->stream)) {
+        } else if (tk_launch_group_q(g->dprobs, g->dtile_end, g->dtile_prob, g->n, g->total, g->dqbeg, GROUP_QUEUES,
+                                     g->dcounter, g->nblocks, ex->stream)) {
             return TAPP_AMD_ERR_DEVICE;
         }
     }

@@ -144,6 +144,12 @@ int tk_launch_scatter(const tk_scatter* s, void* stream);
  * tile_prob (optional, int[total]) maps a tile to its product. */
 int tk_launch_group(const tk_prob* probs, const int64_t* tile_end, const int* tile_prob, int nprobs,
                     int64_t total, unsigned long long* counter, int nblocks, void* stream);
+/* Same with `nq` tile queues: queue q holds tiles [qbeg[q], qbeg[q + 1])
+ * (qbeg on the device, nq + 1 entries) and counts with counter[32 q]; a
+ * workgroup drains queue blockIdx % nq first, then the others. */
+int tk_launch_group_q(const tk_prob* probs, const int64_t* tile_end, const int* tile_prob, int nprobs,
+                      int64_t total, const int64_t* qbeg, int nq, unsigned long long* counter, int nblocks,
+                      void* stream);
 
 /* Number of kernel variants compiled in, for diagnostics. */
 int tk_kernel_count(void);
