@@ -825,10 +825,10 @@ __global__ __launch_bounds__(256, MODE == TK_MODE_SPLIT6_BF16 ? 1 : 2) void k_ge
     const int64_t klen = ke > kb ? ke - kb : 0;
     const int64_t nk = (klen + BK - 1) / BK, nk_full = klen / BK;
 
-    LpTile<AL, BK> la;
-    LpTile<BL, BK> lb;
-    la.rows(g.mA, m0, g.M);
-    lb.rows(g.nB, n0, g.N);
+    LpTile<AL, BK> la0;
+    LpTile<BL, BK> lb0;
+    la0.rows(g.mA, m0, g.M);
+    lb0.rows(g.nB, n0, g.N);
 
     for (int64_t l = blockIdx.y; l < g.L; l += gridDim.y) {
         const float* Ab = (const float*)g.A + g.lA[l];
@@ -842,27 +842,20 @@ __global__ __launch_bounds__(256, MODE == TK_MODE_SPLIT6_BF16 ? 1 : 2) void k_ge
                 if (S6) { sum[fn][fm] = f32x16{}; cmp[fn][fm] = f32x16{}; }
             }
 
-        auto fetch = [&](int64_t kt) {
+        auto fetch = [&](LpTile<AL, BK>& ta, LpTile<BL, BK>& tb, int64_t kt) {
             if (interior && kt < nk_full) {
-                la.template load<false>(Ab, g.kA, m0, kb + kt * BK, g.M, ke);
-                lb.template load<false>(Bb, g.kB, n0, kb + kt * BK, g.N, ke);
+                ta.template load<false>(Ab, g.kA, m0, kb + kt * BK, g.M, ke);
+                tb.template load<false>(Bb, g.kB, n0, kb + kt * BK, g.N, ke);
             } else {
-                la.template load<true>(Ab, g.kA, m0, kb + kt * BK, g.M, ke);
-                lb.template load<true>(Bb, g.kB, n0, kb + kt * BK, g.N, ke);
+                ta.template load<true>(Ab, g.kA, m0, kb + kt * BK, g.M, ke);
+                tb.template load<true>(Bb, g.kB, n0, kb + kt * BK, g.N, ke);
             }
         };
-        auto stash = [&](int buf) {
-            la.template store<MODE>(lds[buf][0], lds[buf][P > 1 ? 1 : 0], lds[buf][P > 2 ? 2 : 0]);
-            lb.template store<MODE>(lds[buf][P], lds[buf][P > 1 ? P + 1 : P], lds[buf][P > 2 ? P + 2 : P]);
+        auto stash = [&](const LpTile<AL, BK>& ta, const LpTile<BL, BK>& tb, int buf) {
+            ta.template store<MODE>(lds[buf][0], lds[buf][P > 1 ? 1 : 0], lds[buf][P > 2 ? 2 : 0]);
+            tb.template store<MODE>(lds[buf][P], lds[buf][P > 1 ? P + 1 : P], lds[buf][P > 2 ? P + 2 : P]);
         };
-        if (nk > 0) {
-            fetch(0);
-            stash(0);
-        }
-        __syncthreads();
-        for (int64_t kt = 0; kt < nk; ++kt) {
-            const int cur = kt & 1;
-            if (kt + 1 < nk) fetch(kt + 1);
+        auto compute = [&](int cur) {
 #pragma unroll
             for (int kk = 0; kk < BK; kk += 16) {
                 uint4 a[P][FM], b[P][FN];
@@ -875,22 +868,19 @@ __global__ __launch_bounds__(256, MODE == TK_MODE_SPLIT6_BF16 ? 1 : 2) void k_ge
                     for (int fn = 0; fn < FN; ++fn)
                         b[q][fn] = lp_operand<LDK>(lds[cur][P + q], wn * FN * S + fn * S + il, kk + 8 * kl);
                 }
+                // products of the split pieces, smallest terms first; each
+                // product over all FN x FM blocks before the next, so that
+                // back-to-back MFMAs write different accumulators
+                constexpr int NPROD = S6 ? 6 : S3 ? 3 : 1;
+                constexpr int PB[6] = {1, 2, 0, 1, 0, 0};   // B piece (h = 0, m = 1, l = 2)
+                constexpr int PA[6] = {1, 0, 2, 0, 1, 0};   // A piece: m*m, h*l, l*h, h*m, m*h, h*h
 #pragma unroll
-                for (int fn = 0; fn < FN; ++fn)
+                for (int pr = 6 - NPROD; pr < 6; ++pr)
 #pragma unroll
-                    for (int fm = 0; fm < FM; ++fm) {
-                        // smallest terms first
-                        if (S6) {
-                            acc[fn][fm] = mfma16<MODE>(b[1][fn], a[1][fm], acc[fn][fm]);   // m*m
-                            acc[fn][fm] = mfma16<MODE>(b[2][fn], a[0][fm], acc[fn][fm]);   // h*l
-                            acc[fn][fm] = mfma16<MODE>(b[0][fn], a[2][fm], acc[fn][fm]);   // l*h
-                        }
-                        if (S3 || S6) {
-                            acc[fn][fm] = mfma16<MODE>(b[1][fn], a[0][fm], acc[fn][fm]);   // h*m (h*lo)
-                            acc[fn][fm] = mfma16<MODE>(b[0][fn], a[1][fm], acc[fn][fm]);   // m*h (lo*h)
-                        }
-                        acc[fn][fm] = mfma16<MODE>(b[0][fn], a[0][fm], acc[fn][fm]);
-                    }
+                    for (int fn = 0; fn < FN; ++fn)
+#pragma unroll
+                        for (int fm = 0; fm < FM; ++fm)
+                            acc[fn][fm] = mfma16<MODE>(b[PB[pr]][fn], a[PA[pr]][fm], acc[fn][fm]);
             }
             if (S6) {
                 // this K tile's FP32 sum into the compensated total
@@ -907,7 +897,18 @@ __global__ __launch_bounds__(256, MODE == TK_MODE_SPLIT6_BF16 ? 1 : 2) void k_ge
                             acc[fn][fm][r] = 0.0f;
                         }
             }
-            if (kt + 1 < nk) stash(cur ^ 1);
+        };
+
+        if (nk > 0) {
+            fetch(la0, lb0, 0);
+            stash(la0, lb0, 0);
+        }
+        __syncthreads();
+        for (int64_t kt = 0; kt < nk; ++kt) {
+            const int cur = kt & 1;
+            if (kt + 1 < nk) fetch(la0, lb0, kt + 1);   // next tile's loads in flight during the MFMAs
+            compute(cur);
+            if (kt + 1 < nk) stash(la0, lb0, cur ^ 1);
             __syncthreads();
         }
         if (S6) {
