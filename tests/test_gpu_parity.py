@@ -398,3 +398,54 @@ def test_reference_style_executor_int(lib, oracle):
     bad = ctypes.c_int(5)
     rc, _ = T.run_lib(lib, c, ctypes.addressof(bad))
     assert rc == 33
+
+
+def _nan_c(case):
+    """beta = 0 and C full of NaN: the reference skips C entirely when beta
+    is zero (product.c:351-360), so no NaN may reach D."""
+    case.beta = 0.0
+    nan = complex(float("nan"), float("nan")) if case.C.type in T.COMPLEX_TYPES else float("nan")
+    case.C.buf[:] = nan
+    return case
+
+
+@pytest.mark.parametrize("dtype,prec,M,N,K,kernel", [
+    (tc.F32, tc.DEFAULT_PREC, 3, 70, 40, tc.KERNEL_SIMPLE),        # thin: one lane per output
+    (tc.F32, tc.DEFAULT_PREC, 4, 4, 2048, tc.KERNEL_SPLITK),       # few outputs, long K
+    (tc.F32, tc.DEFAULT_PREC, 200, 130, 257, tc.KERNEL_MFMA),      # split-K over the FP32 MFMA kernel
+    (tc.F32, tc.DEFAULT_PREC, 640, 520, 48, tc.KERNEL_MFMA),       # FP32 MFMA kernel without split-K
+    (tc.F32, tc.BF16BF16_ACCUM_F32, 200, 130, 257, tc.KERNEL_MFMA),
+    (tc.F64, tc.DEFAULT_PREC, 96, 80, 33, tc.KERNEL_MFMA),
+    (tc.C64, tc.DEFAULT_PREC, 64, 48, 20, tc.KERNEL_MFMA),
+])
+def test_beta_zero_ignores_nan_in_c(lib, oracle, dtype, prec, M, N, K, kernel):
+    _, ex = lib.create_executor()
+    assert lib.attr_set_int(ex, tc.ATTR_KERNEL, kernel) == 0
+    c = _nan_c(_gemm_case(dtype, M, N, K, 1234))
+    c.prec = prec
+    rc, got, want = run_both(lib, ex, oracle, c)
+    assert rc == 0
+    mask = T.written_mask(c.D)
+    assert np.all(np.isfinite(np.asarray(got)[mask])), "NaN from C reached D"
+    assert np.all(np.isfinite(np.asarray(want)[mask]))
+    assert_parity(got, want, c, tol=1e-5 if dtype in (tc.F32, tc.C32) else 1e-12)
+    assert lib.attr_get_int(ex, tc.ATTR_LAST_KERNEL)[1] == kernel
+    lib.destroy_executor(ex)
+
+
+def test_beta_zero_ignores_nan_in_c_in_group(lib, gpu_exec, oracle):
+    from test_group import DevCase, make_group
+    cases = [_nan_c(T.tucker_case(40 + i, 12, 9, 7, 3)) for i in range(3)]
+    cases += [_nan_c(T.four_index_case(50 + i, (6, 7, 8, 5, 9, 4))) for i in range(3)]
+    dcs = [DevCase(lib, c) for c in cases]
+    rc, g = make_group(lib, gpu_exec, dcs)
+    assert rc == 0
+    assert lib.execute_group(g) == 0
+    for d in dcs:
+        _, want = oracle.run(d.case)
+        got = d.d()
+        assert np.all(np.isfinite(got[T.written_mask(d.case.D)]))
+        assert_parity(got, want, d.case)
+    lib.destroy_group(g)
+    for d in dcs:
+        d.close()
