@@ -631,23 +631,30 @@ __global__ __launch_bounds__(256, 2) void k_gett_f32(tk_gett g)
             if (kt + 1 < nk) fetch(kt + 1);
             const float* as = As[cur];
             const float* bs = Bs[cur];
+            auto mfmas = [&](int k_begin, int k_end) {
 #pragma unroll
-            for (int kk = 0; kk < BK; kk += 2) {
-                float a[FM], b[FN];
+                for (int kk = k_begin; kk < k_end; kk += 2) {
+                    float a[FM], b[FN];
 #pragma unroll
-                for (int fm = 0; fm < FM; ++fm) a[fm] = as[(kk + kl) * LDA + wm * FM * S + fm * S + il];
+                    for (int fm = 0; fm < FM; ++fm) a[fm] = as[(kk + kl) * LDA + wm * FM * S + fm * S + il];
 #pragma unroll
-                for (int fn = 0; fn < FN; ++fn) b[fn] = bs[(kk + kl) * LDB + wn * FN * S + fn * S + il];
+                    for (int fn = 0; fn < FN; ++fn) b[fn] = bs[(kk + kl) * LDB + wn * FN * S + fn * S + il];
 #pragma unroll
-                for (int fn = 0; fn < FN; ++fn)
+                    for (int fn = 0; fn < FN; ++fn)
 #pragma unroll
-                    for (int fm = 0; fm < FM; ++fm)
-                        acc[fn][fm] = __builtin_amdgcn_mfma_f32_32x32x2f32(b[fn], a[fm], acc[fn][fm], 0, 0, 0);
-            }
+                        for (int fm = 0; fm < FM; ++fm)
+                            acc[fn][fm] = __builtin_amdgcn_mfma_f32_32x32x2f32(b[fn], a[fm], acc[fn][fm], 0, 0, 0);
+                }
+            };
+            // the other LDS buffer has been free since the last barrier: the
+            // next tile goes there half-way through this tile's MFMAs, so
+            // the stores overlap the MFMA pipe instead of following it
+            mfmas(0, BK / 2);
             if (kt + 1 < nk) {
                 la.template store<LDA>(As[cur ^ 1]);
                 lb.template store<LDB>(Bs[cur ^ 1]);
             }
+            mfmas(BK / 2, BK);
             __syncthreads();
         }
 
