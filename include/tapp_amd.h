@@ -58,7 +58,18 @@ enum {
      * 2 = use cached choices, never measure.  Measuring runs the product
      * several times, so it is skipped when C and D overlap with beta != 0
      * and for requests of the serving loop. */
-    TAPP_AMD_ATTR_AUTOTUNE = 0x1007
+    TAPP_AMD_ATTR_AUTOTUNE = 0x1007,
+    /* int: 1 = self-checking executor, the reference's executor kind 12
+     * (product.c:229-249, 424-448): TAPP_execute_product computes each
+     * product twice with independent kernel families (the check into a
+     * device copy of D), compares the results element by element with the
+     * reference's rule (tblis_bind.cpp:577-640: |(a - b) / max(a, b)| >
+     * 5e-5; binary64 complex also needs an absolute difference > 1e-12),
+     * prints up to 16 differing positions to stdout and returns 137 when any
+     * differ.  Setting it writes 12 (or 1) into the executor's kind, so
+     * writing 12 into the int the executor points at, as a reference caller
+     * does, has the same effect.  Default 0. */
+    TAPP_AMD_ATTR_SELF_CHECK = 0x1008
 };
 
 /* Kernel family identifiers reported by TAPP_AMD_ATTR_LAST_KERNEL. */

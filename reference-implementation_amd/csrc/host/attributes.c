@@ -34,6 +34,7 @@ TAPP_error TAPP_attr_set(TAPP_attr attr, TAPP_key key, void* value)
         e->autotune = a;
         return 0;
     }
+    case TAPP_AMD_ATTR_SELF_CHECK: e->kind = *(int*)value ? 12 : 1; return 0;
     case TAPP_AMD_ATTR_STREAM: {
         hipStream_t s = *(hipStream_t*)value;
         if (e->stream) hipStreamSynchronize(e->stream);
@@ -58,6 +59,7 @@ TAPP_error TAPP_attr_get(TAPP_attr attr, TAPP_key key, void** value)
     case TAPP_AMD_ATTR_LAST_KERNEL: *(int*)value = e->last_kernel; return 0;
     case TAPP_AMD_ATTR_LAST_DEVICE_US: *(double*)value = e->last_device_us; return 0;
     case TAPP_AMD_ATTR_STREAM: *(hipStream_t*)value = e->stream; return 0;
+    case TAPP_AMD_ATTR_SELF_CHECK: *(int*)value = e->kind == 12; return 0;
     default: return TAPP_AMD_ERR_KEY;
     }
 }
@@ -71,6 +73,7 @@ TAPP_error TAPP_attr_clear(TAPP_attr attr, TAPP_key key)
     case TAPP_AMD_ATTR_KERNEL: e->force_kernel = 0; return 0;
     case TAPP_AMD_ATTR_TIMING: e->timing = 0; return 0;
     case TAPP_AMD_ATTR_AUTOTUNE: e->autotune = 0; return 0;
+    case TAPP_AMD_ATTR_SELF_CHECK: e->kind = 1; return 0;
     default: return TAPP_AMD_ERR_KEY;
     }
 }

@@ -377,6 +377,23 @@ typedef struct {
     bool staged;
 } dptr;
 
+/* Kernel family for a planned product: `force` unless it is
+ * TK_KERNEL_AUTO, else by shape.  Symmetric in M and N, so it holds after
+ * the orientation swap. */
+int tk_kernel_for(const tk_problem* P, int force)
+{
+    const int64_t M = P->nM, N = P->nN, K = P->nK, outs = P->nM * P->nN * P->nL;
+    int kern = force;
+    if (kern == TK_KERNEL_AUTO) {
+        if (K == 0) kern = TK_KERNEL_SIMPLE;
+        else if (outs <= 4096 && K >= 1024) kern = TK_KERNEL_SPLITK;
+        else if (M >= 16 && N >= 16 && K >= 4 && outs * K >= (1 << 16)) kern = TK_KERNEL_MFMA;
+        else kern = TK_KERNEL_SIMPLE;
+    }
+    if (kern == TK_KERNEL_SPLITK && outs > (int64_t)1 << 31) kern = TK_KERNEL_SIMPLE;
+    return kern;
+}
+
 int tk_execute(struct tapp_plan* p, struct tapp_exec* ex, const void* alpha, const void* A, const void* B,
                const void* beta, const void* C, void* D)
 {
@@ -437,14 +454,7 @@ int tk_execute_on(struct tapp_plan* p, struct tapp_exec* ex, const void* alpha, 
     /* kernel choice (symmetric in M and N, so it holds after the orientation
      * swap below) */
     const int64_t outs = M * N * L;
-    int kern = ex->force_kernel;
-    if (kern == TK_KERNEL_AUTO) {
-        if (K == 0) kern = TK_KERNEL_SIMPLE;
-        else if (outs <= 4096 && K >= 1024) kern = TK_KERNEL_SPLITK;
-        else if (M >= 16 && N >= 16 && K >= 4 && outs * K >= (1 << 16)) kern = TK_KERNEL_MFMA;
-        else kern = TK_KERNEL_SIMPLE;
-    }
-    if (kern == TK_KERNEL_SPLITK && outs > (int64_t)1 << 31) kern = TK_KERNEL_SIMPLE;
+    int kern = tk_kernel_for(&P, ex->force_kernel);
 
     /* split-K for the FP32-data MFMA kernel when its 128 x 128 tiles cannot
      * fill the GPU: workgroups take K slices (multiples of its 32-deep K

@@ -135,6 +135,10 @@ int tk_execute(struct tapp_plan* p, struct tapp_exec* ex, const void* alpha, con
                const void* B, const void* beta, const void* C, void* D);
 int tk_execute_on(struct tapp_plan* p, struct tapp_exec* ex, const void* alpha, const void* A,
                   const void* B, const void* beta, const void* C, void* D, bool sync);
+int tk_kernel_for(const tk_problem* P, int force);
+/* self-checking execution (executor kind 12, check.c) */
+int tk_execute_checked(struct tapp_plan* p, struct tapp_exec* ex, const void* alpha, const void* A,
+                       const void* B, const void* beta, const void* C, void* D);
 void tk_plan_cache_free(struct tapp_plan_cache* c);
 int tk_problem_tables(struct tapp_plan* plan, const tk_problem* P, int device, const int64_t** mA,
                       const int64_t** kA, const int64_t** lA, const int64_t** nB, const int64_t** kB,

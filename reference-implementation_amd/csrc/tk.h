@@ -151,6 +151,22 @@ int tk_launch_group_q(const tk_prob* probs, const int64_t* tile_end, const int* 
                       int64_t total, const int64_t* qbeg, int nq, unsigned long long* counter, int nblocks,
                       void* stream);
 
+/* Element-wise comparison of two results of one product (self-checking
+ * executor): x[i] and y[i] (storage type `type`, TAPP_datatype) for i < n.
+ * *count receives the number of differing i and first[] up to
+ * TK_COMPARE_KEEP of them, in no particular order. */
+#define TK_COMPARE_KEEP 16
+typedef struct tk_compare {
+    int type;
+    const void* x;
+    const void* y;
+    int64_t n;
+    unsigned long long* count;
+    int64_t* first;          /* [TK_COMPARE_KEEP] */
+} tk_compare;
+
+int tk_launch_compare(const tk_compare* c, void* stream);
+
 /* Number of kernel variants compiled in, for diagnostics. */
 int tk_kernel_count(void);
 

@@ -28,6 +28,7 @@ ATTR_LAST_DEVICE_US = 0x1004
 ATTR_STREAM = 0x1005
 ATTR_TIMING = 0x1006
 ATTR_AUTOTUNE = 0x1007
+ATTR_SELF_CHECK = 0x1008
 # this library's error codes (tapp_amd.h)
 AMD_ERR_UNSUPPORTED, AMD_ERR_DEVICE, AMD_ERR_NOMEM, AMD_ERR_INVALID, AMD_ERR_KEY, AMD_ERR_BUDGET, AMD_ERR_BUSY = \
     range(101, 108)
