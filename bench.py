@@ -529,7 +529,7 @@ def main():
                     help="requests per arrival rate in the serving sweep (the full workload is 200000)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (calibrates the FETCH_SIZE / WRITE_SIZE counters)")
-    ap.add_argument("--only-form", choices=["fp32", "bf16", "f16", "split_bf16x3", "split_bf16x6"],
+    ap.add_argument("--only-form", choices=["fp32", "bf16", "f16", "split_bf16x3", "split_bf16x6", "split_f16x3"],
                     help="only time one form of the ladder (for profiler runs)")
     args = ap.parse_args()
     # autotuned measurements start from an empty cache on every run
@@ -557,7 +557,7 @@ def main():
     if args.only_form:
         prec, split = {"fp32": (tc.DEFAULT_PREC, 0), "bf16": (tc.BF16BF16_ACCUM_F32, 0),
                        "f16": (tc.F16F16_ACCUM_F32, 0), "split_bf16x3": (tc.DEFAULT_PREC, 1),
-                       "split_bf16x6": (tc.DEFAULT_PREC, 2)}[args.only_form]
+                       "split_bf16x6": (tc.DEFAULT_PREC, 2), "split_f16x3": (tc.DEFAULT_PREC, 3)}[args.only_form]
         w, du, k = time_form(lib, torch, prec, split, A, B, D, args.steps, args.warmup)
         print(json.dumps({"form": args.only_form, "kernel_us": du, "gflops": flops / w / 1e9, "kernel": k}))
         return
@@ -577,6 +577,7 @@ def main():
         forms["fp32"] = {"gflops": value, "kernel_us": dev_us,
                          "max_rel_err_vs_fp64": (D_fp32.double() - D64).abs().max().item() / den}
         for name, prec, split in (("split_bf16x3", tc.DEFAULT_PREC, 1), ("split_bf16x6", tc.DEFAULT_PREC, 2),
+                                  ("split_f16x3", tc.DEFAULT_PREC, 3),
                                   ("bf16", tc.BF16BF16_ACCUM_F32, 0), ("f16", tc.F16F16_ACCUM_F32, 0)):
             log(f"bench: form {name}")
             w, du, k = time_form(lib, torch, prec, split, A, B, D, max(1, args.steps // 4), 1)

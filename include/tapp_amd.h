@@ -31,11 +31,21 @@ enum {
  * pointer to an int unless stated. */
 enum {
     /* FP32 products with TAPP_DEFAULT_PREC or TAPP_F32F32_ACCUM_F32 run on
-     * BF16 MFMA: 1 = split-BF16 (x = hi + lo, three products, FP32
-     * accumulate: FP32-level accuracy at BF16 speed); 2 = three-piece
-     * split-BF16 (x = h + m + l, six products, per-K-tile FP32 sums added
-     * with compensation: below FP32's own error, at about FP32 speed).
-     * Opt-in; default 0. */
+     * 16-bit MFMA with FP32 accumulation:
+     * 1 = split-BF16 (x = hi + lo in bfloat16, three products: FP32-level
+     *     accuracy at twice FP32's speed);
+     * 2 = three-piece split-BF16 (x = h + m + l, six products, FP32 sums of
+     *     128 k added with compensation: far below FP32's own error, at
+     *     about FP32 speed);
+     * 3 = split-F16 (x = hi + lo in binary16, three products, FP32 sums of
+     *     128 k added into a total: below FP32's own error at about twice
+     *     its speed).  binary16's range is checked: the kernel reports
+     *     max |A| and max |B|, and an operand outside [2^-3, 2^15) is rerun
+     *     scaled by a power of two, or on the FP32 kernel for NaN, inf and
+     *     magnitudes below 2^-100.  A call that cannot rerun (C overlapping
+     *     D with beta != 0, requests of the serving loop) runs value 2.
+     * Products small enough for the one-lane-per-output kernel run it in
+     * FP32 with value 3.  Opt-in; default 0. */
     TAPP_AMD_ATTR_SPLIT_BF16 = 0x1001,
     /* Force a kernel family: 0 auto, 1 one-lane-per-output FMA, 2 MFMA,
      * 3 split-K.  Default 0. */

@@ -396,7 +396,7 @@ typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 template <int MODE>
 __device__ __forceinline__ f32x16 mfma16(uint4 b, uint4 a, f32x16 c)
 {
-    if (MODE == TK_MODE_F16)
+    if (MODE == TK_MODE_F16 || MODE == TK_MODE_SPLIT_F16)
         return __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(f16x8, b), __builtin_bit_cast(f16x8, a), c,
                                                       0, 0, 0);
     return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, b), __builtin_bit_cast(bf16x8, a), c,
@@ -682,12 +682,16 @@ __device__ __forceinline__ int lp_swz(int row, int q) { return q ^ (2 * ((row >>
 template <int MODE>
 __device__ __forceinline__ void lp_pack(const float (&x)[4], uint2& hi, uint2& lo, uint2& lo2)
 {
-    if (MODE == TK_MODE_F16) {
+    if (MODE == TK_MODE_F16 || MODE == TK_MODE_SPLIT_F16) {
         typedef _Float16 f16x4 __attribute__((ext_vector_type(4)));
-        f16x4 h;
+        f16x4 h, l;
 #pragma unroll
-        for (int e = 0; e < 4; ++e) h[e] = (_Float16)x[e];      // round to nearest even
+        for (int e = 0; e < 4; ++e) {
+            h[e] = (_Float16)x[e];                               // round to nearest even
+            if (MODE == TK_MODE_SPLIT_F16) l[e] = (_Float16)(x[e] - (float)h[e]);   // exact remainder
+        }
         hi = __builtin_bit_cast(uint2, h);
+        if (MODE == TK_MODE_SPLIT_F16) lo = __builtin_bit_cast(uint2, l);
     } else {
         typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
         bf16x4 h, l, l2;
@@ -780,9 +784,12 @@ struct LpTile {
     }
 
     // converted rows into the K-major 16-bit planes (hi; lo for split; the
-    // third piece for the three-piece split)
+    // third piece for the three-piece split).  Split-F16 first raises amax to
+    // the bits of |x| (integer max of the sign-cleared bits: every NaN and
+    // inf lands above every finite value) and multiplies by the scale.
     template <int MODE>
-    __device__ __forceinline__ void store(uint16_t* hi, uint16_t* lo, uint16_t* lo2) const
+    __device__ __forceinline__ void store(uint16_t* hi, uint16_t* lo, uint16_t* lo2, float scale,
+                                          uint32_t& amax) const
     {
 #pragma unroll
         for (int b = 0; b < NB; ++b)
@@ -791,9 +798,21 @@ struct LpTile {
                 const int row = 4 * gi() + j;
                 const int off = row * LDK + 4 * lp_swz(row, gk(b));
                 uint2 h, l, l2;
-                lp_pack<MODE>(v[b][j], h, l, l2);
+                if (MODE == TK_MODE_SPLIT_F16) {
+                    float x[4];
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        const uint32_t u = __float_as_uint(v[b][j][e]) & 0x7fffffffu;
+                        amax = u > amax ? u : amax;
+                        x[e] = v[b][j][e] * scale;
+                    }
+                    lp_pack<MODE>(x, h, l, l2);
+                } else {
+                    lp_pack<MODE>(v[b][j], h, l, l2);
+                }
                 *reinterpret_cast<uint2*>(hi + off) = h;
-                if (MODE == TK_MODE_SPLIT_BF16 || MODE == TK_MODE_SPLIT6_BF16) *reinterpret_cast<uint2*>(lo + off) = l;
+                if (MODE == TK_MODE_SPLIT_BF16 || MODE == TK_MODE_SPLIT6_BF16 || MODE == TK_MODE_SPLIT_F16)
+                    *reinterpret_cast<uint2*>(lo + off) = l;
                 if (MODE == TK_MODE_SPLIT6_BF16) *reinterpret_cast<uint2*>(lo2 + off) = l2;
             }
     }
@@ -806,12 +825,33 @@ __device__ __forceinline__ uint4 lp_operand(const uint16_t* plane, int row, int 
     return *reinterpret_cast<const uint4*>(plane + row * LDK + 4 * lp_swz(row, kk >> 2));
 }
 
+// One wave per SIMD for the three-piece split, and for split-F16 when
+// neither operand loads rows contiguously (four row offsets per operand):
+// at two waves those would spill.
+template <int AL, int BL, int MODE>
+constexpr int lp_waves()
+{
+    return MODE == TK_MODE_SPLIT6_BF16 ||
+                   (MODE == TK_MODE_SPLIT_F16 && AL != TK_CONTIG_ROW && BL != TK_CONTIG_ROW)
+               ? 1
+               : 2;
+}
+
 template <int AL, int BL, int MODE, bool SPLIT>
-__global__ __launch_bounds__(256, MODE == TK_MODE_SPLIT6_BF16 ? 1 : 2) void k_gett_lp(tk_gett g)
+__global__ __launch_bounds__(256, (lp_waves<AL, BL, MODE>())) void k_gett_lp(tk_gett g)
 {
     // K tile: 64 deep (twice the bytes in flight per step) where two
     // workgroups per CU still fit in LDS; the split forms' planes need 32
-    constexpr bool S3 = MODE == TK_MODE_SPLIT_BF16, S6 = MODE == TK_MODE_SPLIT6_BF16;
+    constexpr bool S3 = MODE == TK_MODE_SPLIT_BF16 || MODE == TK_MODE_SPLIT_F16, S6 = MODE == TK_MODE_SPLIT6_BF16;
+    // The accurate forms carry more bits per product than an FP32 sum over
+    // long K keeps, so their MFMA accumulators hold FOLD K tiles (128 k)
+    // at a time and are then added into a total: split-F16 plainly (two
+    // sum levels of about sqrt(K) terms each; the extra accumulator set
+    // still fits two waves per SIMD), the three-piece split-BF16 with
+    // compensation (Kahan).
+    constexpr bool RANGE = MODE == TK_MODE_SPLIT_F16, TWO_LEVEL = S6 || RANGE;
+    constexpr int FOLD = 4;
+    uint32_t amax_a = 0, amax_b = 0;
     constexpr int S = 32, FM = 2, FN = 2, BM = 128, BN = 128, BK = (S3 || S6) ? 32 : 64;
     constexpr int LDK = BK + 8;
     constexpr int P = S6 ? 3 : S3 ? 2 : 1;                         // planes per operand
@@ -840,13 +880,14 @@ __global__ __launch_bounds__(256, MODE == TK_MODE_SPLIT6_BF16 ? 1 : 2) void k_ge
     for (int64_t l = blockIdx.y; l < g.L; l += gridDim.y) {
         const float* Ab = (const float*)g.A + g.lA[l];
         const float* Bb = (const float*)g.B + g.lB[l];
-        f32x16 acc[FN][FM], sum[FN][FM], cmp[FN][FM];
+        f32x16 acc[FN][FM], sum[FN][TWO_LEVEL ? FM : 1], cmp[FN][S6 ? FM : 1];
 #pragma unroll
         for (int fn = 0; fn < FN; ++fn)
 #pragma unroll
             for (int fm = 0; fm < FM; ++fm) {
                 acc[fn][fm] = f32x16{};
-                if (S6) { sum[fn][fm] = f32x16{}; cmp[fn][fm] = f32x16{}; }
+                if (TWO_LEVEL) sum[fn][fm] = f32x16{};
+                if (S6) cmp[fn][fm] = f32x16{};
             }
 
         auto fetch = [&](LpTile<AL, BK>& ta, LpTile<BL, BK>& tb, int64_t kt) {
@@ -859,8 +900,10 @@ __global__ __launch_bounds__(256, MODE == TK_MODE_SPLIT6_BF16 ? 1 : 2) void k_ge
             }
         };
         auto stash = [&](const LpTile<AL, BK>& ta, const LpTile<BL, BK>& tb, int buf) {
-            ta.template store<MODE>(lds[buf][0], lds[buf][P > 1 ? 1 : 0], lds[buf][P > 2 ? 2 : 0]);
-            tb.template store<MODE>(lds[buf][P], lds[buf][P > 1 ? P + 1 : P], lds[buf][P > 2 ? P + 2 : P]);
+            ta.template store<MODE>(lds[buf][0], lds[buf][P > 1 ? 1 : 0], lds[buf][P > 2 ? 2 : 0], g.scale[0],
+                                    amax_a);
+            tb.template store<MODE>(lds[buf][P], lds[buf][P > 1 ? P + 1 : P], lds[buf][P > 2 ? P + 2 : P],
+                                    g.scale[1], amax_b);
         };
         auto compute = [&](int cur) {
 #pragma unroll
@@ -889,21 +932,25 @@ __global__ __launch_bounds__(256, MODE == TK_MODE_SPLIT6_BF16 ? 1 : 2) void k_ge
                         for (int fm = 0; fm < FM; ++fm)
                             acc[fn][fm] = mfma16<MODE>(b[PB[pr]][fn], a[PA[pr]][fm], acc[fn][fm]);
             }
-            if (S6) {
-                // this K tile's FP32 sum into the compensated total
+        };
+        // the FP32 sum of the last FOLD K tiles into the total
+        auto fold = [&]() {
 #pragma unroll
-                for (int fn = 0; fn < FN; ++fn)
+            for (int fn = 0; fn < FN; ++fn)
 #pragma unroll
-                    for (int fm = 0; fm < FM; ++fm)
+                for (int fm = 0; fm < FM; ++fm)
 #pragma unroll
-                        for (int r = 0; r < 16; ++r) {
+                    for (int r = 0; r < 16; ++r) {
+                        if (S6) {
                             const float y = acc[fn][fm][r] - cmp[fn][fm][r];
                             const float t = sum[fn][fm][r] + y;
                             cmp[fn][fm][r] = (t - sum[fn][fm][r]) - y;
                             sum[fn][fm][r] = t;
-                            acc[fn][fm][r] = 0.0f;
+                        } else {
+                            sum[fn][fm][r] += acc[fn][fm][r];
                         }
-            }
+                        acc[fn][fm][r] = 0.0f;
+                    }
         };
 
         if (nk > 0) {
@@ -916,17 +963,32 @@ __global__ __launch_bounds__(256, MODE == TK_MODE_SPLIT6_BF16 ? 1 : 2) void k_ge
             if (kt + 1 < nk) fetch(la0, lb0, kt + 1);   // next tile's loads in flight during the MFMAs
             compute(cur);
             if (kt + 1 < nk) stash(la0, lb0, cur ^ 1);
+            if (TWO_LEVEL && ((kt + 1) % FOLD == 0 || kt + 1 == nk)) fold();
             __syncthreads();
         }
-        if (S6) {
+        if (TWO_LEVEL) {
 #pragma unroll
             for (int fn = 0; fn < FN; ++fn)
 #pragma unroll
                 for (int fm = 0; fm < FM; ++fm)
 #pragma unroll
-                    for (int r = 0; r < 16; ++r) acc[fn][fm][r] = sum[fn][fm][r] - cmp[fn][fm][r];
+                    for (int r = 0; r < 16; ++r)
+                        acc[fn][fm][r] = S6 ? sum[fn][fm][r] - cmp[fn][fm][r] : sum[fn][fm][r];
         }
         tile_epilogue<SPLIT>(g, acc, l, m0, n0, tile, ntm * ntn, wave, lane);
+    }
+    if (RANGE && g.amax) {
+        // the operands' max |x| seen by this wave, one atomic per wave
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            const uint32_t oa = __shfl_xor(amax_a, off, 64), ob = __shfl_xor(amax_b, off, 64);
+            amax_a = oa > amax_a ? oa : amax_a;
+            amax_b = ob > amax_b ? ob : amax_b;
+        }
+        if (lane == 0) {
+            atomicMax(g.amax, amax_a);
+            atomicMax(g.amax + 1, amax_b);
+        }
     }
 }
 
@@ -966,6 +1028,7 @@ static hipError_t launch_gett_f32_modes(const tk_gett& g, hipStream_t st)
     case TK_MODE_BF16: TK_LAUNCH(k_gett_lp, TK_MODE_BF16); break;
     case TK_MODE_F16: TK_LAUNCH(k_gett_lp, TK_MODE_F16); break;
     case TK_MODE_SPLIT_BF16: TK_LAUNCH(k_gett_lp, TK_MODE_SPLIT_BF16); break;
+    case TK_MODE_SPLIT_F16: TK_LAUNCH(k_gett_lp, TK_MODE_SPLIT_F16); break;
     default: TK_LAUNCH(k_gett_lp, TK_MODE_SPLIT6_BF16); break;
     }
 #undef TK_LAUNCH

@@ -17,7 +17,7 @@ TAPP_error TAPP_attr_set(TAPP_attr attr, TAPP_key key, void* value)
     switch (key) {
     case TAPP_AMD_ATTR_SPLIT_BF16: {
         int v = *(int*)value;
-        if (v < 0 || v > 2) return TAPP_AMD_ERR_INVALID;
+        if (v < 0 || v > 3) return TAPP_AMD_ERR_INVALID;
         e->split_bf16 = v;
         return 0;
     }

@@ -8,6 +8,8 @@
  * they are built once per (plan, layout) and kept on the device in the
  * plan's cache; a call with unchanged layouts uploads nothing.
  */
+#include <float.h>
+#include <math.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -394,6 +396,58 @@ int tk_kernel_for(const tk_problem* P, int force)
     return kern;
 }
 
+/* Power-of-two scale exponent for split-F16 from an operand's max |x|
+ * (bits as the kernel reports them): 0 when binary16 already holds the
+ * operand at full precision (max in [2^-3, 2^15): hi and, for the largest
+ * elements, lo are normal numbers, and nothing overflows), else e with
+ * max * 2^e in [2^14, 2^15).  Returns false for NaN / inf and for maxima
+ * below 2^-100, which take the FP32 kernel. */
+static bool f16_scale_exp(uint32_t bits, int* e)
+{
+    *e = 0;
+    if (bits >= 0x7f800000u) return false;
+    float m;
+    memcpy(&m, &bits, sizeof m);
+    if (m == 0.0f || (m >= 0x1p-3f && m < 0x1p15f)) return true;
+    if (m < 0x1p-100f) return false;
+    int ex2;
+    frexpf(m, &ex2);            /* m = f 2^ex2, f in [0.5, 1) */
+    *e = 15 - ex2;
+    return true;
+}
+
+/* Split-F16 GETT: one launch at scale 1 that also reports max |A| and
+ * max |B|; when an operand is outside binary16's comfortable range a second
+ * launch either scales the operands by powers of two (alpha divided by the
+ * same, exactly) or, for NaN / inf / extreme magnitudes, runs the FP32
+ * kernel.  Synchronous; D is rewritten completely by a second launch and C
+ * is intact (the caller checked). */
+static int split_f16_gett(tk_gett* g, uint32_t* amax, hipStream_t st)
+{
+    uint32_t seen[2];
+    g->scale[0] = g->scale[1] = 1.0f;
+    g->amax = amax;
+    HIPCHK(hipMemsetAsync(amax, 0, 2 * sizeof(uint32_t), st));
+    LAUNCHCHK(tk_launch_gett(g, st));
+    HIPCHK(hipMemcpyAsync(seen, amax, sizeof seen, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    int ea, eb;
+    bool ok = f16_scale_exp(seen[0], &ea) && f16_scale_exp(seen[1], &eb);
+    if (ok && ea == 0 && eb == 0) return 0;
+    const double alpha = ok ? ldexp(g->alpha[0], -(ea + eb)) : g->alpha[0];
+    if (ok && alpha != 0.0 && (fabs(alpha) > FLT_MAX || fabs(alpha) < FLT_MIN)) ok = false;
+    if (ok) {
+        g->scale[0] = ldexpf(1.0f, ea);
+        g->scale[1] = ldexpf(1.0f, eb);
+        g->alpha[0] = alpha;
+    } else {
+        g->mode = TK_MODE_PLAIN;
+        g->amax = NULL;
+    }
+    LAUNCHCHK(tk_launch_gett(g, st));
+    return 0;
+}
+
 int tk_execute(struct tapp_plan* p, struct tapp_exec* ex, const void* alpha, const void* A, const void* B,
                const void* beta, const void* C, void* D)
 {
@@ -460,7 +514,18 @@ int tk_execute_on(struct tapp_plan* p, struct tapp_exec* ex, const void* alpha, 
      * fill the GPU: workgroups take K slices (multiples of its 32-deep K
      * tile) and store FP32 partial sums of whole tiles, which a second
      * kernel adds in a fixed order */
-    const bool may_tune = ex->autotune && ex->force_kernel == TK_KERNEL_AUTO && K > 0;
+    const bool may_tune = ex->autotune && ex->force_kernel == TK_KERNEL_AUTO && K > 0 && P.mode != TK_MODE_SPLIT_F16;
+
+    /* split-F16 runs on the MFMA GETT only: the other kernels compute in
+     * FP32, which is as accurate.  Its range check may ask for a second
+     * launch, which needs a synchronous call whose C is still intact after
+     * the first (tune_safe); otherwise the three-piece split-BF16 gives the
+     * same accuracy in one launch. */
+    int mode = P.mode;
+    if (mode == TK_MODE_SPLIT_F16) {
+        if (kern != TK_KERNEL_MFMA) mode = TK_MODE_PLAIN;
+        else if (!sync || !tune_safe(&P, src, needC)) mode = TK_MODE_SPLIT6_BF16;
+    }
     int64_t ks = 1, kchunk = 0;
     size_t part_bytes = 0;
     if ((kern == TK_KERNEL_MFMA || may_tune) && ct == TK_F32 && K >= 64) {
@@ -483,6 +548,8 @@ int tk_execute_on(struct tapp_plan* p, struct tapp_exec* ex, const void* alpha, 
     for (int x = 0; x < 4; x++) { off_tmp[x] = total; total = ALIGN256(total + tmp_sz[x]); }
     off_part = total;
     total = ALIGN256(total + part_bytes);
+    const size_t off_amax = total;
+    if (mode == TK_MODE_SPLIT_F16) total = ALIGN256(total + 2 * sizeof(uint32_t));
     if (total && (rc = ensure_ws(ex, total))) return rc;
     char* ws = (char*)ex->ws;
     hipStream_t st = ex->stream;
@@ -512,7 +579,7 @@ int tk_execute_on(struct tapp_plan* p, struct tapp_exec* ex, const void* alpha, 
     tk_gett g;
     memset(&g, 0, sizeof g);
     g.ctype = ct;
-    g.mode = P.mode;
+    g.mode = mode;
     g.M = M; g.N = N; g.K = K; g.L = L;
     g.conjA = P.op_A; g.conjB = P.op_B; g.conjC = P.op_C; g.conjD = P.op_D;
     g.conjD_via_f32 = P.conjD_via_f32;
@@ -626,7 +693,11 @@ int tk_execute_on(struct tapp_plan* p, struct tapp_exec* ex, const void* alpha, 
             kern = g.kernel;
         }
     }
-    LAUNCHCHK(tk_launch_gett(&g, st));
+    if (mode == TK_MODE_SPLIT_F16) {
+        if ((rc = split_f16_gett(&g, (uint32_t*)(ws + off_amax), st))) return rc;
+    } else {
+        LAUNCHCHK(tk_launch_gett(&g, st));
+    }
     ex->last_kernel = kern;
 
     if (needD) {

@@ -72,7 +72,9 @@ bool tk_group_describe(struct tapp_plan* plan, struct tapp_exec* ex, const tk_pr
                        const void* B, const void* C, void* D, int force_cls, tk_prob* out, int* rc)
 {
     *rc = 0;
-    if (P->ctype != TK_F32 || P->mode == TK_MODE_SPLIT_BF16 || P->mode == TK_MODE_SPLIT6_BF16) return false;
+    if (P->ctype != TK_F32 || P->mode == TK_MODE_SPLIT_BF16 || P->mode == TK_MODE_SPLIT6_BF16 ||
+        P->mode == TK_MODE_SPLIT_F16)
+        return false;
     if (P->nUA != 1 || P->nUB != 1) return false;
     for (int x = 0; x < 4; x++)
         if (P->type[x] != 0 && !(x == 2 && P->beta_zero)) return false;

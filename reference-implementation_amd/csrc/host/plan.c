@@ -343,7 +343,8 @@ int tk_plan_problem(const struct tapp_plan* p, const struct tapp_exec* ex, const
     out->ctype = cplx ? (wide ? TK_C64 : TK_C32) : (wide ? TK_F64 : TK_F32);
     if (ex->split_bf16 && !wide && out->mode == TK_MODE_PLAIN &&
         (p->prec == TAPP_DEFAULT_PREC || p->prec == TAPP_F32F32_ACCUM_F32))
-        out->mode = ex->split_bf16 == 2 ? TK_MODE_SPLIT6_BF16 : TK_MODE_SPLIT_BF16;
+        out->mode = ex->split_bf16 == 3 ? TK_MODE_SPLIT_F16
+                    : ex->split_bf16 == 2 ? TK_MODE_SPLIT6_BF16 : TK_MODE_SPLIT_BF16;
 
     /* alpha and beta are read as type_D (product.c:330-331) */
     double ar, ai;

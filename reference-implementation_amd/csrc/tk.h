@@ -30,8 +30,12 @@ enum {
     TK_MODE_F16 = 1,       /* operands rounded to binary16, FP16 MFMA, FP32 accumulate */
     TK_MODE_BF16 = 2,      /* operands rounded to bfloat16, BF16 MFMA, FP32 accumulate */
     TK_MODE_SPLIT_BF16 = 3, /* x = hi + lo in bfloat16, hi*hi + hi*lo + lo*hi, FP32 accumulate */
-    TK_MODE_SPLIT6_BF16 = 4 /* x = h + m + l in bfloat16, six products (hh, hm, mh, hl, lh, mm),
-                               FP32 sums per K tile added with compensation (Kahan) */
+    TK_MODE_SPLIT6_BF16 = 4, /* x = h + m + l in bfloat16, six products (hh, hm, mh, hl, lh, mm),
+                               FP32 sums of 128 k added with compensation (Kahan) */
+    TK_MODE_SPLIT_F16 = 5   /* x * s = hi + lo in binary16 (s a power of two per operand),
+                               hi*hi + hi*lo + lo*hi, FP32 sums of 128 k added into an FP32
+                               total; the kernel reports max |x| of each operand so the host
+                               can check the range */
 };
 
 /* how a kernel may read an operand tile */
@@ -68,6 +72,9 @@ typedef struct tk_gett {
     int ksplit;              /* > 1: K split over gridDim.z; partial sums go to `partial` */
     int64_t kchunk;          /* K length per split (multiple of the kernel's K tile) */
     float* partial;          /* [ksplit][L][N][M] FP32 partial sums (real compute only) */
+    float scale[2];          /* TK_MODE_SPLIT_F16: A and B are multiplied by these before the split */
+    uint32_t* amax;          /* TK_MODE_SPLIT_F16: [2] raised (atomic max) to the bits of max |A|,
+                                max |B| (unscaled; NaN and inf compare above every finite value) */
 } tk_gett;
 
 enum {

@@ -151,18 +151,21 @@ def test_forced_kernels_on_gemm_shapes(lib, oracle, dtype, kernel):
     lib.destroy_executor(ex)
 
 
-@pytest.mark.parametrize("mode", ["bf16", "f16", "split_bf16", "split6_bf16"])
+SPLIT_ATTR = {"split_bf16": 1, "split6_bf16": 2, "split_f16": 3}
+SPLIT_TOL = {"split_bf16": 1e-5, "split6_bf16": 1e-6, "split_f16": 1e-6}
+
+
+@pytest.mark.parametrize("mode", ["bf16", "f16", "split_bf16", "split6_bf16", "split_f16"])
 def test_low_precision_kernel_load_modes(lib, oracle, mode):
     """The 16-bit-input GETT on GEMM shapes whose layouts reach every pair of
     load modes (rows contiguous, k contiguous, gathered), with ragged edges
     and beta != 0: same input rounding as the oracle, so only the FP32
     summation order differs; split-BF16 against the FP64 oracle."""
-    prec = {"bf16": tc.BF16BF16_ACCUM_F32, "f16": tc.F16F16_ACCUM_F32, "split_bf16": tc.DEFAULT_PREC,
-            "split6_bf16": tc.DEFAULT_PREC}[mode]
+    prec = {"bf16": tc.BF16BF16_ACCUM_F32, "f16": tc.F16F16_ACCUM_F32}.get(mode, tc.DEFAULT_PREC)
     _, ex = lib.create_executor()
     assert lib.attr_set_int(ex, tc.ATTR_KERNEL, tc.KERNEL_MFMA) == 0
     if mode.startswith("split"):
-        assert lib.attr_set_int(ex, tc.ATTR_SPLIT_BF16, 2 if mode == "split6_bf16" else 1) == 0
+        assert lib.attr_set_int(ex, tc.ATTR_SPLIT_BF16, SPLIT_ATTR[mode]) == 0
     shapes = [(200, 130, 257), (129, 70, 37), (256, 384, 96), (64, 64, 64)]
     for i, (M, N, K) in enumerate(shapes):
         for tA, tB, pad in ((False, False, 0), (True, True, 0), (True, False, 0), (False, True, 1)):
@@ -173,8 +176,7 @@ def test_low_precision_kernel_load_modes(lib, oracle, mode):
             assert lib.attr_get_int(ex, tc.ATTR_LAST_KERNEL)[1] == tc.KERNEL_MFMA
             if mode.startswith("split"):
                 _, exact = oracle.run(c, f64=True)
-                tol = 1e-6 if mode == "split6_bf16" else 1e-5
-                assert T.normwise_error(got, exact, T.written_mask(c.D)) < tol, (M, N, K, tA, tB, pad)
+                assert T.normwise_error(got, exact, T.written_mask(c.D)) < SPLIT_TOL[mode], (M, N, K, tA, tB, pad)
             else:
                 _, want = oracle.run(c)
                 assert_parity(got, want, c, tol=1e-5)
@@ -209,6 +211,93 @@ def test_split6_bf16_below_fp32_error_at_long_k(lib, oracle):
     lib.destroy_executor(ex32)
 
 
+def _with_split(lib, value):
+    _, ex = lib.create_executor()
+    assert lib.attr_set_int(ex, tc.ATTR_SPLIT_BF16, value) == 0
+    return ex
+
+
+def test_split_f16_below_1e6_at_long_k(lib, oracle):
+    """Split-F16 (x = hi + lo in binary16, three products, compensated sums
+    of the FP32 K-tile sums) against FP64 at K = 16384: below 1e-6 and below
+    the plain FP32 kernel's own error."""
+    ex16, ex32 = _with_split(lib, 3), _with_split(lib, 0)
+    assert lib.attr_get_int(ex16, tc.ATTR_SPLIT_BF16) == (0, 3)
+    c = _gemm_case(tc.F32, 160, 136, 16384, 930, beta=0.0)
+    _, exact = oracle.run(c, f64=True)
+    mask = T.written_mask(c.D)
+    rc, got16 = T.run_lib(lib, c, ex16)
+    assert rc == 0 and lib.attr_get_int(ex16, tc.ATTR_LAST_KERNEL)[1] == tc.KERNEL_MFMA
+    rc, got32 = T.run_lib(lib, c, ex32)
+    err16, err32 = T.normwise_error(got16, exact, mask), T.normwise_error(got32, exact, mask)
+    assert err16 < 1e-6 and err16 < err32, (err16, err32)
+    lib.destroy_executor(ex16)
+    lib.destroy_executor(ex32)
+
+
+@pytest.mark.parametrize("sa,sb", [(1e6, 1.0), (1.0, 1e-7), (3e9, 2e-9), (1e-12, 1e-9), (4e4, 3.0)])
+def test_split_f16_rescales_out_of_range_operands(lib, oracle, sa, sb):
+    """Operands whose max |x| leaves [2^-3, 2^15) (binary16 overflow above,
+    subnormal pieces below) are split after a power-of-two scaling: the
+    error against FP64 stays below 1e-6 at every magnitude."""
+    ex = _with_split(lib, 3)
+    c = _gemm_case(tc.F32, 136, 72, 2048, 940, beta=0.0)
+    c.A.buf *= np.float32(sa)
+    c.B.buf *= np.float32(sb)
+    c.alpha = 1.0
+    _, exact = oracle.run(c, f64=True)
+    rc, got = T.run_lib(lib, c, ex)
+    assert rc == 0
+    assert np.isfinite(got).all()
+    assert T.normwise_error(got, exact, T.written_mask(c.D)) < 1e-6, (sa, sb)
+    lib.destroy_executor(ex)
+
+
+def test_split_f16_falls_back_bitwise(lib):
+    """Where split-F16 does not apply, D is bit-identical to the form that
+    runs instead: plain FP32 for inf / NaN inputs, for magnitudes below
+    2^-100 and for products small enough for the one-lane kernel; the
+    three-piece split-BF16 for D = A B + beta D in place (a second launch
+    would read the overwritten D)."""
+    ex16, ex32, ex6 = _with_split(lib, 3), _with_split(lib, 0), _with_split(lib, 2)
+
+    def same(case, other):
+        rc, a = T.run_lib(lib, case, ex16)
+        rc2, b = T.run_lib(lib, case, other)
+        assert rc == 0 and rc2 == 0
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), case.name
+
+    c = _gemm_case(tc.F32, 136, 72, 512, 950)
+    c.A.buf[c.A.offset + 5] = np.inf
+    same(c, ex32)
+    c = _gemm_case(tc.F32, 136, 72, 512, 951)
+    c.B.buf[c.B.offset + 7] = np.nan
+    same(c, ex32)
+    c = _gemm_case(tc.F32, 136, 72, 512, 952)
+    c.A.buf *= np.float32(1e-32)
+    same(c, ex32)
+    c = _gemm_case(tc.F32, 3, 700, 96, 953, beta=0.25)
+    same(c, ex32)
+    assert lib.attr_get_int(ex16, tc.ATTR_LAST_KERNEL)[1] == tc.KERNEL_SIMPLE
+
+    def in_place(case, ex):
+        D = case.D.copy()
+        infos = [lib.create_info(o.type, o.extents, o.strides)[1] for o in (case.A, case.B, D, D)]
+        _, plan = lib.create_plan([0, 0, 0, 0], infos, [case.A.idx, case.B.idx, D.idx, D.idx], case.prec)
+        al, be = T.scalar_bytes(case.alpha, tc.F32), T.scalar_bytes(case.beta, tc.F32)
+        rc = lib.execute(plan, ex, al.ctypes.data, case.A.ptr(), case.B.ptr(), be.ctypes.data, D.ptr(), D.ptr())
+        lib.destroy_plan(plan)
+        for h in infos:
+            lib.destroy_info(h)
+        assert rc == 0
+        return D.buf
+
+    c = _gemm_case(tc.F32, 136, 72, 512, 954, beta=0.5)
+    assert np.array_equal(in_place(c, ex16).view(np.uint32), in_place(c, ex6).view(np.uint32))
+    for ex in (ex16, ex32, ex6):
+        lib.destroy_executor(ex)
+
+
 @pytest.mark.parametrize("dtype,M", [(tc.F64, 512), (tc.C32, 512), (tc.C64, 256)])
 def test_large_tile_variants(lib, gpu_exec, oracle, dtype, M):
     """Batched products with at least 4 large-tile workgroups per CU (1024
@@ -227,17 +316,16 @@ def test_large_tile_variants(lib, gpu_exec, oracle, dtype, M):
     assert_parity(got, want, c)
 
 
-@pytest.mark.parametrize("mode", ["plain", "bf16", "f16", "split_bf16", "split6_bf16"])
+@pytest.mark.parametrize("mode", ["plain", "bf16", "f16", "split_bf16", "split6_bf16", "split_f16"])
 def test_split_k_few_tiles(lib, oracle, mode):
     """Few 128 x 128 output tiles and long K: the FP32 MFMA kernel splits K
     over workgroups and a second kernel sums the partials.  Ragged K (not a
     multiple of the 32-deep K tile), a Hadamard mode, beta != 0 and a
     transposed operand are covered."""
-    prec = {"plain": tc.DEFAULT_PREC, "bf16": tc.BF16BF16_ACCUM_F32, "f16": tc.F16F16_ACCUM_F32,
-            "split_bf16": tc.DEFAULT_PREC, "split6_bf16": tc.DEFAULT_PREC}[mode]
+    prec = {"bf16": tc.BF16BF16_ACCUM_F32, "f16": tc.F16F16_ACCUM_F32}.get(mode, tc.DEFAULT_PREC)
     _, ex = lib.create_executor()
     if mode.startswith("split"):
-        assert lib.attr_set_int(ex, tc.ATTR_SPLIT_BF16, 2 if mode == "split6_bf16" else 1) == 0
+        assert lib.attr_set_int(ex, tc.ATTR_SPLIT_BF16, SPLIT_ATTR[mode]) == 0
     for i, (M, N, K, L) in enumerate([(100, 90, 301, 3), (256, 256, 256, 1), (40, 200, 1000, 2)]):
         A = T.make_operand(tc.F32, [M, K, L], "mkl", 40 + i, order=[1, 0, 2] if i == 1 else None)
         B = T.make_operand(tc.F32, [K, N, L], "knl", 50 + i)
@@ -249,7 +337,7 @@ def test_split_k_few_tiles(lib, oracle, mode):
         assert lib.attr_get_int(ex, tc.ATTR_LAST_KERNEL)[1] == tc.KERNEL_MFMA
         if mode.startswith("split"):
             _, exact = oracle.run(c, f64=True)
-            assert T.normwise_error(got, exact, T.written_mask(c.D)) < (1e-6 if mode == "split6_bf16" else 1e-5)
+            assert T.normwise_error(got, exact, T.written_mask(c.D)) < SPLIT_TOL[mode]
         else:
             assert_parity(got, want, c)
     lib.destroy_executor(ex)
