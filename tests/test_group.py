@@ -161,3 +161,30 @@ def test_batched_product_device_pointers(lib, gpu_exec, oracle):
         assert_parity(d.d(), want, d.case)
     for d in sets:
         d.close()
+
+
+@pytest.mark.parametrize("split", [1, 2, 3])
+def test_group_with_split_forms(lib, oracle, split):
+    """An executor with a split form: the group runs those FP32 products one
+    by one inside the launch (the persistent kernel has no split form), so
+    each D equals a single call on the same executor bit for bit, and stays
+    within 1e-5 of the FP64 oracle."""
+    _, ex = lib.create_executor()
+    assert lib.attr_set_int(ex, tc.ATTR_SPLIT_BF16, split) == 0
+    cases = T.batching_mix(31, 8, tucker_range=(4, 16), r_range=(2, 8), c0_range=(3, 7))
+    cases.append(T.random_case(32, "contraction", tc.F32, max_ext=12, min_ext=8, layouts=True))
+    dcs = [DevCase(lib, c) for c in cases]
+    rc, g = make_group(lib, ex, dcs)
+    assert rc == 0
+    assert lib.execute_group(g) == 0
+    grouped = [d.d().copy() for d in dcs]
+    for d, got in zip(dcs, grouped):
+        d.reset_d()
+        assert d.run_single(ex) == 0
+        assert np.array_equal(got.view(np.uint32), d.d().view(np.uint32)), d.case.name
+        _, exact = oracle.run(d.case, f64=True)
+        assert T.normwise_error(got, exact, T.written_mask(d.case.D)) < 1e-5, d.case.name
+    lib.destroy_group(g)
+    for d in dcs:
+        d.close()
+    lib.destroy_executor(ex)

@@ -123,3 +123,42 @@ def test_budget_stalls_then_completes(lib, gpu_exec, oracle):
         assert_parity(c2.D.buf, want, c2)
         _free(lib, plan, infos)
     lib.destroy_server(srv)
+
+
+@pytest.mark.gpu
+def test_split_f16_requests_run_the_three_piece_split(lib, gpu_exec, oracle):
+    """The serving loop runs requests asynchronously, where split-F16 cannot
+    check its range and run again: an MFMA product takes the three-piece
+    split-BF16, a product for the one-lane kernel plain FP32.  Each D equals
+    a single call in that form, bit for bit."""
+    from test_group import DevCase
+    cases = [T.random_case(60 + i, "contraction", tc.F32, max_ext=12, min_ext=8, layouts=True) for i in range(3)]
+    cases += [T.random_case(70 + i, "contraction", tc.F32, max_ext=3) for i in range(3)]
+    dcs = [DevCase(lib, c) for c in cases]
+    exs = {}
+    for v in (0, 2, 3):
+        _, exs[v] = lib.create_executor()
+        assert lib.attr_set_int(exs[v], tc.ATTR_SPLIT_BF16, v) == 0
+    want = []
+    for d in dcs:
+        assert d.run_single(exs[3]) == 0
+        mfma = lib.attr_get_int(exs[3], tc.ATTR_LAST_KERNEL)[1] == tc.KERNEL_MFMA
+        d.reset_d()
+        assert d.run_single(exs[2 if mfma else 0]) == 0
+        want.append(d.d().copy())
+        d.reset_d()
+    assert any(lib.attr_get_int(exs[3], tc.ATTR_LAST_KERNEL)[1] == k for k in (tc.KERNEL_SIMPLE, tc.KERNEL_MFMA))
+    rc, srv = lib.create_server(exs[3], 1 << 30, 4, 16)
+    assert rc == 0
+    failed, rcs, late = tc.trace_replay(lib, srv, [d.plan for d in dcs], [d.al.ctypes.data for d in dcs],
+                                        [d.ptr["A"] for d in dcs], [d.ptr["B"] for d in dcs],
+                                        [d.be.ctypes.data for d in dcs], [d.c_ptr() for d in dcs],
+                                        [d.ptr["D"] for d in dcs], [0.0] * len(dcs))
+    assert failed == 0 and all(r == 0 for r in rcs), rcs
+    for d, w in zip(dcs, want):
+        assert np.array_equal(d.d().view(np.uint32), w.view(np.uint32)), d.case.name
+    lib.destroy_server(srv)
+    for d in dcs:
+        d.close()
+    for ex in exs.values():
+        lib.destroy_executor(ex)
