@@ -648,13 +648,19 @@ __global__ __launch_bounds__(256, 2) void k_gett_f32(tk_gett g)
             };
             // the other LDS buffer has been free since the last barrier: the
             // next tile goes there half-way through this tile's MFMAs, so
-            // the stores overlap the MFMA pipe instead of following it
+            // the stores overlap the MFMA pipe instead of following it.  The
+            // MFMA stretches run at raised wave priority, so the SIMD's other
+            // wave does its loads and stores in their shadow.
+            __builtin_amdgcn_s_setprio(1);
             mfmas(0, BK / 2);
+            __builtin_amdgcn_s_setprio(0);
             if (kt + 1 < nk) {
                 la.template store<LDA>(As[cur ^ 1]);
                 lb.template store<LDB>(Bs[cur ^ 1]);
             }
+            __builtin_amdgcn_s_setprio(1);
             mfmas(BK / 2, BK);
+            __builtin_amdgcn_s_setprio(0);
             __syncthreads();
         }
 
@@ -961,7 +967,9 @@ __global__ __launch_bounds__(256, (lp_waves<AL, BL, MODE>())) void k_gett_lp(tk_
         for (int64_t kt = 0; kt < nk; ++kt) {
             const int cur = kt & 1;
             if (kt + 1 < nk) fetch(la0, lb0, kt + 1);   // next tile's loads in flight during the MFMAs
+            __builtin_amdgcn_s_setprio(1);               // MFMAs ahead of the other wave's conversions
             compute(cur);
+            __builtin_amdgcn_s_setprio(0);
             if (kt + 1 < nk) stash(la0, lb0, cur ^ 1);
             if (TWO_LEVEL && ((kt + 1) % FOLD == 0 || kt + 1 == nk)) fold();
             __syncthreads();
