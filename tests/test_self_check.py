@@ -217,3 +217,17 @@ def test_device_operands(lib, gpu_exec, capfd):
         lib.destroy_plan(plan)
         for h in infos:
             lib.destroy_info(h)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dtype", [tc.F16, tc.BF16])
+def test_sixteen_bit_storage(lib, gpu_exec, capfd, dtype):
+    """D stored as binary16 or bfloat16, compared as binary32 values."""
+    for seed in range(3):
+        case = T.random_case(960 + seed, "contraction", tc.F32, max_ext=8, min_ext=4, beta_zero=False)
+        for o in case.operands():
+            lo, hi = o.span()
+            o.type = dtype
+            o.buf = T.fill(dtype, max(1, hi - lo + 1), 970 + seed)
+        case.prec = tc.F32F32_ACCUM_F32
+        _assert_outcome(lib, case, capfd)
