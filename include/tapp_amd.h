@@ -44,8 +44,8 @@ enum {
      *     scaled by a power of two, or on the FP32 kernel for NaN, inf and
      *     magnitudes below 2^-100.  A call that cannot rerun (C overlapping
      *     D with beta != 0, requests of the serving loop) runs value 2.
-     * Products small enough for the one-lane-per-output kernel run it in
-     * FP32 with value 3.  Opt-in; default 0. */
+     * Products small enough for the one-lane-per-output kernel, and complex
+     * products, run in FP32 with value 3.  Opt-in; default 0. */
     TAPP_AMD_ATTR_SPLIT_BF16 = 0x1001,
     /* Force a kernel family: 0 auto, 1 one-lane-per-output FMA, 2 MFMA,
      * 3 split-K.  Default 0. */

@@ -516,14 +516,14 @@ int tk_execute_on(struct tapp_plan* p, struct tapp_exec* ex, const void* alpha, 
      * kernel adds in a fixed order */
     const bool may_tune = ex->autotune && ex->force_kernel == TK_KERNEL_AUTO && K > 0 && P.mode != TK_MODE_SPLIT_F16;
 
-    /* split-F16 runs on the MFMA GETT only: the other kernels compute in
-     * FP32, which is as accurate.  Its range check may ask for a second
-     * launch, which needs a synchronous call whose C is still intact after
-     * the first (tune_safe); otherwise the three-piece split-BF16 gives the
-     * same accuracy in one launch. */
+    /* split-F16 runs on the real FP32 MFMA GETT only: the other kernels,
+     * and complex products, compute in FP32, which is as accurate.  Its
+     * range check may ask for a second launch, which needs a synchronous
+     * call whose C is still intact after the first (tune_safe); otherwise
+     * the three-piece split-BF16 gives the same accuracy in one launch. */
     int mode = P.mode;
     if (mode == TK_MODE_SPLIT_F16) {
-        if (kern != TK_KERNEL_MFMA) mode = TK_MODE_PLAIN;
+        if (kern != TK_KERNEL_MFMA || ct != TK_F32) mode = TK_MODE_PLAIN;
         else if (!sync || !tune_safe(&P, src, needC)) mode = TK_MODE_SPLIT6_BF16;
     }
     int64_t ks = 1, kchunk = 0;

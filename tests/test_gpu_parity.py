@@ -279,6 +279,11 @@ def test_split_f16_falls_back_bitwise(lib):
     c = _gemm_case(tc.F32, 3, 700, 96, 953, beta=0.25)
     same(c, ex32)
     assert lib.attr_get_int(ex16, tc.ATTR_LAST_KERNEL)[1] == tc.KERNEL_SIMPLE
+    c = _gemm_case(tc.C32, 96, 80, 64, 955, conj=(1, 0))   # complex: the C32 MFMA kernel in FP32
+    rc, a = T.run_lib(lib, c, ex16)
+    rc2, b = T.run_lib(lib, c, ex32)
+    assert rc == 0 and rc2 == 0 and np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    assert lib.attr_get_int(ex16, tc.ATTR_LAST_KERNEL)[1] == tc.KERNEL_MFMA
 
     def in_place(case, ex):
         D = case.D.copy()
