@@ -451,6 +451,7 @@ def config_cases(lib, torch, steps):
     padded strides, TAPP_F64F64_ACCUM_F64) on device operands."""
     L = lambda t: [ord(c) for c in t]  # noqa: E731
     out = {}
+    operands, specs_by_name = {}, {}
 
     def span(ext, st):
         lo = sum(min(0, (e - 1) * s) for e, s in zip(ext, st))
@@ -458,6 +459,7 @@ def config_cases(lib, torch, steps):
         return -lo, hi - lo + 1
 
     def run(name, specs, prec, alpha, beta, ctype, flop, bytes_):
+        specs_by_name[name] = (specs, prec, alpha, beta, ctype, flop)
         dt = torch.float32 if ctype == tc.F32 else torch.complex128
         infos, bufs, ptrs = [], [], []
         for ext, st, lab, dtype in specs:
@@ -488,6 +490,8 @@ def config_cases(lib, torch, steps):
             dev.append(lib.attr_get_double(ex, tc.ATTR_LAST_DEVICE_US)[1])
         wall = (time.perf_counter() - t0) / steps
         d_us = float(np.mean(dev))
+        torch.cuda.synchronize()
+        operands[name] = [b.cpu() for b in bufs]     # D holds the GPU result
         lib.destroy_executor(ex)
         lib.destroy_plan(plan)
         for h in infos:
@@ -513,6 +517,42 @@ def config_cases(lib, torch, steps):
                      ([a, b, h, i], sD, "abhi", tc.C64), ([a, b, h, i], sD, "abhi", tc.C64)],
         tc.F64F64_ACCUM_F64, 0.7 - 0.3j, 1.1 + 0.2j, tc.C64,
         8.0 * h * a * (b * i) * c, 16.0 * (a * c * h * u + h * b * c * i * v + 2 * a * b * h * i))
+    return out, (operands, specs_by_name)
+
+
+def cpu_reference_configs(cases):
+    """configs[0] and configs[1] run whole on one host core through the
+    compiled reference (oracle/_ref), on the operands the GPU used; the GPU's
+    D against the reference's."""
+    path = os.path.join(ROOT, "oracle", "_ref", "libtapp-reference.so")
+    if not os.path.exists(path):
+        return None
+    operands, specs_by_name = cases
+    ref = tc.TappLib(path)
+    L = lambda t: [ord(c) for c in t]  # noqa: E731
+    out = {}
+    for name, (specs, prec, alpha, beta, ctype, flop) in specs_by_name.items():
+        host = [t.numpy().copy() for t in operands[name]]
+        gpu_d = host[3].copy()
+        infos = [ref.create_info(dtype, ext, st)[1] for ext, st, lab, dtype in specs]
+        _, plan = ref.create_plan([0, 1 if name == "configs1" else 0, 0, 0], infos, [L(x[2]) for x in specs], prec)
+        npd = np.complex128 if ctype == tc.C64 else np.float32
+        al, be = np.array([alpha], dtype=npd), np.array([beta], dtype=npd)
+        offs = []
+        for (ext, st, lab, dtype), h in zip(specs, host):
+            lo = sum(min(0, (e - 1) * x) for e, x in zip(ext, st))
+            offs.append(h.ctypes.data - lo * h.itemsize)
+        _, ex = ref.create_executor()
+        t0 = time.perf_counter()
+        rc = ref.execute(plan, ex, al.ctypes.data, offs[0], offs[1], be.ctypes.data, offs[2], offs[3])
+        dt = time.perf_counter() - t0
+        assert rc == 0, rc
+        err = float(np.abs(gpu_d - host[3]).max() / np.abs(host[3]).max())
+        out[name] = {"one_core_s": dt, "one_core_gflops": flop / dt / 1e9, "gpu_vs_reference_normwise_err": err}
+        ref.destroy_executor(ex)
+        ref.destroy_plan(plan)
+        for hnd in infos:
+            ref.destroy_info(hnd)
     return out
 
 
@@ -587,8 +627,19 @@ def main():
         log("bench: FP64 reference result")
         D64 = fp64_reference(lib, torch, A, B)
         den = D64.abs().max().item()
+        keep = D64.abs() >= 1e-3 * den        # elementwise errors where |D| is not tiny
+
+        def elementwise(Dx):
+            e = ((Dx.double() - D64).abs() / D64.abs())[keep]
+            return {"median": e.median().item(), "max": e.max().item(), "floor": "|D| >= 1e-3 max|D|"}
+
         forms["fp32"] = {"gflops": value, "kernel_us": dev_us,
-                         "max_rel_err_vs_fp64": (D_fp32.double() - D64).abs().max().item() / den}
+                         "max_rel_err_vs_fp64": (D_fp32.double() - D64).abs().max().item() / den,
+                         "elementwise_rel_err_vs_fp64": elementwise(D_fp32),
+                         "roofline_frac": flops / (dev_us * 1e-6) / (PEAK_FP32_TFLOPS * 1e12),
+                         "peak_unit_tflops": PEAK_FP32_TFLOPS}
+        peak_unit = {"split_bf16x3": PEAK_BF16_TFLOPS / 3, "split_bf16x6": PEAK_BF16_TFLOPS / 6,
+                     "split_f16x3": PEAK_BF16_TFLOPS / 3, "bf16": PEAK_BF16_TFLOPS, "f16": PEAK_BF16_TFLOPS}
         for name, prec, split in (("split_bf16x3", tc.DEFAULT_PREC, 1), ("split_bf16x6", tc.DEFAULT_PREC, 2),
                                   ("split_f16x3", tc.DEFAULT_PREC, 3),
                                   ("bf16", tc.BF16BF16_ACCUM_F32, 0), ("f16", tc.F16F16_ACCUM_F32, 0)):
@@ -596,6 +647,9 @@ def main():
             w, du, k = time_form(lib, torch, prec, split, A, B, D, max(1, args.steps // 4), 1)
             err = (D.double() - D64).abs().max().item() / den
             forms[name] = {"gflops": flops / w / 1e9, "kernel_us": du, "kernel": k, "max_rel_err_vs_fp64": err,
+                           "elementwise_rel_err_vs_fp64": elementwise(D),
+                           "roofline_frac": flops / (du * 1e-6) / (peak_unit[name] * 1e12),
+                           "peak_unit_tflops": peak_unit[name],
                            "speedup_vs_fp32": (flops / w / 1e9) / value}
         del D64
 
@@ -605,7 +659,11 @@ def main():
         cpu = cpu_baseline(A, B, D_fp32)
 
     log("bench: configs[0] and configs[1] per call")
-    others = config_cases(lib, torch, max(5, args.steps))
+    others, config_operands = config_cases(lib, torch, max(5, args.steps))
+    if cpu is not None:
+        log("bench: configs[0] and configs[1] on one host core through the reference")
+        cpu["configs"] = cpu_reference_configs(config_operands)
+    del config_operands
 
     batched = {}
     if not args.no_batch:
