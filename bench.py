@@ -185,6 +185,78 @@ def batching_problems(seed=0, count=8192):
     return out
 
 
+def cpu_reference_batch(sample=16):
+    """The compiled reference (oracle/_ref) on one host core for the batching
+    and serving shape classes: the first `sample` Tucker and configs[0]-form
+    products of the batching draw, timed one by one and extrapolated per
+    class to the 8192-product batch; and one virtual-extent-64 ladder-form
+    serving request on a D-slice (i = j = 0, 1/1024 of D) at its full K,
+    extrapolated linearly."""
+    path = os.path.join(ROOT, "oracle", "_ref", "libtapp-reference.so")
+    if not os.path.exists(path):
+        return None
+    ref = tc.TappLib(path)
+    _, ex = ref.create_executor()
+    rng = np.random.default_rng(11)
+    L = lambda t: [ord(c) for c in t]  # noqa: E731
+
+    def dense(ext):
+        st, acc = [], 1
+        for x in ext:
+            st.append(acc)
+            acc *= x
+        return st, acc
+
+    def run(ops, alpha, beta, strides=None):
+        infos, bufs = [], []
+        for n, (ext, lab) in enumerate(ops):
+            st, size = dense(ext)
+            if strides and strides[n]:
+                st, size = strides[n]
+            infos.append(ref.create_info(tc.F32, ext, st)[1])
+            bufs.append((rng.random(size, dtype=np.float32) * 2 - 1).astype(np.float32))
+        _, plan = ref.create_plan([0, 0, 0, 0], infos, [L(x[1]) for x in ops], tc.DEFAULT_PREC)
+        al, be = np.array([alpha], np.float32), np.array([beta], np.float32)
+        t0 = time.perf_counter()
+        rc = ref.execute(plan, ex, al.ctypes.data, bufs[0].ctypes.data, bufs[1].ctypes.data, be.ctypes.data,
+                         bufs[2].ctypes.data, bufs[3].ctypes.data)
+        dt = time.perf_counter() - t0
+        assert rc == 0, rc
+        ref.destroy_plan(plan)
+        for h in infos:
+            ref.destroy_info(h)
+        return dt
+
+    probs = batching_problems()
+    counts = {"tucker": 0, "c0": 0}
+    times = {"tucker": [], "c0": []}
+    for kind, e in probs:
+        counts[kind] += 1
+        if len(times[kind]) >= sample:
+            continue
+        if kind == "tucker":
+            i, j, k, r = e
+            times[kind].append(run([([i, j, k], "ijk"), ([r, i], "ri"), ([r, j, k], "rjk"), ([r, j, k], "rjk")],
+                                   1.0, 0.0))
+        else:
+            a, b, c, d, ee, ff = e
+            times[kind].append(run([([a, ee, b, ff], "aebf"), ([d, ff, c, ee], "dfce"), ([a, b, c, d], "abcd"),
+                                    ([a, b, c, d], "abcd")], 1.5, 0.5))
+    mean = {k: float(np.mean(v)) for k, v in times.items()}
+    batch_s = sum(counts[k] * mean[k] for k in counts)
+    # one ladder-form serving request at virtual extent 64, on the D-slice i = j = 0
+    v, o = 64, 32
+    slice_s = run([([v, v, v, v], "abcd"), ([v, 1, v, 1], "cidj"), ([v, v, 1, 1], "abij"), ([v, v, 1, 1], "abij")],
+                  1.0, 0.0)
+    ref.destroy_executor(ex)
+    return {"batch_8192_s_extrapolated": batch_s,
+            "batch_sample": f"first {sample} Tucker and {sample} configs[0]-form products of the seed-0 draw, "
+                            f"one core; extrapolated per class to {counts['tucker']} + {counts['c0']}",
+            "tucker_ms_mean": 1e3 * mean["tucker"], "configs0_form_ms_mean": 1e3 * mean["c0"],
+            "serving_ladder64_request_s_extrapolated": slice_s * (o * o),
+            "serving_ladder64_sample": "D-slice i = j = 0 (1/1024 of D) at full K = 4096, one core"}
+
+
 def batching_bench(lib, torch, steps, prec=tc.DEFAULT_PREC):
     """one persistent group launch vs one TAPP_execute_product per
     product, same device operands, same GPU."""
@@ -663,6 +735,8 @@ def main():
     if cpu is not None:
         log("bench: configs[0] and configs[1] on one host core through the reference")
         cpu["configs"] = cpu_reference_configs(config_operands)
+        log("bench: batching and serving shape classes on one host core through the reference (sampled)")
+        cpu["classes"] = cpu_reference_batch()
     del config_operands
 
     batched = {}
