@@ -8,8 +8,8 @@
 // sorted by cost on the host, so long tiles start first; product groups keep
 // one queue per XCD (a product's tiles on one XCD's L2).
 //
-//  thin tile  (S <= 16): 256 columns g, all s.  The S x 32 chunk of X and the
-//             32 x 256 chunk of Y go through LDS (Y loaded along whichever of
+//  thin tile  (S <= 16): 256 columns g, all s.  The S x 16 chunk of X and the
+//             16 x 256 chunk of Y go through LDS (Y loaded along whichever of
 //             k or g has unit stride, so global reads are coalesced); each lane
 //             then owns one column g and accumulates S outputs with FMA.  This
 //             is the HBM-streaming shape of Tucker mode products.
@@ -24,11 +24,14 @@ namespace tk {
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 constexpr int GT = 256;                   // thin tile columns
-constexpr int KC = 32;                    // thin K chunk
+constexpr int KC = 16;                    // thin K chunk (small LDS: six workgroups per CU)
 constexpr int YP = GT + 1;                // LDS pitch of the Y chunk
 constexpr int SMAX = 16;
 constexpr int MT = 64, MK = 32, MP = MT + 4;
+constexpr int NBC = MT / 16;              // B columns per lane when B is read along K
 constexpr int KO = 256;                   // K offsets staged in LDS per chunk (mfma tile)
+// the mfma tile's A and B steps share the thin tile's LDS array
+static_assert(2 * MK * MP <= KC * YP + SMAX * KC, "mfma tile staging must fit the shared LDS array");
 
 // Pointers read from a descriptor in memory are generic to the compiler, and
 // generic (FLAT) loads wait conservatively; the operands and tables all live
@@ -157,21 +160,27 @@ __device__ __forceinline__ void mfma_tile(const GView& p, int64_t l, int64_t m0,
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave & 1, wn = wave >> 1, il = lane & 31, kl = lane >> 5;
     const int64_t lx = p.lX[l], ly = p.lY[l];
-    // fixed rows of this lane: A element (tid % 64, tid / 64 + 4 t); B element
-    // (n, k) chosen so that the unit-stride index runs across lanes
+    // fixed rows of this lane: A element (tid % 64, tid / 64 + 4 t).  B
+    // element (n, k) with the unit-stride index across lanes: along n, lane
+    // column tid % 64 and k = tid / 64 + 4 t; along k, 16 lanes take 16
+    // consecutive k of a column, so a lane touches NBC columns
+    // (n = tid / 16 + 16 c) at k = tid % 16 + 16 j.
     const int am = tid % MT;
     const bool arow_ok = m0 + am < p.S;
     const int64_t arow = arow_ok ? lx + p.sX[m0 + am] : 0;
     const bool bk = p.y_kfast != 0;
-    const int bn = bk ? tid / MK : tid % MT;
-    int64_t bcol[(MT * MK) / 256];
-    bool bok[(MT * MK) / 256];          // offsets may be negative: validity kept apart
+    int64_t bcol[NBC];
+    bool bok[NBC];                      // offsets may be negative: validity kept apart
 #pragma unroll
-    for (int t = 0; t < (MT * MK) / 256; ++t) {
-        const int n = bk ? bn + (256 / MK) * t : bn;
-        bok[t] = n0 + n < p.G;
-        bcol[t] = bok[t] ? ly + p.gY[n0 + n] : 0;
+    for (int c = 0; c < NBC; ++c) {
+        const int n = bk ? tid / 16 + 16 * c : tid % MT;
+        bok[c] = n0 + n < p.G;
+        bcol[c] = bok[c] ? ly + p.gY[n0 + n] : 0;
     }
+    // B element t of this lane: column slot, n and k
+    auto bslot = [&](int t) { return bk ? t % NBC : 0; };
+    auto bn_of = [&](int t) { return bk ? tid / 16 + 16 * (t % NBC) : tid % MT; };
+    auto bk_of = [&](int t) { return bk ? tid % 16 + 16 * (t / NBC) : tid / MT + (256 / MT) * t; };
     float ra[(MT * MK) / 256], rb[(MT * MK) / 256];
     // loads of one K step: offsets of K come from LDS (staged per KO-chunk)
     auto fetch = [&](int64_t k0) {
@@ -183,8 +192,8 @@ __device__ __forceinline__ void mfma_tile(const GView& p, int64_t l, int64_t m0,
         }
 #pragma unroll
         for (int t = 0; t < (MT * MK) / 256; ++t) {
-            const int k = bk ? tid % MK : tid / MT + (256 / MT) * t;
-            rb[t] = (bok[t] && k0 + k < p.K) ? rnd_in(p.Y[bcol[t] + koy[kb + k]], p.round) : 0.0f;
+            const int k = bk_of(t), c = bslot(t);
+            rb[t] = (bok[c] && k0 + k < p.K) ? rnd_in(p.Y[bcol[c] + koy[kb + k]], p.round) : 0.0f;
         }
     };
     auto stage = [&](int64_t k0) {      // K offsets for k0 .. k0 + KO
@@ -206,11 +215,7 @@ __device__ __forceinline__ void mfma_tile(const GView& p, int64_t l, int64_t m0,
 #pragma unroll
         for (int t = 0; t < (MT * MK) / 256; ++t) As[(tid / MT + (256 / MT) * t) * MP + am] = ra[t];
 #pragma unroll
-        for (int t = 0; t < (MT * MK) / 256; ++t) {
-            const int k = bk ? tid % MK : tid / MT + (256 / MT) * t;
-            const int n = bk ? bn + (256 / MK) * t : bn;
-            Bs[k * MP + n] = rb[t];
-        }
+        for (int t = 0; t < (MT * MK) / 256; ++t) Bs[bk_of(t) * MP + bn_of(t)] = rb[t];
         __syncthreads();
         const int64_t kn = k0 + MK;
         if (kn < p.K) {
@@ -250,7 +255,7 @@ __device__ __forceinline__ void mfma_tile(const GView& p, int64_t l, int64_t m0,
 // XCDs round-robin, so a product's tiles share that XCD's L2 -- and then
 // takes what is left in the others, in order, so every workgroup ends after
 // every queue is empty.
-__global__ __launch_bounds__(256, 3) void k_group(const tk_prob* __restrict__ probs, const int64_t* __restrict__ tile_end,
+__global__ __launch_bounds__(256, TK_GROUP_WG_PER_CU) void k_group(const tk_prob* __restrict__ probs, const int64_t* __restrict__ tile_end,
                                                const int* __restrict__ tile_prob, int nprobs, int64_t total,
                                                const int64_t* __restrict__ qbeg, int nq,
                                                unsigned long long* counter)

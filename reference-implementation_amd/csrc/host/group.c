@@ -162,7 +162,7 @@ static int launch_group_ctx(void* c)
  * given; the caller sorts them as the group will run them). */
 static double time_descriptors(struct tapp_exec* ex, const tk_prob* h, int n)
 {
-    const int64_t nb = 3 * (int64_t)ex->cus;
+    const int64_t nb = TK_GROUP_WG_PER_CU * (int64_t)ex->cus;
     int64_t* he = malloc(sizeof(int64_t) * (size_t)n);
     if (!he) return -1.0;
     int64_t total = 0;
@@ -423,7 +423,7 @@ static int create_group(TAPP_AMD_product_group* group, TAPP_executor exec, int c
         int cus = 0;
         if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ex->device) != hipSuccess || cus <= 0)
             cus = 256;
-        int64_t nb = (int64_t)cus * 3;
+        int64_t nb = (int64_t)cus * TK_GROUP_WG_PER_CU;
         g->nblocks = (int)(total < nb ? total : nb);
         /* capture zero-counter + persistent kernel once; replays reuse it */
         if (capture && hipStreamBeginCapture(ex->stream, hipStreamCaptureModeThreadLocal) == hipSuccess) {

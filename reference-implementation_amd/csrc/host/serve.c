@@ -169,7 +169,7 @@ static void launch_batch(struct tapp_server* sv, srv_slot* s)
             (unsigned long long*)((char*)s->dmem + ((sizeof(tk_prob) + sizeof(int64_t)) * s->cap + 255) / 256 * 256);
         int e = (int)hipMemcpyAsync(dprobs, s->hprobs, sizeof(tk_prob) * (size_t)n, hipMemcpyHostToDevice, ex->stream);
         if (!e) e = (int)hipMemcpyAsync(dend, s->hend, sizeof(int64_t) * (size_t)n, hipMemcpyHostToDevice, ex->stream);
-        int64_t nb = (int64_t)sv->cus * 3;
+        int64_t nb = (int64_t)sv->cus * TK_GROUP_WG_PER_CU;
         if (!e) e = tk_launch_group(dprobs, dend, NULL, n, total, counter, (int)(total < nb ? total : nb), ex->stream);
         if (e)
             for (srv_request* r = s->batch; r; r = r->next)

@@ -119,6 +119,12 @@ typedef struct tk_scatter {
  * group's problem order. */
 enum { TK_GROUP_THIN = 0, TK_GROUP_MFMA = 1 };
 
+/* Persistent group workgroups per CU: the group kernel's occupancy bound
+ * (launch bounds) and the grid the host launches.  Six 256-lane workgroups
+ * (24 KB of LDS each) give each CU six independent tiles in flight, which
+ * is what hides the tiles' dependent table and data round trips. */
+#define TK_GROUP_WG_PER_CU 6
+
 typedef struct tk_prob {
     int64_t S, G, K, L;
     int64_t tiles_s, tiles_g;
