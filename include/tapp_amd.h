@@ -127,7 +127,10 @@ TAPP_EXPORT TAPP_error TAPP_AMD_group_info(TAPP_AMD_product_group group, int64_t
  * streams.  Each request returns a TAPP_status (status.h:9-17) that
  * TAPP_AMD_status_wait waits on and TAPP_destroy_status releases.  Operands
  * must stay valid, and plans unchanged, until the request completes; the
- * plans of queued requests must not be executed elsewhere concurrently. */
+ * plans of queued requests must not be executed elsewhere concurrently.
+ * Requests in flight together run in no particular order (a batch's
+ * persistent launch first, its requests completing when it ends): submit a
+ * request that reads another's D only after that one has completed. */
 typedef intptr_t TAPP_AMD_server;
 
 enum { TAPP_AMD_SERVER_PAUSED = 1 };   /* create paused: queue and admission only */

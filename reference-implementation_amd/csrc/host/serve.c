@@ -14,8 +14,10 @@
  * one persistent group launch -- a work queue over all their tiles, mixed
  * shape classes in one kernel -- and every other request (large GETTs,
  * complex, FP64, split-BF16, unary labels, host operands) runs through the
- * TAPP_execute_product path on the same stream without waiting.  Completion is
- * tracked with one HIP event per batch.
+ * TAPP_execute_product path on the same stream without waiting.  The
+ * persistent launch goes first and has its own HIP event, so its (small)
+ * requests complete when it ends, not behind the batch's large products;
+ * a second event marks the end of the batch.
  *
  * Footprint of a request = bytes of its host operands (staged into HBM) plus
  * the pre-pass buffers it needs; device operands and resident tensors cost
@@ -49,6 +51,7 @@ typedef struct srv_request {
     void* D;
     size_t footprint;
     double t_submit, t_start, t_done;
+    int in_group;               /* in the batch: 1 persistent launch, 0 general path, -1 nothing to launch */
     struct srv_request* next;   /* queue / batch list */
 } srv_request;
 
@@ -57,7 +60,9 @@ typedef struct {
     bool busy;
     srv_request* batch;         /* requests of the batch in flight */
     size_t batch_bytes;
-    hipEvent_t done;
+    hipEvent_t done;            /* end of the batch */
+    hipEvent_t done_group;      /* end of the batch's persistent launch */
+    bool group_pending;         /* its requests not yet completed */
     /* descriptor staging for the persistent launch */
     tk_prob* hprobs;
     int64_t* hend;
@@ -135,14 +140,17 @@ static void launch_batch(struct tapp_server* sv, srv_slot* s)
     int n = 0;
     int64_t total = 0;
     size_t count = 0;
+    s->group_pending = false;
     for (srv_request* r = s->batch; r; r = r->next) count++;
     if (ensure_desc(s, count)) {
-        for (srv_request* r = s->batch; r; r = r->next) r->rc = TAPP_AMD_ERR_NOMEM;
+        for (srv_request* r = s->batch; r; r = r->next) { r->rc = TAPP_AMD_ERR_NOMEM; r->in_group = -1; }
         hipEventRecord(s->done, ex->stream);
         return;
     }
+    /* which requests join the persistent launch */
     for (srv_request* r = s->batch; r; r = r->next) {
         r->t_start = now_us();
+        r->in_group = -1;
         tk_problem P;
         int rc = tk_plan_problem(r->plan, ex, r->alpha, r->beta, r->C, &P);
         if (rc) { r->rc = rc; continue; }
@@ -156,12 +164,14 @@ static void launch_batch(struct tapp_server* sv, srv_slot* s)
             s->hend[n] = total;
             n++;
             r->rc = 0;
+            r->in_group = 1;
         } else if (rc) {
             r->rc = rc;
         } else {
-            r->rc = tk_execute_on(r->plan, ex, r->alpha, r->A, r->B, r->beta, r->C, r->D, false);
+            r->in_group = 0;
         }
     }
+    /* the persistent launch first, with its own event */
     if (n > 0) {
         tk_prob* dprobs = (tk_prob*)s->dmem;
         int64_t* dend = (int64_t*)((char*)s->dmem + sizeof(tk_prob) * s->cap);
@@ -173,20 +183,28 @@ static void launch_batch(struct tapp_server* sv, srv_slot* s)
         if (!e) e = tk_launch_group(dprobs, dend, NULL, n, total, counter, (int)(total < nb ? total : nb), ex->stream);
         if (e)
             for (srv_request* r = s->batch; r; r = r->next)
-                if (!r->rc) r->rc = TAPP_AMD_ERR_DEVICE;
+                if (r->in_group == 1) r->rc = TAPP_AMD_ERR_DEVICE;
         sv->group_products += n;
+        if (hipEventRecord(s->done_group, ex->stream) == hipSuccess) s->group_pending = true;
     }
+    /* then the general path */
+    for (srv_request* r = s->batch; r; r = r->next)
+        if (r->in_group == 0) r->rc = tk_execute_on(r->plan, ex, r->alpha, r->A, r->B, r->beta, r->C, r->D, false);
     hipEventRecord(s->done, ex->stream);
 }
 
 /* --- dispatcher ----------------------------------------------------------- */
 
-static void finish_slot(struct tapp_server* sv, srv_slot* s)
+/* Completes the batch's requests that are done: with group_only, those of
+ * the persistent launch; otherwise all that remain, and the slot frees. */
+static void finish_requests(struct tapp_server* sv, srv_slot* s, bool group_only)
 {
     double t = now_us();
-    srv_request* r = s->batch;
-    while (r) {
-        srv_request* nx = r->next;
+    srv_request** link = &s->batch;
+    while (*link) {
+        srv_request* r = *link;
+        if (group_only && r->in_group != 1) { link = &r->next; continue; }
+        *link = r->next;
         r->t_done = t;
         r->state = 2;
         if (sv->nlat == sv->caplat) {
@@ -200,12 +218,13 @@ static void finish_slot(struct tapp_server* sv, srv_slot* s)
         sv->t_last = t;
         r->next = NULL;
         release(r);
-        r = nx;
     }
+    if (group_only) return;
     sv->in_use -= s->batch_bytes;
     s->batch = NULL;
     s->batch_bytes = 0;
     s->busy = false;
+    s->group_pending = false;
 }
 
 static void* dispatcher(void* arg)
@@ -218,8 +237,13 @@ static void* dispatcher(void* arg)
         /* completions */
         for (int i = 0; i < sv->nslots; i++) {
             srv_slot* s = &sv->slot[i];
+            if (s->busy && s->group_pending && hipEventQuery(s->done_group) == hipSuccess) {
+                finish_requests(sv, s, true);
+                s->group_pending = false;
+                progressed = true;
+            }
             if (s->busy && hipEventQuery(s->done) == hipSuccess) {
-                finish_slot(sv, s);
+                finish_requests(sv, s, false);
                 progressed = true;
             }
         }
@@ -326,6 +350,7 @@ TAPP_error TAPP_AMD_create_server(TAPP_AMD_server* server, TAPP_executor exec, s
             if (hipSetDevice(base->device) != hipSuccess ||
                 hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess ||
                 hipEventCreateWithFlags(&sv->slot[i].done, hipEventDisableTiming) != hipSuccess ||
+                hipEventCreateWithFlags(&sv->slot[i].done_group, hipEventDisableTiming) != hipSuccess ||
                 hipEventCreate(&e->ev0) != hipSuccess || hipEventCreate(&e->ev1) != hipSuccess) {
                 (void)hipGetLastError();
                 TAPP_AMD_destroy_server((TAPP_AMD_server)sv);
@@ -555,6 +580,7 @@ TAPP_error TAPP_AMD_destroy_server(TAPP_AMD_server server)
         if (s->dmem) hipFree(s->dmem);
         if (s->hprobs) { hipHostFree(s->hprobs); hipHostFree(s->hend); }
         if (s->done) hipEventDestroy(s->done);
+        if (s->done_group) hipEventDestroy(s->done_group);
         if (s->ex.ev0) hipEventDestroy(s->ex.ev0);
         if (s->ex.ev1) hipEventDestroy(s->ex.ev1);
         if (s->ex.own_stream && s->ex.stream) hipStreamDestroy(s->ex.stream);
