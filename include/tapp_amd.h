@@ -124,7 +124,9 @@ TAPP_EXPORT TAPP_error TAPP_AMD_group_info(TAPP_AMD_product_group group, int64_t
  * buffers; device operands cost nothing -- fits `hbm_budget_bytes`, launches
  * them as one batch (small FP32 products in one persistent launch, the rest
  * on the general path) and keeps admitting while batches run on other
- * streams.  Each request returns a TAPP_status (status.h:9-17) that
+ * streams.  A heavy request (above 2^31 flop, or staging more than 4 MiB)
+ * is passed over while every other stream holds one, so that one stream
+ * stays free for the small requests queued behind it.  Each request returns a TAPP_status (status.h:9-17) that
  * TAPP_AMD_status_wait waits on and TAPP_destroy_status releases.  Operands
  * must stay valid, and plans unchanged, until the request completes; the
  * plans of queued requests must not be executed elsewhere concurrently.

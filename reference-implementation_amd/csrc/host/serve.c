@@ -19,6 +19,12 @@
  * requests complete when it ends, not behind the batch's large products;
  * a second event marks the end of the batch.
  *
+ * Heavy requests -- above the persistent launch's flop limit, or staging
+ * more than heavy_bytes over PCIe -- hold a stream for a long time.  While
+ * every other slot holds one, admission passes over them (they keep their
+ * place in the queue) and takes the small requests behind them, so small
+ * requests never wait for a stream behind heavy ones.
+ *
  * Footprint of a request = bytes of its host operands (staged into HBM) plus
  * the pre-pass buffers it needs; device operands and resident tensors cost
  * nothing.  A request larger than the whole budget is rejected at submit
@@ -52,6 +58,7 @@ typedef struct srv_request {
     size_t footprint;
     double t_submit, t_start, t_done;
     int in_group;               /* in the batch: 1 persistent launch, 0 general path, -1 nothing to launch */
+    bool heavy;                 /* above the persistent launch's flop limit, or staging above heavy_bytes */
     struct srv_request* next;   /* queue / batch list */
 } srv_request;
 
@@ -63,6 +70,7 @@ typedef struct {
     hipEvent_t done;            /* end of the batch */
     hipEvent_t done_group;      /* end of the batch's persistent launch */
     bool group_pending;         /* its requests not yet completed */
+    bool heavy;                 /* the batch holds a heavy request */
     /* descriptor staging for the persistent launch */
     tk_prob* hprobs;
     int64_t* hend;
@@ -77,6 +85,7 @@ struct tapp_server {
     int nslots, max_batch;
     size_t budget, in_use;
     int64_t large_flop;         /* products above this run on the general GETT path */
+    size_t heavy_bytes;         /* staging above this makes a request heavy */
     int cus;
     srv_slot slot[SRV_MAX_SLOTS];
     pthread_t thread;
@@ -253,35 +262,49 @@ static void* dispatcher(void* arg)
             for (int i = 0; i < sv->nslots && sv->head; i++) {
                 srv_slot* s = &sv->slot[i];
                 if (s->busy) continue;
-                srv_request *first = NULL, *last = NULL;
+                /* heavy requests are passed over while every other slot
+                 * holds one, so that one stream stays free for small ones */
+                int heavy_busy = 0;
+                for (int j = 0; j < sv->nslots; j++) heavy_busy += sv->slot[j].busy && sv->slot[j].heavy;
+                const bool allow_heavy = heavy_busy < sv->nslots - 1 || sv->nslots == 1;
+                srv_request *first = NULL, *last = NULL, *prev = NULL;
+                srv_request** link = &sv->head;
                 int taken = 0;
                 size_t bytes = 0;
-                while (sv->head && taken < sv->max_batch) {
-                    srv_request* r = sv->head;
+                bool heavy = false;
+                while (*link && taken < sv->max_batch) {
+                    srv_request* r = *link;
                     if (r->state == 3) {          /* cancelled while queued */
-                        sv->head = r->next;
-                        if (!sv->head) sv->tail = NULL;
+                        *link = r->next;
+                        if (sv->tail == r) sv->tail = prev;
                         sv->queued--;
                         r->next = NULL;
                         release(r);
+                        continue;
+                    }
+                    if (r->heavy && !allow_heavy) {
+                        prev = r;
+                        link = &r->next;
                         continue;
                     }
                     if (sv->in_use + bytes + r->footprint > sv->budget) {
                         if (taken == 0) sv->stalls++;
                         break;
                     }
-                    sv->head = r->next;
-                    if (!sv->head) sv->tail = NULL;
+                    *link = r->next;
+                    if (sv->tail == r) sv->tail = prev;
                     sv->queued--;
                     r->next = NULL;
                     r->state = 1;
                     if (last) last->next = r; else first = r;
                     last = r;
                     bytes += r->footprint;
+                    heavy |= r->heavy;
                     taken++;
                 }
                 if (!taken) break;
                 s->batch = first;
+                s->heavy = heavy;
                 s->batch_bytes = bytes;
                 sv->in_use += bytes;
                 sv->inflight += taken;
@@ -329,6 +352,7 @@ TAPP_error TAPP_AMD_create_server(TAPP_AMD_server* server, TAPP_executor exec, s
     sv->max_batch = max_batch;
     sv->budget = hbm_budget_bytes;
     sv->large_flop = (int64_t)1 << 31;
+    sv->heavy_bytes = (size_t)4 << 20;
     sv->paused = (flags & TAPP_AMD_SERVER_PAUSED) != 0;
     if (!base->stream || hipDeviceGetAttribute(&sv->cus, hipDeviceAttributeMultiprocessorCount, base->device) !=
                              hipSuccess || sv->cus <= 0) {
@@ -372,7 +396,7 @@ TAPP_error TAPP_AMD_create_server(TAPP_AMD_server* server, TAPP_executor exec, s
 /* staging footprint of a request: host operands over their address spans,
  * pre-pass buffers for unary labels or storage conversion */
 static int footprint(struct tapp_plan* p, const void* alpha, const void* A, const void* B, const void* beta,
-                     const void* C, void* D, size_t* out)
+                     const void* C, void* D, size_t* out, int64_t* flop)
 {
     struct tapp_exec probe;
     memset(&probe, 0, sizeof probe);
@@ -395,6 +419,7 @@ static int footprint(struct tapp_plan* p, const void* alpha, const void* A, cons
     if (P.nUB > 1 || P.type[1] != (P.ctype == TK_F32 ? 0 : P.ctype == TK_F64 ? 1 : P.ctype == TK_C32 ? 2 : 3))
         f += (size_t)(P.nN * P.nK * P.nL) * cs;
     *out = f;
+    *flop = P.empty ? 0 : product_flop(&P);
     return 0;
 }
 
@@ -406,7 +431,8 @@ TAPP_error TAPP_AMD_server_submit(TAPP_AMD_server server, TAPP_tensor_product pl
     if (!sv || sv->magic != TAPP_AMD_EXEC_MAGIC || !plan || !status) return TAPP_AMD_ERR_INVALID;
     struct tapp_plan* p = (struct tapp_plan*)plan;
     size_t f = 0;
-    int rc = footprint(p, alpha, A, B, beta, C, D, &f);
+    int64_t fl = 0;
+    int rc = footprint(p, alpha, A, B, beta, C, D, &f, &fl);
     if (rc) return rc;
     if (f > sv->budget) return TAPP_AMD_ERR_BUDGET;
     srv_request* r = calloc(1, sizeof *r);
@@ -420,6 +446,7 @@ TAPP_error TAPP_AMD_server_submit(TAPP_AMD_server server, TAPP_tensor_product pl
     if (beta && ss) memcpy(r->beta, beta, ss);
     r->A = A; r->B = B; r->C = C; r->D = D;
     r->footprint = f;
+    r->heavy = fl > sv->large_flop || f > sv->heavy_bytes;
     r->t_submit = now_us();
     pthread_mutex_lock(&sv->mu);
     if (!sv->t_first) sv->t_first = r->t_submit;

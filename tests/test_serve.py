@@ -162,3 +162,38 @@ def test_split_f16_requests_run_the_three_piece_split(lib, gpu_exec, oracle):
         d.close()
     for ex in exs.values():
         lib.destroy_executor(ex)
+
+
+@pytest.mark.gpu
+def test_heavy_requests_leave_a_stream_to_small_ones(lib, gpu_exec, oracle):
+    """Requests that stage more than 4 MiB are heavy: while every other slot
+    holds one, admission passes over them and takes the small requests
+    queued behind them.  Two slots, one request per batch, heavy requests
+    queued first: every request completes, each D as the oracle's."""
+    from test_gpu_parity import assert_parity
+    heavy = [T.tucker_case(200 + i, 128, 96, 96, 4) for i in range(3)]     # A alone is 4.5 MiB
+    small = [T.tucker_case(210 + i, 12, 10, 8, 3) for i in range(6)]
+    cases = [heavy[0], heavy[1], small[0], small[1], heavy[2]] + small[2:]
+    rc, srv = lib.create_server(gpu_exec, 1 << 30, 2, 1, lib.SERVER_PAUSED)
+    assert rc == 0
+    keep, sts = [], []
+    for c in cases:
+        c2 = c.copy()
+        plan, infos = _plan(lib, c2)
+        al = T.scalar_bytes(c2.alpha, c2.D.type)
+        be = T.scalar_bytes(c2.beta, c2.D.type)
+        rc, st = lib.submit(srv, plan, al.ctypes.data, c2.A.ptr(), c2.B.ptr(), be.ctypes.data, None, c2.D.ptr())
+        assert rc == 0
+        keep.append((c2, plan, infos, al, be))
+        sts.append(st)
+    assert lib.set_paused(srv, 0) == 0
+    for st in sts:
+        assert lib.wait(srv, st) == (0, 0)
+        lib.destroy_status(st)
+    rc, stats = lib.server_stats(srv)
+    assert stats["completed"] == len(cases) and stats["batches"] == len(cases), stats
+    for c2, plan, infos, al, be in keep:
+        _, want = oracle.run(c2)
+        assert_parity(c2.D.buf, want, c2)
+        _free(lib, plan, infos)
+    lib.destroy_server(srv)
