@@ -660,7 +660,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU reference baseline")
     ap.add_argument("--no-batch", action="store_true", help="skip the uneven-batch (product group) measurement")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop sweep")
-    ap.add_argument("--serve-requests", type=int, default=2000,
+    ap.add_argument("--serve-requests", type=int, default=10000,
                     help="requests per arrival rate in the serving sweep (the full workload is 200000)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (calibrates the FETCH_SIZE / WRITE_SIZE counters)")
@@ -673,6 +673,12 @@ def main():
         print("bench.py measures one GPU: this build has no multi-GPU path (products shard over D tiles; "
               "see the sharding design).  Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
+    # The serving loop runs its 4 slots on 4 streams besides the executor's
+    # and torch's.  HIP's default of 4 hardware queues per process makes
+    # slot streams share an in-order queue, where a small request's kernel
+    # waits behind a large product's; 8 gives every stream its own.  Read by
+    # the HIP runtime when it starts, so set before torch loads it.
+    os.environ.setdefault("GPU_MAX_HW_QUEUES", "8")
 
     import torch
     assert torch.cuda.is_available()

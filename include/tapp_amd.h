@@ -126,7 +126,11 @@ TAPP_EXPORT TAPP_error TAPP_AMD_group_info(TAPP_AMD_product_group group, int64_t
  * on the general path) and keeps admitting while batches run on other
  * streams.  A heavy request (above 2^31 flop, or staging more than 4 MiB)
  * is passed over while every other stream holds one, so that one stream
- * stays free for the small requests queued behind it.  Each request returns a TAPP_status (status.h:9-17) that
+ * stays free for the small requests queued behind it.  The process needs a
+ * hardware queue per stream it uses (HIP's GPU_MAX_HW_QUEUES, default 4,
+ * read when the runtime starts): at least `slots` + 2, or slot streams
+ * share in-order queues and small requests wait behind large kernels.
+ * Each request returns a TAPP_status (status.h:9-17) that
  * TAPP_AMD_status_wait waits on and TAPP_destroy_status releases.  Operands
  * must stay valid, and plans unchanged, until the request completes; the
  * plans of queued requests must not be executed elsewhere concurrently.
