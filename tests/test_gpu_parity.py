@@ -324,9 +324,10 @@ def test_large_tile_variants(lib, gpu_exec, oracle, dtype, M):
 @pytest.mark.parametrize("mode", ["plain", "bf16", "f16", "split_bf16", "split6_bf16", "split_f16"])
 def test_split_k_few_tiles(lib, oracle, mode):
     """Few 128 x 128 output tiles and long K: the FP32 MFMA kernel splits K
-    over workgroups and a second kernel sums the partials.  Ragged K (not a
-    multiple of the 32-deep K tile), a Hadamard mode, beta != 0 and a
-    transposed operand are covered."""
+    over workgroups and a second kernel sums the partials in split order.
+    Ragged K (not a multiple of the 32-deep K tile), a Hadamard mode,
+    beta != 0 and a transposed operand are covered, and each product run
+    again gives the same bits."""
     prec = {"bf16": tc.BF16BF16_ACCUM_F32, "f16": tc.F16F16_ACCUM_F32}.get(mode, tc.DEFAULT_PREC)
     _, ex = lib.create_executor()
     if mode.startswith("split"):
@@ -340,6 +341,8 @@ def test_split_k_few_tiles(lib, oracle, mode):
         rc, got, want = run_both(lib, ex, oracle, c)
         assert rc == 0
         assert lib.attr_get_int(ex, tc.ATTR_LAST_KERNEL)[1] == tc.KERNEL_MFMA
+        rc, again = T.run_lib(lib, c, ex)
+        assert rc == 0 and np.array_equal(again.view(np.uint32), got.view(np.uint32)), c.name
         if mode.startswith("split"):
             _, exact = oracle.run(c, f64=True)
             assert T.normwise_error(got, exact, T.written_mask(c.D)) < SPLIT_TOL[mode]
