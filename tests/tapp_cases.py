@@ -445,6 +445,45 @@ def random_case(seed, scenario="contraction", dtype=tc.F32, max_ext=4, prec=tc.D
 # --------------------------------------------------------------------------
 # the batching workload (uneven products)
 # --------------------------------------------------------------------------
+def layout_mix_case(seed, dtype=tc.F32, prec=tc.DEFAULT_PREC, max_size=3e7):
+    """A multi-tile binary contraction with every layout freedom at once:
+    D[a,b,c,d,(h)] = alpha sum_{e,f} A[a,e,b,f,(h)] op(B[e,c,f,d,(h)]) + beta C,
+    extents a..d ~ U{2..48}, e, f ~ U{2..40}, an optional Hadamard mode h,
+    and each operand with its own label order, storage order, padding and
+    sometimes one negative stride; D's storage order differs from C's.
+    Complex cases conjugate B half the time; beta is zero 30 % of the time."""
+    rng = np.random.default_rng(seed)
+    while True:
+        ext = {c: int(rng.integers(2, 49)) for c in "abcd"}
+        ext.update({c: int(rng.integers(2, 41)) for c in "ef"})
+        ext["h"] = int(rng.integers(1, 5)) if rng.random() < 0.5 else 0
+        total = np.prod([ext[c] for c in "abcdef"]) * max(1, ext["h"])
+        if 2e5 < total < max_size:
+            break
+
+    def operand(labs, s):
+        labs = [x for x in labs if x != "h" or ext["h"]]
+        perm = [str(x) for x in rng.permutation(labs)]             # label order
+        order = [int(x) for x in rng.permutation(len(perm))]     # storage order of the modes
+        neg = (int(rng.integers(len(perm))),) if rng.random() < 0.25 else ()
+        pad = int(rng.choice([0, 0, 1, 3]))
+        return make_operand(dtype, [ext[x] for x in perm], "".join(perm), s, order=order, pad=pad, negative=neg)
+
+    A = operand("aebfh", seed * 4 + 1)
+    B = operand("ecfdh", seed * 4 + 2)
+    C = operand("abcdh", seed * 4 + 3)
+    D = make_operand(dtype, C.extents, "".join(chr(x) for x in C.idx), seed * 4 + 4,
+                     order=[int(x) for x in rng.permutation(len(C.extents))], pad=int(rng.choice([0, 2])))
+    if dtype in COMPLEX_TYPES:
+        B.op = int(rng.random() < 0.5)
+        alpha = complex(rng.normal(), rng.normal())
+        beta = 0.0 if rng.random() < 0.3 else complex(rng.normal(), rng.normal())
+    else:
+        alpha = float(rng.normal())
+        beta = 0.0 if rng.random() < 0.3 else float(rng.normal())
+    return Case(A, B, C, D, prec, alpha, beta, name=f"layout_mix{seed}_{dtype}")
+
+
 def tucker_case(seed, i, j, k, r, dtype=tc.F32, prec=tc.DEFAULT_PREC):
     """D[r,j,k] = sum_i A[i,j,k] U[r,i] (a mode-1 Tucker product), dense."""
     A = make_operand(dtype, [i, j, k], "ijk", seed * 4 + 0)

@@ -81,6 +81,34 @@ def test_random_scenarios(lib, gpu_exec, oracle, dtype, prec, scenario):
             assert_parity(got, want, c)
 
 
+@pytest.mark.parametrize("dtype", [tc.F32, tc.F64, tc.C32, tc.C64])
+def test_layout_mix_multi_tile(lib, gpu_exec, oracle, dtype):
+    """Multi-tile products with random label orders, storage orders, padded
+    and negative strides, a Hadamard mode and conj(B): the auto kernel
+    choice and the MFMA kernel forced, each against the oracle."""
+    _, exm = lib.create_executor()
+    assert lib.attr_set_int(exm, tc.ATTR_KERNEL, tc.KERNEL_MFMA) == 0
+    for seed in range(6):
+        c = T.layout_mix_case(1000 * dtype + seed, dtype)
+        rc, got, want = run_both(lib, gpu_exec if seed % 2 == 0 else exm, oracle, c)
+        assert rc == 0
+        assert_parity(got, want, c)
+    lib.destroy_executor(exm)
+
+
+@pytest.mark.parametrize("mode", ["split_bf16", "split6_bf16", "split_f16"])
+def test_layout_mix_split_forms(lib, oracle, mode):
+    """The split forms on the same layouts, against the FP64 oracle."""
+    _, ex = lib.create_executor()
+    assert lib.attr_set_int(ex, tc.ATTR_SPLIT_BF16, SPLIT_ATTR[mode]) == 0
+    for seed in range(4):
+        c = T.layout_mix_case(7000 + 100 * SPLIT_ATTR[mode] + seed, tc.F32)
+        rc, got, exact = run_both(lib, ex, oracle, c, f64_oracle=True)
+        assert rc == 0
+        assert T.normwise_error(got, exact, T.written_mask(c.D)) < SPLIT_TOL[mode], c.name
+    lib.destroy_executor(ex)
+
+
 def test_reference_program_known_answers(lib, gpu_exec):
     from test_oracle import _case_from_kat, load_kats
     for k in load_kats():
