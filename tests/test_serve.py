@@ -197,3 +197,56 @@ def test_heavy_requests_leave_a_stream_to_small_ones(lib, gpu_exec, oracle):
         assert_parity(c2.D.buf, want, c2)
         _free(lib, plan, infos)
     lib.destroy_server(srv)
+
+
+@pytest.mark.gpu
+def test_concurrent_client_threads(lib, gpu_exec, oracle):
+    """Four client threads submit their own mixes at the same time (ctypes
+    calls release the GIL, so submits, waits and the dispatcher interleave),
+    twice each; every request completes with 0 and every D matches the same
+    product run alone."""
+    import threading
+    from test_group import DevCase, mixed_cases
+    from test_gpu_parity import assert_parity
+    sets = [[DevCase(lib, c) for c in mixed_cases(41 + 7 * t)] for t in range(4)]
+    alone = []
+    for dcs in sets:
+        row = []
+        for d in dcs:
+            assert d.run_single(gpu_exec) == 0
+            row.append(d.d().copy())
+            d.reset_d()
+        alone.append(row)
+    rc, srv = lib.create_server(gpu_exec, 1 << 30, 4, 8)
+    assert rc == 0
+    errors = []
+
+    def client(dcs):
+        try:
+            for _ in range(2):
+                sts = []
+                for d in dcs:
+                    rc, st = lib.submit(srv, d.plan, d.al.ctypes.data, d.ptr["A"], d.ptr["B"], d.be.ctypes.data,
+                                        d.c_ptr(), d.ptr["D"])
+                    assert rc == 0, rc
+                    sts.append(st)
+                for st in sts:
+                    assert lib.wait(srv, st) == (0, 0)
+                    lib.destroy_status(st)
+        except Exception as e:  # reported by the main thread
+            errors.append(repr(e))
+
+    threads = [threading.Thread(target=client, args=(dcs,)) for dcs in sets]
+    for th in threads:
+        th.start()
+    for th in threads:
+        th.join(timeout=100)
+    assert not any(th.is_alive() for th in threads), "client threads did not finish"
+    assert not errors, errors
+    rc, stats = lib.server_stats(srv)
+    assert stats["completed"] == 2 * sum(len(s) for s in sets), stats
+    lib.destroy_server(srv)
+    for dcs, row in zip(sets, alone):
+        for d, a in zip(dcs, row):
+            assert_parity(d.d(), a, d.case)
+            d.close()
