@@ -149,3 +149,57 @@ def test_unsupported_precision_is_an_error_code(lib):
     lib.destroy_plan(plan)
     for h in infos:
         lib.destroy_info(h)
+
+
+C_PROGRAM = r"""
+#include <stdio.h>
+#include <string.h>
+#include "tapp.h"
+#include "tapp_amd.h"
+
+int main(void)
+{
+    /* D[a,b] = sum_c A[a,c] B[c,b]: the object API a reference caller uses */
+    int64_t ea[2] = {4, 3}, sa[2] = {1, 4}, eb[2] = {3, 5}, sb[2] = {1, 3}, ed[2] = {4, 5}, sd[2] = {1, 4};
+    int64_t ia[2] = {'a', 'c'}, ib[2] = {'c', 'b'}, id[2] = {'a', 'b'};
+    TAPP_tensor_info A, B, C, D;
+    if (TAPP_create_tensor_info(&A, TAPP_F32, 2, ea, sa) || TAPP_create_tensor_info(&B, TAPP_F32, 2, eb, sb) ||
+        TAPP_create_tensor_info(&C, TAPP_F32, 2, ed, sd) || TAPP_create_tensor_info(&D, TAPP_F32, 2, ed, sd))
+        return 1;
+    if (TAPP_get_nmodes(A) != 2) return 2;
+    TAPP_tensor_product plan;
+    if (TAPP_create_tensor_product(&plan, (TAPP_handle)0, TAPP_IDENTITY, A, ia, TAPP_IDENTITY, B, ib,
+                                   TAPP_IDENTITY, C, id, TAPP_IDENTITY, D, id, TAPP_DEFAULT_PREC))
+        return 3;
+    char msg[128];
+    size_t n = TAPP_explain_error(0, sizeof msg, msg);
+    printf("version=%s explain0=%s n=%zu\n", TAPP_AMD_version(), msg, n);
+    if (TAPP_destroy_tensor_product(plan)) return 4;
+    TAPP_destroy_tensor_info(A); TAPP_destroy_tensor_info(B);
+    TAPP_destroy_tensor_info(C); TAPP_destroy_tensor_info(D);
+    return 0;
+}
+"""
+
+
+def test_c_program_builds_against_the_headers_and_links(tmp_path):
+    """A reference caller's C code compiles against include/ (C99, -Wall
+    -Werror), links to the library by name, and runs its object API calls
+    without a GPU; the headers also compile as C++ (extern "C" guards)."""
+    import shutil
+    import subprocess
+    cc = shutil.which("gcc")
+    cxx = shutil.which("g++")
+    if not cc or not cxx:
+        pytest.skip("gcc / g++ not found")
+    libdir = os.path.join(T.PKG, "lib")
+    src = tmp_path / "caller.c"
+    src.write_text(C_PROGRAM)
+    exe = tmp_path / "caller"
+    subprocess.run([cc, "-std=c99", "-Wall", "-Wextra", "-Werror", "-I", INCLUDE, str(src), "-L", libdir,
+                    "-ltapp_amd", f"-Wl,-rpath,{libdir}", "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout
+    assert out.startswith("version=") and "n=" in out, out
+    hdr = tmp_path / "headers.cpp"
+    hdr.write_text('#include "tapp.h"\n#include "tapp_amd.h"\nint main() { return TAPP_AMD_ERR_BUSY == 107 ? 0 : 1; }\n')
+    subprocess.run([cxx, "-std=c++11", "-Wall", "-Werror", "-I", INCLUDE, "-fsyntax-only", str(hdr)], check=True)
