@@ -124,7 +124,7 @@ TAPP_EXPORT TAPP_error TAPP_AMD_group_info(TAPP_AMD_product_group group, int64_t
  * buffers; device operands cost nothing -- fits `hbm_budget_bytes`, launches
  * them as one batch (small FP32 products in one persistent launch, the rest
  * on the general path) and keeps admitting while batches run on other
- * streams.  A heavy request (above 2^31 flop, or staging more than 4 MiB)
+ * streams.  A heavy request (above 2^31 flop, or a footprint above 4 MiB)
  * is passed over while every other stream holds one, so that one stream
  * stays free for the small requests queued behind it.  The process needs a
  * hardware queue per stream it uses (HIP's GPU_MAX_HW_QUEUES, default 4,

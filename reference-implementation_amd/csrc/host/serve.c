@@ -19,8 +19,9 @@
  * requests complete when it ends, not behind the batch's large products;
  * a second event marks the end of the batch.
  *
- * Heavy requests -- above the persistent launch's flop limit, or staging
- * more than heavy_bytes over PCIe -- hold a stream for a long time.  While
+ * Heavy requests -- above the persistent launch's flop limit, or with a
+ * footprint (host operands to stage plus pre-pass buffers) above
+ * heavy_bytes -- hold a stream for a long time.  While
  * every other slot holds one, admission passes over them (they keep their
  * place in the queue) and takes the small requests behind them, so small
  * requests never wait for a stream behind heavy ones.
@@ -58,7 +59,7 @@ typedef struct srv_request {
     size_t footprint;
     double t_submit, t_start, t_done;
     int in_group;               /* in the batch: 1 persistent launch, 0 general path, -1 nothing to launch */
-    bool heavy;                 /* above the persistent launch's flop limit, or staging above heavy_bytes */
+    bool heavy;                 /* above the persistent launch's flop limit, or footprint above heavy_bytes */
     struct srv_request* next;   /* queue / batch list */
 } srv_request;
 
@@ -85,7 +86,7 @@ struct tapp_server {
     int nslots, max_batch;
     size_t budget, in_use;
     int64_t large_flop;         /* products above this run on the general GETT path */
-    size_t heavy_bytes;         /* staging above this makes a request heavy */
+    size_t heavy_bytes;         /* a footprint above this makes a request heavy */
     int cus;
     srv_slot slot[SRV_MAX_SLOTS];
     pthread_t thread;

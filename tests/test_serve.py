@@ -167,11 +167,11 @@ def test_split_f16_requests_run_the_three_piece_split(lib, gpu_exec, oracle):
 @pytest.mark.gpu
 @pytest.mark.parametrize("slots", [1, 2, 4])
 def test_heavy_requests_leave_a_stream_to_small_ones(lib, gpu_exec, oracle, slots):
-    """Requests that stage more than 4 MiB are heavy: while every other slot
-    holds one, admission passes over them and takes the small requests
-    queued behind them (with one slot, heavy requests are admitted as they
-    come).  One request per batch, heavy requests queued first: every
-    request completes, each D as the oracle's."""
+    """Requests with a footprint above 4 MiB (here: a staged host A) are
+    heavy: while every other slot holds one, admission passes over them and
+    takes the small requests queued behind them (with one slot, heavy
+    requests are admitted as they come).  One request per batch, heavy
+    requests queued first: every request completes, each D as the oracle's."""
     from test_gpu_parity import assert_parity
     heavy = [T.tucker_case(200 + i, 128, 96, 96, 4) for i in range(3)]     # A alone is 4.5 MiB
     small = [T.tucker_case(210 + i, 12, 10, 8, 3) for i in range(6)]
