@@ -15,13 +15,21 @@
  *     F64F64_ACCUM_F64: bit-exact against the compiled reference on random
  *     cases (same per-element operation order), and against the known
  *     answers printed by the reference's demo/driver/exercise programs.
- *   - orc_execute, F16F16_* precisions: tolerance against the clang build of
- *     the reference with TAPP_REFERENCE_ENABLE_F16 (whose _Float16
- *     arithmetic may keep excess precision, so not bit-exact).
- *   - orc_execute, BF16BF16_ACCUM_F32: the reference's BF16 code does not
- *     compile (product.c:1219, 1268, 1586), so this form has no runnable
- *     reference: pinned by the bfloat16 rounding known answers and by the
- *     FP64 oracle with the u_bf16 + gamma_K bound.
+ *   - orc_execute, mixed storage types (loop_generic): bit-exact against
+ *     the compiled reference on every combination of F32/F64/C32/C64 over
+ *     A, B, C, D under DEFAULT_PREC, F32F32_ACCUM_F32 and F64F64_ACCUM_F64.
+ *   - orc_execute, TAPP_F16F16_ACCUM_F32 / _F16 and DEFAULT_PREC with
+ *     binary16 storage (loop_generic): bit-exact against the clang build of
+ *     the reference with TAPP_REFERENCE_ENABLE_F16 on every scenario class,
+ *     including unary sums, beta != 0 and mixed storage types.  loop_generic
+ *     also gives the per-type loops' bits on uniform F32/F64/C32/C64 data.
+ *   - orc_execute, BF16BF16_ACCUM_F32 and bfloat16 storage: the reference's
+ *     BF16 code does not compile (product.c:1219, 1268, 1586, 3562), so the
+ *     __bf16 evaluation is restated by analogy with _Float16 (same rules,
+ *     8-bit significand) and is PARITY UNPINNED; what is pinned is the
+ *     ORC_GPU_PRODUCTS form the GPU library computes (equal to the binary32
+ *     loop on bfloat16-rounded inputs), the bfloat16 rounding known answers
+ *     and the FP64 oracle with the 2 u_bf16 + gamma_K bound.
  *   - orc_execute_f64: pinned by agreement with orc_execute at F64F64
  *     precision (same operation order, bit-exact for F64 data) and by closed
  *     forms (rank-1 inputs).
@@ -101,6 +109,24 @@ static uint16_t float_to_half_bits(float x)
 float orc_round_f16(float x) { return half_bits_to_float(float_to_half_bits(x)); }
 
 static float bf16_bits_to_float(uint16_t b) { return u2f((uint32_t)b << 16); }
+
+/* round-to-nearest-even of a binary64 value to p significant bits with
+ * smallest normal exponent emin (subnormal spacing 2^(emin - p + 1)) and
+ * overflow to infinity at or above `ovf`: one rounding from the exact
+ * value, as a conversion from binary64 or binary32 does */
+static double rnd_bits(double x, int p, int emin, double ovf)
+{
+    if (!isfinite(x) || x == 0.0) return x;
+    double a = fabs(x);
+    int e;
+    frexp(a, &e);                    /* a = f 2^e, f in [0.5, 1): ulp exponent e - p */
+    int q = e - p;
+    if (q < emin - p + 1) q = emin - p + 1;
+    double r = ldexp(rint(ldexp(a, -q)), q);
+    if (r >= ovf) r = INFINITY;
+    return copysign(r, x);
+}
+
 
 static bool is_complex_type(int t) { return t == T_C32 || t == T_C64; }
 
@@ -396,8 +422,13 @@ static void store_z(void* base, int type, int64_t off, double complex v)
     case T_F64: ((double*)base)[off] = creal(v); break;
     case T_C32: { float* p = (float*)base + 2 * off; p[0] = (float)creal(v); p[1] = (float)cimag(v); } break;
     case T_C64: { double* p = (double*)base + 2 * off; p[0] = creal(v); p[1] = cimag(v); } break;
-    case T_F16: ((uint16_t*)base)[off] = float_to_half_bits((float)creal(v)); break;
-    case T_BF16: ((uint16_t*)base)[off] = (uint16_t)(f2u(orc_round_bf16((float)creal(v))) >> 16); break;
+    /* one rounding from the accumulator's value, as the conversion in
+     * assign_D does (a binary32 accumulator converts exactly to binary64) */
+    case T_F16: ((uint16_t*)base)[off] = float_to_half_bits((float)rnd_bits(creal(v), 11, -14, 65520.0)); break;
+    case T_BF16:
+        ((uint16_t*)base)[off] =
+            (uint16_t)(f2u((float)rnd_bits(creal(v), 8, -126, ldexp(2.0 - ldexp(1.0, -8), 127))) >> 16);
+        break;
     default: break;
     }
 }
@@ -475,6 +506,211 @@ ORC_LOOP(loop_f64, double, CONV_D)
 ORC_LOOP(loop_c32, float complex, CONV_CF)
 ORC_LOOP(loop_c64, double complex, CONV_CD)
 
+/* ------------------------------------------------------------------ */
+/* the loop nest with binary16 / bfloat16 values (product.c:312-399)  */
+/* ------------------------------------------------------------------ */
+
+/* Which precision every value of the loop nest has.  The reference keeps
+ * each quantity in a type chosen per precision branch: accum by alloc_accum
+ * (:1012-1102), sum_A / sum_B / val_C by alloc_val (:1104-1230), alpha and
+ * beta by create_prec_scalar (:1232-1490).  Those types reach _Float16
+ * (TAPP_F16F16_* precisions, and DEFAULT_PREC with binary16 storage) and
+ * __bf16.  The reference's F16 build is compiled by clang for x86-64, which
+ * has no binary16 arithmetic: every _Float16 operation is evaluated in
+ * binary32 and rounded to binary16 only at an assignment or when the
+ * _Float16-typed result of an expression is converted to another type
+ * (clang's default -ffloat16-excess-precision=standard).  Within one
+ * expression clang may contract a*b + c into llvm.fmuladd, which baseline
+ * x86-64 (no FMA) executes as a rounded multiply then a rounded add.  The
+ * rules below restate exactly that evaluation, one rounding per
+ * operation.  Every value is carried in binary64, which holds each of these
+ * formats exactly; binary32 operations are computed in binary64 and rounded
+ * once to binary32, which for +, -, * of binary32 operands gives the
+ * binary32 result (53 >= 2 * 24 + 2). */
+enum { K_H = 1, K_B = 2, K_F = 3, K_D = 4 };   /* _Float16, __bf16, float, double */
+
+typedef struct { int k; bool c; } slot;      /* precision, complex */
+
+static int kind_of_storage(int t)
+{
+    switch (t) {
+    case T_F16: return K_H;
+    case T_BF16: return K_B;
+    case T_F32: case T_C32: return K_F;
+    default: return K_D;
+    }
+}
+
+/* the common (semantic) type of a binary operation */
+static int kmax(int a, int b)
+{
+    if (a == b) return a;
+    if (a == K_D || b == K_D) return K_D;
+    return K_F;                      /* float with anything narrower; _Float16 with __bf16 */
+}
+
+/* the type an operation is evaluated in */
+static int keval(int k) { return k == K_H || k == K_B ? K_F : k; }
+static bool lowp(int k) { return k == K_H || k == K_B; }
+
+static double rnd_k(double x, int k)
+{
+    switch (k) {
+    case K_H: return rnd_bits(x, 11, -14, 65520.0);
+    case K_B: return rnd_bits(x, 8, -126, ldexp(2.0 - ldexp(1.0, -8), 127));
+    case K_F: return (double)(float)x;
+    default: return x;
+    }
+}
+
+static double complex rnd_z(double complex z, slot s)
+{
+    return CMPLX(rnd_k(creal(z), s.k), s.c ? rnd_k(cimag(z), s.k) : 0.0);
+}
+
+/* x * y evaluated in precision k (C's textbook complex product, one rounding
+ * per operation; a real operand multiplies each part) */
+static double complex mul_k(double complex x, bool xc, double complex y, bool yc, int k)
+{
+    double xr = creal(x), xi = cimag(x), yr = creal(y), yi = cimag(y);
+    if (xc && yc)
+        return CMPLX(rnd_k(rnd_k(xr * yr, k) - rnd_k(xi * yi, k), k), rnd_k(rnd_k(xi * yr, k) + rnd_k(xr * yi, k), k));
+    if (xc) return CMPLX(rnd_k(xr * yr, k), rnd_k(xi * yr, k));
+    if (yc) return CMPLX(rnd_k(xr * yr, k), rnd_k(xr * yi, k));
+    return rnd_k(xr * yr, k);
+}
+
+/* dst op= (value v of semantic slot vs): C's compound assignment with the
+ * excess-precision rule above.  A _Float16 / __bf16 right-hand side is
+ * rounded to its own type first unless dst has that same type. */
+static double complex assign_add(double complex dst, slot ds, double complex v, slot vs)
+{
+    if (lowp(vs.k) && vs.k != ds.k) v = rnd_z(v, vs);
+    const int e = keval(kmax(ds.k, vs.k));
+    double re = rnd_k(rnd_k(creal(dst) + creal(v), e), ds.k);
+    double im = cimag(dst);
+    if (ds.c && vs.c) im = rnd_k(rnd_k(cimag(dst) + cimag(v), e), ds.k);
+    return CMPLX(re, ds.c ? im : 0.0);
+}
+
+static double complex assign_set(slot ds, double complex v, slot vs)
+{
+    if (lowp(vs.k) && vs.k != ds.k) v = rnd_z(v, vs);
+    return CMPLX(rnd_k(creal(v), ds.k), ds.c && vs.c ? rnd_k(cimag(v), ds.k) : 0.0);
+}
+
+typedef struct {
+    slot acc, sa, sb, vc, scal;   /* accum, sum_A, sum_B, val_C, alpha/beta */
+    int conj_sums;                /* sum_unary_contractions applies op_A / op_B */
+    bool no_store;                /* F16F16_ACCUM_F16 with complex D: assign_D stores nothing */
+    bool gpu_products;            /* products and accumulation in binary32 (not the reference) */
+} slots;
+
+/* Slot precisions of one call (alloc_accum :1012, alloc_val :1104,
+ * create_prec_scalar :1232, sum_unary_contractions :1863, assign_D
+ * :10989-11118). */
+static void choose_slots(int prec, const orc_tensor* A, const orc_tensor* B, const orc_tensor* C,
+                         const orc_tensor* D, slots* q)
+{
+    int k_acc, k_val;
+    memset(q, 0, sizeof *q);
+    switch (prec) {
+    case P_DEFAULT: k_acc = kind_of_storage(D->type); k_val = 0; break;
+    case P_F32: case P_F16F16_F32: case P_BF16BF16_F32: k_acc = K_F; break;
+    case P_F64: k_acc = K_D; break;
+    default: k_acc = K_H; break;                 /* P_F16F16_F16 */
+    }
+    switch (prec) {
+    case P_DEFAULT: k_val = 0; break;
+    case P_F32: k_val = K_F; break;
+    case P_F64: k_val = K_D; break;
+    case P_BF16BF16_F32: k_val = K_B; break;
+    default: k_val = K_H; break;
+    }
+    q->acc = (slot){k_acc, is_complex_type(D->type)};
+    q->sa = (slot){k_val ? k_val : kind_of_storage(A->type), is_complex_type(A->type)};
+    q->sb = (slot){k_val ? k_val : kind_of_storage(B->type), is_complex_type(B->type)};
+    q->vc = (slot){k_val ? k_val : kind_of_storage(C->type), is_complex_type(C->type)};
+    q->scal = (slot){k_val ? k_val : kind_of_storage(D->type), is_complex_type(D->type)};
+    q->conj_sums = prec == P_DEFAULT || prec == P_F32 || prec == P_F64;
+    q->no_store = prec == P_F16F16_F16 && is_complex_type(D->type);
+}
+
+static bool slots_lowp(const slots* q)
+{
+    return lowp(q->acc.k) || lowp(q->sa.k) || lowp(q->sb.k) || lowp(q->vc.k) || lowp(q->scal.k);
+}
+
+/* The loop nest of product.c:312-399 over slot precisions: the same
+ * operation order as ORC_LOOP, with each value rounded as its slot and each
+ * expression as the rules above. */
+static void loop_generic(const plan* p, const semantics* s, const slots* q)
+{
+    int nd = p->d.n;
+    int64_t* cf = calloc((size_t)nd + 1, 8);
+    int64_t* ck = calloc((size_t)p->nk + 1, 8);
+    int64_t* cua = calloc((size_t)p->nua + 1, 8);
+    int64_t* cub = calloc((size_t)p->nub + 1, 8);
+    slot acc_s = q->acc, sa_s = q->sa, sb_s = q->sb;
+    if (q->gpu_products) { acc_s.k = K_F; }
+    const double complex alpha = rnd_z(s->alpha ? load_z(s->alpha, s->type_alpha, 0) : 0, q->scal);
+    const double complex beta = rnd_z(s->beta ? load_z(s->beta, s->type_alpha, 0) : 0, q->scal);
+    const int kA = kind_of_storage(s->tA), kB = kind_of_storage(s->tB);
+    /* alpha * sum_A * sum_B, left to right (:3431-10735) */
+    const slot ab = {kmax(q->scal.k, sa_s.k), q->scal.c || sa_s.c};
+    slot prod = {kmax(ab.k, sb_s.k), ab.c || sb_s.c};
+    const int e_ab = keval(ab.k), e_prod = keval(prod.k);
+    if (q->gpu_products) prod.k = K_F;
+    const slot bc = {kmax(q->scal.k, q->vc.k), q->scal.c || q->vc.c};
+    for (int64_t i = 0; i < p->size_free; i++) {
+        int64_t oa = 0, ob = 0, oc = 0, od = 0;
+        for (int j = 0; j < nd; j++) {
+            oa += cf[j] * p->fa[j]; ob += cf[j] * p->fb[j];
+            oc += cf[j] * p->c.str[j]; od += cf[j] * p->d.str[j];
+        }
+        double complex acc = 0;
+        if (!s->beta_zero) {                                            /* :351-360 */
+            double complex c = rnd_z(load_z(s->C, s->tC, oc), q->vc);   /* get_val :10830 */
+            c = opz(c, s->op_C);
+            slot bcs = bc;
+            if (q->gpu_products) bcs.k = K_F;
+            acc = assign_set(acc_s, mul_k(beta, q->scal.c, c, q->vc.c, keval(bc.k)), bcs);
+        }
+        for (int64_t j = 0; j < p->size_k; j++) {
+            int64_t ka = oa, kb = ob;
+            for (int k = 0; k < p->nk; k++) { ka += ck[k] * p->k_sa[k]; kb += ck[k] * p->k_sb[k]; }
+            double complex sa = 0, sb = 0;                              /* zero_sum :1516 */
+            for (int64_t u = 0; u < p->size_ua; u++) {                  /* :370-380, :1863 */
+                int64_t x = ka;
+                for (int l = 0; l < p->nua; l++) x += cua[l] * p->ua_s[l];
+                double complex v = load_z(s->A, s->tA, x);
+                if (q->conj_sums) v = opz(v, s->op_A);
+                sa = assign_add(sa, sa_s, v, (slot){kA, is_complex_type(s->tA)});
+                odometer(cua, p->nua, p->ua_ext);
+            }
+            for (int64_t u = 0; u < p->size_ub; u++) {                  /* :381-391 */
+                int64_t x = kb;
+                for (int l = 0; l < p->nub; l++) x += cub[l] * p->ub_s[l];
+                double complex v = load_z(s->B, s->tB, x);
+                if (q->conj_sums) v = opz(v, s->op_B);
+                sb = assign_add(sb, sb_s, v, (slot){kB, is_complex_type(s->tB)});
+                odometer(cub, p->nub, p->ub_ext);
+            }
+            double complex t = mul_k(alpha, q->scal.c, sa, sa_s.c, e_ab);
+            t = mul_k(t, ab.c, sb, sb_s.c, e_prod);
+            acc = assign_add(acc, acc_s, t, prod);                       /* :393 */
+            odometer(ck, p->nk, p->k_ext);
+        }
+        if (s->op_D == OP_CONJ) {                                       /* :10737-10828 */
+            if (s->conjf_roundtrip) acc = conjf((float complex)acc);
+            else acc = conj(acc);
+        }
+        if (!q->no_store) store_z(s->D, s->tD, od, acc);                /* :10989-11118 */
+        odometer(cf, nd, p->d.ext);
+    }
+    free(cf); free(ck); free(cua); free(cub);
+}
+
 static bool known_type(int t) { return t >= T_F32 && t <= T_BF16; }
 
 /* Picks the compute type from the precision and the storage types, and the
@@ -515,11 +751,11 @@ static int resolve(const orc_tensor* A, const orc_tensor* B, const orc_tensor* C
     return 0;
 }
 
-int orc_execute(const orc_tensor* A, const orc_tensor* B, const orc_tensor* C,
-                const orc_tensor* D, int prec,
-                const void* alpha, const void* A_data, const void* B_data,
-                const void* beta, const void* C_data, void* D_data,
-                const int* exec_kind)
+int orc_execute_ex(const orc_tensor* A, const orc_tensor* B, const orc_tensor* C,
+                   const orc_tensor* D, int prec,
+                   const void* alpha, const void* A_data, const void* B_data,
+                   const void* beta, const void* C_data, void* D_data,
+                   const int* exec_kind, int flags)
 {
     int rc = orc_check(A, B, C, D, beta, C_data, exec_kind);
     if (rc) return rc;
@@ -530,12 +766,30 @@ int orc_execute(const orc_tensor* A, const orc_tensor* B, const orc_tensor* C,
     s.A = A_data; s.B = B_data; s.C = C_data; s.D = D_data;
     s.tA = A->type; s.tB = B->type; s.tC = C->type; s.tD = D->type;
     s.beta_zero = scalar_is_zero(beta, D->type);
+    slots q;
+    choose_slots(prec, A, B, C, D, &q);
+    q.gpu_products = (flags & ORC_GPU_PRODUCTS) != 0;
+    if (q.gpu_products) q.no_store = false;
     plan p;
     plan_build(A, B, C, D, &p);
-    if (cplx) { if (wide) loop_c64(&p, &s); else loop_c32(&p, &s); }
-    else      { if (wide) loop_f64(&p, &s); else loop_f32(&p, &s); }
+    /* the per-type loops hold every value in one compute type, which is the
+     * reference's only when all four storage types are equal and no value is
+     * binary16 / bfloat16; otherwise each value keeps its own slot type */
+    const bool uniform = A->type == B->type && A->type == C->type && A->type == D->type;
+    if (!uniform || slots_lowp(&q) || (flags & (ORC_GENERIC | ORC_GPU_PRODUCTS))) loop_generic(&p, &s, &q);
+    else if (cplx) { if (wide) loop_c64(&p, &s); else loop_c32(&p, &s); }
+    else           { if (wide) loop_f64(&p, &s); else loop_f32(&p, &s); }
     plan_free(&p);
     return 0;
+}
+
+int orc_execute(const orc_tensor* A, const orc_tensor* B, const orc_tensor* C,
+                const orc_tensor* D, int prec,
+                const void* alpha, const void* A_data, const void* B_data,
+                const void* beta, const void* C_data, void* D_data,
+                const int* exec_kind)
+{
+    return orc_execute_ex(A, B, C, D, prec, alpha, A_data, B_data, beta, C_data, D_data, exec_kind, 0);
 }
 
 int orc_execute_f64(const orc_tensor* A, const orc_tensor* B, const orc_tensor* C,

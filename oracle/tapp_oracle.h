@@ -48,6 +48,25 @@ int orc_execute(const orc_tensor* A, const orc_tensor* B, const orc_tensor* C,
                 const void* beta, const void* C_data, void* D_data,
                 const int* exec_kind);
 
+/* orc_execute with options:
+ *   ORC_GENERIC: run the slot-precision loop (the one binary16 / bfloat16
+ *     values need) for every precision; on binary32 / binary64 data it
+ *     must give the same bits as the per-type loops, which tests check.
+ *   ORC_GPU_PRODUCTS: the TAPP_F16F16_* / TAPP_BF16BF16_ACCUM_F32 and
+ *     binary16-storage semantics of the GPU library, not of the reference:
+ *     alpha, beta, C and the unary sums are rounded as the reference does
+ *     (inputs rounded to the 16-bit format, sums rounded at each addition),
+ *     but alpha * sum_A * sum_B is not rounded to 16 bits and the
+ *     accumulator is binary32.  For products without unary labels this is
+ *     the binary32 loop on inputs rounded to the 16-bit format. */
+#define ORC_GENERIC 1
+#define ORC_GPU_PRODUCTS 2
+int orc_execute_ex(const orc_tensor* A, const orc_tensor* B, const orc_tensor* C,
+                   const orc_tensor* D, int prec,
+                   const void* alpha, const void* A_data, const void* B_data,
+                   const void* beta, const void* C_data, void* D_data,
+                   const int* exec_kind, int flags);
+
 /* Same as orc_execute but computing every operation in binary64 (complex
  * binary64 when any operand is complex) whatever the precision, after
  * applying the input rounding `round_inputs` (0 none, 1 binary16, 2

@@ -231,6 +231,8 @@ class Oracle:
         P = ctypes.POINTER(_OrcTensor)
         self.lib.orc_execute.argtypes = [P, P, P, P, ctypes.c_int] + [ctypes.c_void_p] * 6 + [
             ctypes.POINTER(ctypes.c_int)]
+        self.lib.orc_execute_ex.argtypes = [P, P, P, P, ctypes.c_int] + [ctypes.c_void_p] * 6 + [
+            ctypes.POINTER(ctypes.c_int), ctypes.c_int]
         self.lib.orc_execute_f64.argtypes = [P, P, P, P, ctypes.c_int] + [ctypes.c_void_p] * 6 + [ctypes.c_int]
         self.lib.orc_check.argtypes = [P, P, P, P, ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]
         self.lib.orc_classify.argtypes = [P, P, P, P, ctypes.POINTER(_OrcClasses)]
@@ -250,7 +252,16 @@ class Oracle:
     def tensors(self, case, keep):
         return [ctypes.byref(self._t(o, keep)) for o in case.operands()]
 
-    def run(self, case, exec_kind=1, f64=False, round_inputs=0):
+    GENERIC = 1        # ORC_GENERIC: the slot-precision loop for every precision
+    GPU_PRODUCTS = 2   # ORC_GPU_PRODUCTS: the GPU library's 16-bit semantics (binary32 products)
+
+    def run(self, case, exec_kind=1, f64=False, round_inputs=0, flags=0, gpu=False):
+        """gpu=True: the GPU library's semantics for products that involve
+        16-bit values (its one deviation from the reference there: products
+        and accumulation in binary32, see ORC_GPU_PRODUCTS); the same as
+        gpu=False for every other product."""
+        if gpu and not f64:
+            flags |= gpu_semantics_flags(case)
         case = case.copy()
         keep = []
         ts = self.tensors(case, keep)
@@ -262,8 +273,8 @@ class Oracle:
                                           c_ptr, case.D.ptr(), round_inputs)
         else:
             kind = None if exec_kind is None else ctypes.byref(ctypes.c_int(exec_kind))
-            rc = self.lib.orc_execute(*ts, case.prec, al.ctypes.data, case.A.ptr(), case.B.ptr(), be.ctypes.data,
-                                      c_ptr, case.D.ptr(), kind)
+            rc = self.lib.orc_execute_ex(*ts, case.prec, al.ctypes.data, case.A.ptr(), case.B.ptr(), be.ctypes.data,
+                                         c_ptr, case.D.ptr(), kind, flags)
         return rc, case.D.buf
 
     def check(self, case, exec_kind=1):
@@ -279,6 +290,16 @@ class Oracle:
         out = _OrcClasses()
         rc = self.lib.orc_classify(*self.tensors(case, keep), ctypes.byref(out))
         return rc, out
+
+
+LOWP_PRECS = (tc.F16F16_ACCUM_F32, tc.F16F16_ACCUM_F16, tc.BF16BF16_ACCUM_F32)
+
+
+def gpu_semantics_flags(case):
+    """Oracle flags for what the GPU computes: binary32 products and
+    accumulation wherever the reference would keep them in 16 bits."""
+    lowp = case.prec in LOWP_PRECS or any(o.type in (tc.F16, tc.BF16) for o in case.operands())
+    return Oracle.GPU_PRODUCTS if lowp else 0
 
 
 # --------------------------------------------------------------------------
@@ -349,6 +370,32 @@ def configs1_case(seed=1, scale=1):
     C = make_operand(tc.C64, [a, b, h, i], "abhi", seed * 4 + 2)
     D = make_operand(tc.C64, [a, b, h, i], "abhi", seed * 4 + 3)
     return Case(A, B, C, D, tc.F64F64_ACCUM_F64, 0.7 - 0.3j, 1.1 + 0.2j, name="configs1")
+
+
+def ladder_case(seed=1, v=24, o=8, prec=tc.DEFAULT_PREC, beta=0.0):
+    """The precision-ladder contraction D[a,b,i,j] = alpha sum_{c,d}
+    A[a,b,c,d] B[c,i,d,j] + beta C[a,b,i,j] (a CCSD-like particle-particle
+    ladder; B permuted so that c and d are not adjacent), dense column-major,
+    virtual extents a..d = v and occupied i, j = o.  Entries are N(0, 1)
+    exp(-|a - c| / 16) in A and N(0, 1) exp(-|c - d| / 16) in B, each scaled
+    so that max |x| <= 8."""
+    rng = np.random.default_rng(seed)
+    idx = np.arange(v, dtype=np.float64)
+    decay = np.exp(-np.abs(idx[:, None] - idx[None, :]) / 16.0)
+    a = rng.standard_normal((v, v, v, v)) * decay[:, None, :, None]          # [a, b, c, d]: decay over (a, c)
+    b = rng.standard_normal((v, o, v, o)) * decay[:, None, :, None]          # [c, i, d, j]: decay over (c, d)
+    ops = []
+    for x in (a, b):
+        m = np.abs(x).max()
+        if m > 8.0:
+            x = x * (8.0 / m)
+        ops.append(x.astype(np.float32))
+    a, b = ops
+    A = Operand(tc.F32, [v, v, v, v], layout([v] * 4)[0], labels("abcd"), 0, a.reshape(-1, order="F").copy(), 0)
+    B = Operand(tc.F32, [v, o, v, o], layout([v, o, v, o])[0], labels("cidj"), 0, b.reshape(-1, order="F").copy(), 0)
+    C = make_operand(tc.F32, [v, v, o, o], "abij", seed * 4 + 2)
+    D = make_operand(tc.F32, [v, v, o, o], "abij", seed * 4 + 3)
+    return Case(A, B, C, D, prec, 1.0, beta, name=f"ladder{v}x{o}")
 
 
 def run_reference(path, case):

@@ -251,15 +251,171 @@ def test_f64_oracle_closed_form_rank1(oracle):
     assert rc == 0 and np.allclose(d, want, rtol=1e-14, atol=0)
 
 
-def test_oracle_f16_precision_vs_reference_f16_build(oracle):
+@pytest.fixture(scope="module")
+def ref16():
     if not os.path.exists(T.REF_F16_LIB):
         pytest.skip("F16 reference build missing")
-    ref16 = tc.TappLib(T.REF_F16_LIB)
-    for seed in range(3):
-        c = T.random_case(seed + 40, "contraction", tc.F32, prec=tc.F16F16_ACCUM_F32, max_ext=6)
-        rc1, d1 = T.run_lib(ref16, c)
+    return tc.TappLib(T.REF_F16_LIB)
+
+
+F16_PINNED = [(tc.F16F16_ACCUM_F32, d) for d in (tc.F32, tc.F64, tc.C32, tc.C64, tc.F16)] + \
+             [(tc.F16F16_ACCUM_F16, d) for d in (tc.F32, tc.F64, tc.F16)] + [(tc.DEFAULT_PREC, tc.F16)]
+
+
+@pytest.mark.parametrize("prec,dtype", F16_PINNED)
+def test_oracle_f16_semantics_bit_exact_vs_reference_f16_build(oracle, ref16, prec, dtype):
+    """The binary16 branches (product.c:1094, 1956-1970, 3272-3366,
+    10412-10615; DEFAULT_PREC with binary16 storage): every scenario class,
+    with unary labels, diagonals and beta != 0, gives the same bits as the
+    reference compiled by clang with TAPP_REFERENCE_ENABLE_F16."""
+    for scen in sorted(T.SCENARIOS):
+        for seed in range(4):
+            c = T.random_case(seed * 13 + 7, scen, dtype, prec=prec, max_ext=5, repeated_D=(seed == 3))
+            rc1, d1 = T.run_lib(ref16, c)
+            rc2, d2 = oracle.run(c)
+            assert rc1 == rc2, (scen, seed)
+            assert same_bits(d1, d2), (scen, seed, T.normwise_error(d2, d1, T.written_mask(c.D)))
+
+
+def _mixed_case(i, seed, types, prec):
+    c = T.random_case(900 + i * 5 + seed, "everything", tc.F32, max_ext=4, beta_zero=(seed == 2), prec=prec)
+    for o, t in zip(c.operands(), types):
+        lo, hi = o.span()
+        o.type = t
+        o.buf = T.fill(t, max(1, hi - lo + 1), 900 + i + t + seed)
+        o.op = 1 if t in T.COMPLEX_TYPES else 0
+    c.alpha, c.beta = (0.5 + 0.25j, -0.75 + 0.5j) if types[3] in T.COMPLEX_TYPES else (0.5, -0.75)
+    if seed == 2:
+        c.beta = 0.0
+    return c
+
+
+MIXED16 = [(tc.F16, tc.F16, tc.F32, tc.F32), (tc.F16, tc.F32, tc.F32, tc.F16), (tc.F32, tc.F16, tc.F16, tc.F64),
+           (tc.F64, tc.F16, tc.F32, tc.F16), (tc.F16, tc.C32, tc.C32, tc.C32), (tc.C64, tc.F16, tc.C64, tc.C32),
+           (tc.F16, tc.F16, tc.F16, tc.F16)]
+
+
+@pytest.mark.parametrize("prec", [tc.DEFAULT_PREC, tc.F16F16_ACCUM_F32, tc.F16F16_ACCUM_F16, tc.F32F32_ACCUM_F32,
+                                  tc.F64F64_ACCUM_F64])
+def test_oracle_mixed_storage_with_binary16_vs_reference_f16_build(oracle, ref16, prec):
+    """Mixed storage types with binary16 operands: the slot types of every
+    value (sum_A in A's type under DEFAULT_PREC, binary16 under the F16
+    precisions, ...) give the reference's bits.  Left out: explicit
+    precisions with one real and one complex operand (the reference zeroes
+    sum_B with is_complex_A, product.c:381, reading or writing past the
+    allocation: undefined) and F16F16_ACCUM_F16 with complex D (a 2-byte
+    accumulator written as complex, product.c:1094)."""
+    for i, types in enumerate(MIXED16):
+        cplx = [t in T.COMPLEX_TYPES for t in types]
+        if prec != tc.DEFAULT_PREC and cplx[0] != cplx[1]:
+            continue
+        if prec == tc.F16F16_ACCUM_F16 and cplx[3]:
+            continue
+        for seed in range(3):
+            c = _mixed_case(i, seed, types, prec)
+            rc1, d1 = T.run_lib(ref16, c)
+            rc2, d2 = oracle.run(c)
+            assert rc1 == rc2 == 0, (types, seed)
+            assert same_bits(d1, d2), (types, seed, T.normwise_error(d2, d1, T.written_mask(c.D)))
+
+
+def test_f16_build_deviates_on_complex_double_D_with_real_C(oracle, ref, ref16):
+    """One path where the clang build with TAPP_REFERENCE_ENABLE_F16 differs
+    from the plain build on the same source: DEFAULT_PREC, complex binary64
+    D and a real C.  The F16 build's beta * C carries an error of about
+    1e-12 in its real part that the plain build (and plain arithmetic) does
+    not have; the oracle follows the plain build there, bit for bit."""
+    c = _mixed_case(9, 0, (tc.F32, tc.F32, tc.F32, tc.C64), tc.DEFAULT_PREC)
+    c.beta = -0.75 + 0.5j
+    _, d_plain = T.run_lib(ref, c)
+    _, d_16 = T.run_lib(ref16, c)
+    _, d_o = oracle.run(c)
+    assert same_bits(d_plain, d_o)
+    assert not same_bits(d_plain, d_16)
+    assert T.normwise_error(d_16, d_plain, T.written_mask(c.D)) < 1e-10
+
+
+def test_oracle_generic_loop_equals_typed_loops(oracle):
+    """The slot-precision loop (used whenever a value is binary16 or
+    bfloat16) on binary32 / binary64 / complex data gives the same bits as
+    the per-type loops, which are pinned to the compiled reference."""
+    for dtype in (tc.F32, tc.F64, tc.C32, tc.C64):
+        for prec in (tc.DEFAULT_PREC, tc.F32F32_ACCUM_F32, tc.F64F64_ACCUM_F64):
+            for scen in sorted(T.SCENARIOS):
+                for seed in range(2):
+                    c = T.random_case(seed * 7 + 3, scen, dtype, prec=prec, max_ext=5, repeated_D=seed == 1)
+                    r1, d1 = oracle.run(c)
+                    r2, d2 = oracle.run(c, flags=oracle.GENERIC)
+                    assert r1 == r2 and same_bits(d1, d2), (dtype, prec, scen, seed)
+
+
+TYPES4 = (tc.F32, tc.F64, tc.C32, tc.C64)
+
+
+@pytest.mark.parametrize("prec", [tc.DEFAULT_PREC, tc.F32F32_ACCUM_F32, tc.F64F64_ACCUM_F64])
+def test_oracle_mixed_storage_types_bit_exact_vs_reference(oracle, ref, prec):
+    """Every combination of F32/F64/C32/C64 storage over A, B, C and D (not
+    all equal): each value keeps the reference's slot type (sum_A in A's
+    type under DEFAULT_PREC, binary32 under F32F32_ACCUM_F32, ...), and the
+    result has the compiled reference's bits.  Explicit precisions with one
+    real and one complex operand are left out (product.c:381 zeroes sum_B
+    with is_complex_A: undefined)."""
+    import itertools
+    for i, types in enumerate(itertools.product(TYPES4, repeat=4)):
+        if len(set(types)) == 1:
+            continue
+        cplx = [t in T.COMPLEX_TYPES for t in types]
+        if prec != tc.DEFAULT_PREC and cplx[0] != cplx[1]:
+            continue
+        c = _mixed_case(i % 50, i % 3, types, prec)
+        rc1, d1 = T.run_lib(ref, c)
         rc2, d2 = oracle.run(c)
-        assert rc1 == rc2 == 0
-        # both round inputs to binary16 and accumulate in binary32; the
-        # reference's _Float16 products may keep excess precision
-        assert T.normwise_error(d2, d1, T.written_mask(c.D)) < 2e-3
+        assert rc1 == rc2 == 0 and same_bits(d1, d2), (types, T.normwise_error(d2, d1, T.written_mask(c.D)))
+
+
+def _rounded_inputs(case, rnd):
+    """The case with A, B, C, alpha and beta rounded by `rnd` (binary32 in,
+    binary32 out) and the binary32 precision: what the GPU's 16-bit forms
+    compute when no unary labels are present."""
+    c = case.copy()
+    for o in (c.A, c.B, c.C):
+        if o.type == tc.F32:
+            o.buf = rnd(o.buf).astype(np.float32)
+    c.alpha = float(rnd(np.array([c.alpha], dtype=np.float32))[0])
+    c.beta = float(rnd(np.array([c.beta], dtype=np.float32))[0])
+    c.prec = tc.F32F32_ACCUM_F32
+    return c
+
+
+@pytest.mark.parametrize("prec", [tc.F16F16_ACCUM_F32, tc.F16F16_ACCUM_F16, tc.BF16BF16_ACCUM_F32])
+def test_gpu_products_variant(oracle, ref16, prec):
+    """ORC_GPU_PRODUCTS (the GPU library's 16-bit semantics) without unary
+    labels is exactly the binary32 oracle on rounded inputs -- so it is
+    pinned through the binary32 path -- and it differs from the
+    reference's F16 semantics (per-product binary16 rounding), which the
+    faithful oracle reproduces."""
+    rnd = (lambda x: np.asarray(x, dtype=np.float32).astype(np.float16).astype(np.float32)) \
+        if prec != tc.BF16BF16_ACCUM_F32 else T.bf16_round
+    differ = 0
+    for seed in range(6):
+        c = T.random_case(seed + 40, "hadamard_contraction" if seed % 2 else "contraction", tc.F32, prec=prec,
+                          max_ext=6, beta_zero=seed == 4)
+        _, gp = oracle.run(c, flags=oracle.GPU_PRODUCTS)
+        _, f32 = oracle.run(_rounded_inputs(c, rnd))
+        assert same_bits(gp, f32), seed
+        if prec != tc.BF16BF16_ACCUM_F32:
+            _, faithful = oracle.run(c)
+            _, r16 = T.run_lib(ref16, c)
+            assert same_bits(faithful, r16)
+            differ += not same_bits(gp, faithful)
+    assert prec == tc.BF16BF16_ACCUM_F32 or differ == 6
+
+
+def test_f16_accum_f16_complex_D_is_not_written(oracle, ref16):
+    """TAPP_F16F16_ACCUM_F16 with complex D: the reference's assign_D stores
+    nothing (product.c:11096-11099), and neither does the oracle."""
+    c = T.random_case(5, "contraction", tc.C32, prec=tc.F16F16_ACCUM_F16, max_ext=4)
+    rc1, d1 = T.run_lib(ref16, c)
+    rc2, d2 = oracle.run(c)
+    assert rc1 == rc2 == 0
+    assert same_bits(d1, c.D.buf) and same_bits(d2, c.D.buf)
