@@ -40,6 +40,23 @@ template <typename R> __device__ __forceinline__ R round_mode(R x, int mode)
     return x;
 }
 
+// Round-to-nearest-even of a binary64 value to binary16 (TK_MODE_F16) or
+// bfloat16 (TK_MODE_BF16) in one step (no intermediate binary32 rounding),
+// returned as the float holding it.
+__device__ __forceinline__ float round16_d(double x, int mode)
+{
+    if (!(fabs(x) <= 1.7976931348623157e308) || x == 0.0) return (float)x;   // NaN, inf, zero
+    const int p = mode == TK_MODE_F16 ? 11 : 8, emin = mode == TK_MODE_F16 ? -14 : -126;
+    const double ovf = mode == TK_MODE_F16 ? 65520.0 : 3.3961775292304611e38;   // (2 - 2^-8) 2^127
+    int e;
+    frexp(fabs(x), &e);
+    int q = e - p;
+    if (q < emin - p + 1) q = emin - p + 1;
+    double r = ldexp(rint(ldexp(fabs(x), -q)), q);
+    if (r >= ovf) r = __builtin_inf();
+    return (float)copysign(r, x);
+}
+
 __device__ __forceinline__ float bf16_bits_to_float(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
 __device__ __forceinline__ uint16_t float_to_bf16_bits(float x) { return (uint16_t)(__float_as_uint(round_bf16(x)) >> 16); }
 
@@ -64,8 +81,8 @@ __device__ __forceinline__ void store_storage(void* base, int st, int64_t off, d
     case S_F64: ((double*)base)[off] = re; break;
     case S_C32: ((float2*)base)[off] = make_float2((float)re, (float)im); break;
     case S_C64: ((double2*)base)[off] = make_double2(re, im); break;
-    case S_F16: ((_Float16*)base)[off] = (_Float16)(float)re; break;
-    case S_BF16: ((uint16_t*)base)[off] = float_to_bf16_bits((float)re); break;
+    case S_F16: ((_Float16*)base)[off] = (_Float16)round16_d(re, TK_MODE_F16); break;
+    case S_BF16: ((uint16_t*)base)[off] = (uint16_t)(__float_as_uint(round16_d(re, TK_MODE_BF16)) >> 16); break;
     default: break;
     }
 }

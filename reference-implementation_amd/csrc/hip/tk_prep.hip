@@ -5,7 +5,8 @@
 //   The sum over u runs in ascending u, i.e. in the order of the reference's
 //   unary odometer (product.c:370-391), so the reduced value is the same
 //   number the reference's sum_A / sum_B holds when the compute type is the
-//   reference's sum type.  One lane per output; consecutive lanes take
+//   reference's sum type; a 16-bit sum (round != 0) is rounded at every
+//   addition as the reference's _Float16 sum is.  One lane per output; consecutive lanes take
 //   consecutive i0, which is the unit-stride direction the host picks.
 // k_prep2: A's and B's pre-passes in one launch.
 // k_scatter: compute-type dense buffer -> D of another storage type.
@@ -17,6 +18,7 @@ template <typename R, bool CPLX>
 __device__ __forceinline__ void prep_body(const tk_prep& p, int64_t block, int64_t nblocks)
 {
     const int64_t total = p.n0 * p.n1 * p.n2;
+    const bool wide_src = p.src_type == S_F64 || p.src_type == S_C64;
     for (int64_t o = block * 256 + threadIdx.x; o < total; o += nblocks * 256) {
         const int64_t i0 = o % p.n0, t = o / p.n0, i1 = t % p.n1, i2 = t / p.n1;
         const int64_t base = p.t0[i0] + p.t1[i1] + p.t2[i2];
@@ -33,8 +35,17 @@ __device__ __forceinline__ void prep_body(const tk_prep& p, int64_t block, int64
 #pragma unroll
             for (int j = 0; j < UB; ++j) {
                 if (u0 + j >= p.nu) break;
+                if (p.round) {
+                    // a 16-bit sum, as the reference's _Float16 / __bf16 sum_A:
+                    // sum = round16(sum + x), the addition in binary32
+                    // (binary64 for a binary64 source) and x unrounded
+                    double tr = (double)sr + re[j], ti = (double)si + (p.conj ? -im[j] : im[j]);
+                    if (!wide_src) { tr = (double)(float)tr; ti = (double)(float)ti; }
+                    sr = (R)round16_d(tr, p.round);
+                    if (CPLX) si = (R)round16_d(ti, p.round);
+                    continue;
+                }
                 R vr = (R)re[j], vi = (R)im[j];
-                if (p.round) { vr = round_mode(vr, p.round); vi = round_mode(vi, p.round); }
                 if (p.conj) vi = -vi;
                 sr += vr;
                 if (CPLX) si += vi;

@@ -228,7 +228,9 @@ int tk_problem_tables(struct tapp_plan* plan, const tk_problem* P, int device, c
                       const int64_t** mC, const int64_t** nC, const int64_t** lC)
 {
     struct tapp_plan_cache* c;
+    pthread_mutex_lock(&plan->mu);
     int rc = build_tables(plan, P, device, &c);
+    pthread_mutex_unlock(&plan->mu);
     if (rc) return rc;
     const int64_t* T = c->dev;
     *mA = T + c->off[T_mA]; *kA = T + c->off[T_kA]; *lA = T + c->off[T_lA];
@@ -454,17 +456,30 @@ int tk_execute(struct tapp_plan* p, struct tapp_exec* ex, const void* alpha, con
     return tk_execute_on(p, ex, alpha, A, B, beta, C, D, true);
 }
 
+static int execute_on(struct tapp_plan* p, struct tapp_exec* ex, const void* alpha, const void* A, const void* B,
+                      const void* beta, const void* C, void* D, bool sync);
+
 /* With sync == false every step stays ordered on ex->stream and the call
  * returns after enqueueing: host operands must stay valid, and D is written,
- * once the stream has reached this point. */
+ * once the stream has reached this point.  The plan is locked while its
+ * offset tables are built or read and the work is enqueued. */
 int tk_execute_on(struct tapp_plan* p, struct tapp_exec* ex, const void* alpha, const void* A, const void* B,
                   const void* beta, const void* C, void* D, bool sync)
+{
+    pthread_mutex_lock(&p->mu);
+    int rc = execute_on(p, ex, alpha, A, B, beta, C, D, sync);
+    pthread_mutex_unlock(&p->mu);
+    return rc;
+}
+
+static int execute_on(struct tapp_plan* p, struct tapp_exec* ex, const void* alpha, const void* A, const void* B,
+                      const void* beta, const void* C, void* D, bool sync)
 {
     tk_problem P;
     int rc = tk_plan_problem(p, ex, alpha, beta, C, &P);
     if (rc) return rc;
     if (P.empty) return 0;
-    if (!ex->stream) return device_error("executor without a device stream", 0, __LINE__);
+    if (!ex->have_stream) return device_error("executor without a device stream", 0, __LINE__);
     if (!A || !B || !D || (!P.beta_zero && !C)) return TAPP_AMD_ERR_INVALID;
     HIPCHK(hipSetDevice(ex->device));
 
@@ -601,7 +616,7 @@ int tk_execute_on(struct tapp_plan* p, struct tapp_exec* ex, const void* alpha, 
         const int64_t* tabs[3] = {DT(T_mA), DT(T_kA), DT(T_lA)};
         const int64_t sizes[3] = {M, K, L};
         const int* o = tc->ordA;
-        prA = (tk_prep){P.type[0], ct, P.op_A, (P.nUA > 1 && (P.mode == TK_MODE_F16 || P.mode == TK_MODE_BF16)) ? P.mode : 0,
+        prA = (tk_prep){P.type[0], ct, P.op_A, P.sum_round[0],
                         d[0].base, ws + off_tmp[0], sizes[o[0]], sizes[o[1]], sizes[o[2]], P.nUA,
                         tabs[o[0]], tabs[o[1]], tabs[o[2]], DT(T_uA)};
         g.A = ws + off_tmp[0]; g.mA = DT(T_dmA); g.kA = DT(T_dkA); g.lA = DT(T_dlA); g.conjA = 0;
@@ -611,7 +626,7 @@ int tk_execute_on(struct tapp_plan* p, struct tapp_exec* ex, const void* alpha, 
         const int64_t* tabs[3] = {DT(T_nB), DT(T_kB), DT(T_lB)};
         const int64_t sizes[3] = {N, K, L};
         const int* o = tc->ordB;
-        prB = (tk_prep){P.type[1], ct, P.op_B, (P.nUB > 1 && (P.mode == TK_MODE_F16 || P.mode == TK_MODE_BF16)) ? P.mode : 0,
+        prB = (tk_prep){P.type[1], ct, P.op_B, P.sum_round[1],
                         d[1].base, ws + off_tmp[1], sizes[o[0]], sizes[o[1]], sizes[o[2]], P.nUB,
                         tabs[o[0]], tabs[o[1]], tabs[o[2]], DT(T_uB)};
         g.B = ws + off_tmp[1]; g.nB = DT(T_dnB); g.kB = DT(T_dkB); g.lB = DT(T_dlB); g.conjB = 0;
@@ -621,7 +636,8 @@ int tk_execute_on(struct tapp_plan* p, struct tapp_exec* ex, const void* alpha, 
     else if (needA && K > 0) LAUNCHCHK(tk_launch_prep(&prA, st));
     else if (needB && K > 0) LAUNCHCHK(tk_launch_prep(&prB, st));
     if (needC) {
-        tk_prep pr = {P.type[2], ct, 0, 0, d[2].base, ws + off_tmp[2], M, N, L, 1,
+        /* a C rounded to 16 bits is rounded once from its storage value */
+        tk_prep pr = {P.type[2], ct, 0, P.round_C ? P.mode : 0, d[2].base, ws + off_tmp[2], M, N, L, 1,
                       DT(T_mC), DT(T_nC), DT(T_lC), DT(T_zero)};
         LAUNCHCHK(tk_launch_prep(&pr, st));
         g.C = ws + off_tmp[2]; g.mC = DT(T_dmD); g.nC = DT(T_dnD); g.lC = DT(T_dlD);

@@ -4,6 +4,7 @@
 #ifndef TAPP_AMD_INTERNAL_H_
 #define TAPP_AMD_INTERNAL_H_
 
+#include <pthread.h>
 #include <stdbool.h>
 #include <stdint.h>
 
@@ -36,6 +37,9 @@ struct tapp_plan {
     int nidx[4];               /* label count copied at creation */
     TAPP_prectype prec;
     struct tapp_plan_cache* cache;
+    /* recursive; held while a call builds or reads the cache and enqueues
+     * work that reads it, so calls on one plan from several threads are safe */
+    pthread_mutex_t mu;
 };
 
 #define TAPP_AMD_EXEC_MAGIC 0x54415050u /* "TAPP" */
@@ -50,6 +54,13 @@ struct tapp_exec {
     int cus;                   /* compute units of the device */
     hipStream_t stream;
     bool own_stream;
+    bool have_stream;          /* a stream is bound (the null stream, 0, is a valid binding) */
+    /* recursive; held by every API call that uses the context (stream,
+     * workspace, events, last-call fields), so threads sharing an executor
+     * -- or the process-wide default context of bare int executors -- take
+     * turns instead of racing on the workspace */
+    pthread_mutex_t mu;
+    bool mu_init;
     hipEvent_t ev0, ev1;
     /* device workspace: grown on demand, reused across calls */
     void* ws;
@@ -92,6 +103,10 @@ typedef struct {
     int op_A, op_B, op_C, op_D;
     bool conjD_via_f32;
     bool round_C;
+    /* sums of A's and B's unary labels held in a 16-bit format, rounded at
+     * every addition (0, TK_MODE_F16 or TK_MODE_BF16): the TAPP_F16F16_* and
+     * BF16BF16 precisions, and DEFAULT_PREC with 16-bit storage */
+    int sum_round[2];
     bool beta_zero;
     double alpha[2], beta[2];
     tk_group L, M, N, K, UA, UB;
@@ -123,6 +138,12 @@ int64_t tk_tune_bucket(int64_t x);
 bool tk_tune_get(int device, uint64_t key, int* value);
 void tk_tune_put(int device, uint64_t key, int value, const char* desc);
 double tk_tune_time(struct tapp_exec* ex, int (*launch)(void* ctx), void* ctx, int reps);
+
+/* locks (executor.c): recursive mutex set-up; executor locks are no-ops on
+ * contexts without one (the stand-in of an invalid bare int executor) */
+void tk_mutex_init_recursive(pthread_mutex_t* m);
+void tk_exec_lock(struct tapp_exec* e);
+void tk_exec_unlock(struct tapp_exec* e);
 
 /* registry of live executors / handles */
 struct tapp_exec* tk_exec_lookup(TAPP_executor e);

@@ -149,36 +149,21 @@ static void scalar_value(const void* s, int t, double* re, double* im)
     }
 }
 
-/* --- rounding of alpha/beta for the low-precision precisions -------------- */
-
-static float rnd_bf16(float x)
+/* Round-to-nearest-even of a binary64 value to binary16 (mode TK_MODE_F16) or
+ * bfloat16 (TK_MODE_BF16) in one step, as the reference's conversion of a
+ * float or double scalar to _Float16 does (create_prec_scalar, :1232-1490). */
+static double rnd16_d(double x, int mode)
 {
-    uint32_t u;
-    memcpy(&u, &x, 4);
-    if ((u & 0x7f800000u) == 0x7f800000u) {
-        if (u & 0x007fffffu) u |= 0x00400000u;
-        u &= 0xffff0000u;
-    } else {
-        u += 0x7fffu + ((u >> 16) & 1u);
-        u &= 0xffff0000u;
-    }
-    memcpy(&x, &u, 4);
-    return x;
-}
-
-static float rnd_f16(float x)
-{
-    /* round to the binary16 grid with round-to-nearest-even */
-    if (!isfinite(x)) return x;
-    float a = fabsf(x);
-    if (a >= 65520.0f) return copysignf(INFINITY, x);
+    if (!isfinite(x) || x == 0.0) return x;
+    const int p = mode == TK_MODE_F16 ? 11 : 8, emin = mode == TK_MODE_F16 ? -14 : -126;
+    const double ovf = mode == TK_MODE_F16 ? 65520.0 : ldexp(2.0 - ldexp(1.0, -8), 127);
     int e;
-    frexpf(a, &e);                      /* a = f * 2^e, f in [0.5, 1) */
-    int q = e - 11;                     /* ulp exponent: 11 significant bits */
-    if (q < -24) q = -24;               /* subnormal spacing 2^-24 */
-    float scaled = ldexpf(a, -q);
-    float r = rintf(scaled);            /* ties to even in the default mode */
-    return copysignf(ldexpf(r, q), x);
+    frexp(fabs(x), &e);
+    int q = e - p;
+    if (q < emin - p + 1) q = emin - p + 1;
+    double r = ldexp(rint(ldexp(fabs(x), -q)), q);
+    if (r >= ovf) r = INFINITY;
+    return copysign(r, x);
 }
 
 /* --- groups --------------------------------------------------------------- */
@@ -350,13 +335,20 @@ int tk_plan_problem(const struct tapp_plan* p, const struct tapp_exec* ex, const
     double ar, ai;
     scalar_value(alpha, D->type, &ar, &ai);
     if (!cplx) { ai = 0; bi = 0; }
-    if (!wide) {
+    if (out->mode == TK_MODE_BF16 || out->mode == TK_MODE_F16) {
+        /* one rounding from the value read as type_D */
+        ar = rnd16_d(ar, out->mode); ai = rnd16_d(ai, out->mode);
+        br = rnd16_d(br, out->mode); bi = rnd16_d(bi, out->mode);
+    } else if (!wide) {
         ar = (float)ar; ai = (float)ai; br = (float)br; bi = (float)bi;
     }
-    if (out->mode == TK_MODE_BF16) {
-        ar = rnd_bf16((float)ar); ai = rnd_bf16((float)ai); br = rnd_bf16((float)br); bi = rnd_bf16((float)bi);
-    } else if (out->mode == TK_MODE_F16) {
-        ar = rnd_f16((float)ar); ai = rnd_f16((float)ai); br = rnd_f16((float)br); bi = rnd_f16((float)bi);
+    /* the 16-bit sums of unary labels (alloc_val :1104-1230: _Float16 /
+     * __bf16 under the 16-bit precisions, the operand's own type under
+     * DEFAULT_PREC) */
+    for (int x = 0; x < 2; x++) {
+        if (out->mode == TK_MODE_F16 || out->mode == TK_MODE_BF16) out->sum_round[x] = out->mode;
+        else if (p->prec == TAPP_DEFAULT_PREC && v[x].type == T_F16) out->sum_round[x] = TK_MODE_F16;
+        else if (p->prec == TAPP_DEFAULT_PREC && v[x].type == T_BF16) out->sum_round[x] = TK_MODE_BF16;
     }
     out->alpha[0] = ar; out->alpha[1] = ai;
     out->beta[0] = br; out->beta[1] = bi;
@@ -384,8 +376,11 @@ int tk_plan_problem(const struct tapp_plan* p, const struct tapp_exec* ex, const
         if (findc(&ca, cb.idx[i]) < 0 && findc(&cd, cb.idx[i]) < 0)
             add_mode(&out->UB, cb.ext[i], 0, cb.str[i], 0, 0);
 
-    /* any zero extent among D's modes: nothing is written */
-    out->empty = group_size(&out->L) == 0 || group_size(&out->M) == 0 || group_size(&out->N) == 0;
+    /* any zero extent among D's modes: nothing is written; nor with
+     * TAPP_F16F16_ACCUM_F16 and a complex D, whose assign_D stores nothing
+     * (product.c:11096-11099) */
+    out->empty = group_size(&out->L) == 0 || group_size(&out->M) == 0 || group_size(&out->N) == 0 ||
+                 (p->prec == TAPP_F16F16_ACCUM_F16 && tk_type_complex(D->type));
 
     simplify_group(&out->L, true, out->pin);
     simplify_group(&out->M, true, out->pin);
