@@ -26,6 +26,20 @@ enum {
     TAPP_AMD_ERR_BUSY = 107         /* serving queue full */
 };
 
+/* Threads and ordering.
+ *   - TAPP_execute_product may be called from several threads at once, with
+ *     the same or different plans and executors.  An executor is one device
+ *     context (stream + workspace): calls sharing it -- including every call
+ *     made with a bare int executor, which share one process-wide context --
+ *     run one at a time.  Use one executor per thread for concurrency.
+ *   - Each executor launches on its own non-blocking stream, which does not
+ *     wait for work queued on other streams (the null stream included).
+ *     Device operands must be complete when the call is made: synchronize
+ *     the producing stream first, or bind that stream with
+ *     TAPP_AMD_ATTR_STREAM (the null stream, 0, is a valid binding), after
+ *     which the call's kernels run in order after the work queued on it.
+ *     The call still returns only when D is written. */
+
 /* Attribute keys for TAPP_attr_set / TAPP_attr_get / TAPP_attr_clear on an
  * executor (pass the TAPP_executor as the TAPP_attr).  Values are passed by
  * pointer to an int unless stated. */
@@ -56,7 +70,9 @@ enum {
      * measured with HIP events on the executor's stream (double). */
     TAPP_AMD_ATTR_LAST_DEVICE_US = 0x1004,
     /* The HIP stream (hipStream_t) the executor launches on; set to bind an
-     * external stream, get to read it.  Value: pointer to a void*. */
+     * external stream (0 = the null stream), get to read it.  The executor
+     * waits for its own stream before switching.  Value: pointer to a
+     * void*. */
     TAPP_AMD_ATTR_STREAM = 0x1005,
     /* 1: time every execute with HIP events (needed for LAST_DEVICE_US). */
     TAPP_AMD_ATTR_TIMING = 0x1006,
@@ -98,9 +114,12 @@ enum {
  * TAPP_execute_batched_product (product.h:66-75) and defines nothing.
  *
  * create: validates every product exactly as TAPP_execute_product would and
- *   returns the first failing code; reads alpha[i]/beta[i] now (values are
- *   kept); keeps the operand pointers.  Layouts and data pointers must not
- *   change while the group exists (the group replays the same addresses).
+ *   returns the first failing code; reads alpha[i]/beta[i] now (the values
+ *   are copied: the caller's scalars may be freed after create); keeps the
+ *   operand pointers.  Layouts and data pointers must not change while the
+ *   group exists (the group replays the same addresses), and the plans and
+ *   their tensor infos must outlive the group: its launch reads the plans'
+ *   device offset tables, and products run one by one execute their plan.
  * execute: runs every product; returns when all D are written.
  * Products outside the persistent kernel's scope (non-FP32 compute, unary
  * labels, storage conversion, host pointers) run one by one inside execute. */
@@ -167,7 +186,10 @@ TAPP_EXPORT TAPP_error TAPP_AMD_destroy_server(TAPP_AMD_server server);
  * with `handle` -- TAPP_execute_product, product groups, the serving loop --
  * that passes, as A, B or C, a host pointer whose accessed bytes lie inside
  * [host, host + bytes) reads the device copy instead of staging over PCIe.
- * The copy is taken now and refreshed only by TAPP_AMD_update_resident; D is
+ * The copy is taken now and refreshed only by TAPP_AMD_update_resident, which
+ * first waits for every kernel already queued on the device (so none reads a
+ * half-updated copy; requests queued but not yet launched read the new
+ * contents); D is
  * always written through the caller's pointer.  Ranges of one handle must not
  * overlap.  Release (or destroy the handle) only after the requests and
  * product groups that read a copy have completed or been destroyed.

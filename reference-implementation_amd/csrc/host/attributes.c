@@ -36,11 +36,16 @@ TAPP_error TAPP_attr_set(TAPP_attr attr, TAPP_key key, void* value)
     }
     case TAPP_AMD_ATTR_SELF_CHECK: e->kind = *(int*)value ? 12 : 1; return 0;
     case TAPP_AMD_ATTR_STREAM: {
+        /* any stream, the null stream (0) included: later calls run ordered
+         * after the work already queued on it */
         hipStream_t s = *(hipStream_t*)value;
-        if (e->stream) hipStreamSynchronize(e->stream);
+        tk_exec_lock(e);
+        if (e->have_stream) hipStreamSynchronize(e->stream);
         if (e->own_stream && e->stream) hipStreamDestroy(e->stream);
         e->stream = s;
         e->own_stream = false;
+        e->have_stream = true;
+        tk_exec_unlock(e);
         return 0;
     }
     default: return TAPP_AMD_ERR_KEY;

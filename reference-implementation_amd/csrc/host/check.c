@@ -44,7 +44,7 @@ int tk_execute_checked(struct tapp_plan* p, struct tapp_exec* ex, const void* al
     int rc = tk_plan_problem(p, ex, alpha, beta, C, &P);
     if (rc) return rc;
     if (P.empty) return 0;
-    if (!ex->stream) return TAPP_AMD_ERR_DEVICE;
+    if (!ex->have_stream) return TAPP_AMD_ERR_DEVICE;
     if (!A || !B || !D || (!P.beta_zero && !C)) return TAPP_AMD_ERR_INVALID;
     if (hipSetDevice(ex->device) != hipSuccess) return TAPP_AMD_ERR_DEVICE;
 

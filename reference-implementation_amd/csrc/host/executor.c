@@ -9,6 +9,7 @@
  * executors from a caller's bare int: the latter runs on a process-wide
  * default context.
  */
+#define _GNU_SOURCE   /* PTHREAD_MUTEX_RECURSIVE */
 #include <pthread.h>
 #include <stdlib.h>
 
@@ -46,10 +47,31 @@ struct tapp_exec* tk_exec_lookup(TAPP_executor h)
     return found;
 }
 
+void tk_mutex_init_recursive(pthread_mutex_t* m)
+{
+    pthread_mutexattr_t a;
+    pthread_mutexattr_init(&a);
+    pthread_mutexattr_settype(&a, PTHREAD_MUTEX_RECURSIVE);
+    pthread_mutex_init(m, &a);
+    pthread_mutexattr_destroy(&a);
+}
+
+void tk_exec_lock(struct tapp_exec* e)
+{
+    if (e && e->mu_init) pthread_mutex_lock(&e->mu);
+}
+
+void tk_exec_unlock(struct tapp_exec* e)
+{
+    if (e && e->mu_init) pthread_mutex_unlock(&e->mu);
+}
+
 static int exec_init(struct tapp_exec* e)
 {
     e->kind = 1;
     e->magic = TAPP_AMD_EXEC_MAGIC;
+    tk_mutex_init_recursive(&e->mu);
+    e->mu_init = true;
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return TAPP_AMD_ERR_DEVICE;
     e->device = dev;
@@ -57,6 +79,7 @@ static int exec_init(struct tapp_exec* e)
         e->cus = 256;
     if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) return TAPP_AMD_ERR_DEVICE;
     e->own_stream = true;
+    e->have_stream = true;
     if (hipEventCreate(&e->ev0) != hipSuccess || hipEventCreate(&e->ev1) != hipSuccess) return TAPP_AMD_ERR_DEVICE;
     return 0;
 }
@@ -86,6 +109,7 @@ TAPP_error TAPP_create_executor(TAPP_executor* exec)
         e->kind = 1;
         e->stream = NULL;
         e->own_stream = false;
+        e->have_stream = false;
     }
     tk_exec_register(e);
     *exec = (TAPP_executor)e;
@@ -97,11 +121,15 @@ TAPP_error TAPP_destroy_executor(TAPP_executor exec)
     struct tapp_exec* e = tk_exec_lookup(exec);
     if (!e) return 0;
     tk_exec_unregister(e);
-    if (e->stream) hipStreamSynchronize(e->stream);
+    /* waits for a call another thread may still be making on it */
+    tk_exec_lock(e);
+    if (e->have_stream) hipStreamSynchronize(e->stream);
     if (e->ws) hipFree(e->ws);
     if (e->ev0) hipEventDestroy(e->ev0);
     if (e->ev1) hipEventDestroy(e->ev1);
     if (e->own_stream && e->stream) hipStreamDestroy(e->stream);
+    tk_exec_unlock(e);
+    if (e->mu_init) pthread_mutex_destroy(&e->mu);
     free(e);
     return 0;
 }

@@ -40,8 +40,11 @@ struct tapp_group {
     /* products executed one by one */
     int nrest;
     struct tapp_plan** rest_plan;
-    const void **rest_alpha, **rest_A, **rest_B, **rest_beta, **rest_C;
+    const void **rest_A, **rest_B, **rest_C;
     void** rest_D;
+    /* alpha and beta of each, copied at creation (read as type_D: at most
+     * 16 bytes, a binary64 complex) */
+    unsigned char (*rest_scalar)[2][16];
 };
 
 
@@ -306,10 +309,15 @@ static void group_free(struct tapp_group* g)
     if (!g) return;
     if (g->have_graph) { hipGraphExecDestroy(g->gexec); hipGraphDestroy(g->graph); }
     if (g->dmem) hipFree(g->dmem);
-    free(g->rest_plan); free(g->rest_alpha); free(g->rest_A); free(g->rest_B);
-    free(g->rest_beta); free(g->rest_C); free(g->rest_D);
+    free(g->rest_plan); free(g->rest_A); free(g->rest_B);
+    free(g->rest_C); free(g->rest_D); free(g->rest_scalar);
     free(g);
 }
+
+static int create_group_on(TAPP_AMD_product_group* group, struct tapp_exec* ex, int count,
+                           const TAPP_tensor_product* plans, const void* const* alpha, const void* const* A,
+                           const void* const* B, const void* const* beta, const void* const* C, void* const* D,
+                           bool capture);
 
 static int create_group(TAPP_AMD_product_group* group, TAPP_executor exec, int count,
                         const TAPP_tensor_product* plans, const void* const* alpha, const void* const* A,
@@ -322,21 +330,30 @@ static int create_group(TAPP_AMD_product_group* group, TAPP_executor exec, int c
     if (!ex) ex = (exec && (*(const int*)exec == 1 || *(const int*)exec == 2 || *(const int*)exec == 12))
                       ? tk_default_exec() : NULL;
     if (!ex) return 33;
-    if (!ex->stream) return TAPP_AMD_ERR_DEVICE;
+    tk_exec_lock(ex);
+    int rc = create_group_on(group, ex, count, plans, alpha, A, B, beta, C, D, capture);
+    tk_exec_unlock(ex);
+    return rc;
+}
+
+static int create_group_on(TAPP_AMD_product_group* group, struct tapp_exec* ex, int count,
+                           const TAPP_tensor_product* plans, const void* const* alpha, const void* const* A,
+                           const void* const* B, const void* const* beta, const void* const* C, void* const* D,
+                           bool capture)
+{
+    if (!ex->have_stream) return TAPP_AMD_ERR_DEVICE;
     struct tapp_group* g = calloc(1, sizeof *g);
     if (!g) return TAPP_AMD_ERR_NOMEM;
     g->magic = TAPP_AMD_EXEC_MAGIC;
     g->ex = ex;
     ranked* list = malloc(sizeof(ranked) * (size_t)(count > 0 ? count : 1));
     g->rest_plan = malloc(sizeof(void*) * (size_t)(count > 0 ? count : 1));
-    g->rest_alpha = malloc(sizeof(void*) * (size_t)(count > 0 ? count : 1));
     g->rest_A = malloc(sizeof(void*) * (size_t)(count > 0 ? count : 1));
     g->rest_B = malloc(sizeof(void*) * (size_t)(count > 0 ? count : 1));
-    g->rest_beta = malloc(sizeof(void*) * (size_t)(count > 0 ? count : 1));
     g->rest_C = malloc(sizeof(void*) * (size_t)(count > 0 ? count : 1));
     g->rest_D = malloc(sizeof(void*) * (size_t)(count > 0 ? count : 1));
-    if (!list || !g->rest_plan || !g->rest_alpha || !g->rest_A || !g->rest_B || !g->rest_beta || !g->rest_C ||
-        !g->rest_D) {
+    g->rest_scalar = calloc((size_t)(count > 0 ? count : 1), sizeof *g->rest_scalar);
+    if (!list || !g->rest_plan || !g->rest_A || !g->rest_B || !g->rest_C || !g->rest_D || !g->rest_scalar) {
         free(list); group_free(g);
         return TAPP_AMD_ERR_NOMEM;
     }
@@ -359,8 +376,11 @@ static int create_group(TAPP_AMD_product_group* group, TAPP_executor exec, int c
         } else {
             if (rc) { free(list); group_free(g); return rc; }
             int r = g->nrest++;
-            g->rest_plan[r] = p; g->rest_alpha[r] = alpha[i]; g->rest_A[r] = A[i]; g->rest_B[r] = B[i];
-            g->rest_beta[r] = beta[i]; g->rest_C[r] = c; g->rest_D[r] = D[i];
+            g->rest_plan[r] = p; g->rest_A[r] = A[i]; g->rest_B[r] = B[i];
+            g->rest_C[r] = c; g->rest_D[r] = D[i];
+            const size_t ss = tk_type_size(P.type[3]);
+            if (alpha[i] && ss) memcpy(g->rest_scalar[r][0], alpha[i], ss);
+            if (beta[i] && ss) memcpy(g->rest_scalar[r][1], beta[i], ss);
         }
     }
     if (ex->autotune && n > 0) {
@@ -449,6 +469,8 @@ TAPP_error TAPP_AMD_create_product_group(TAPP_AMD_product_group* group, TAPP_exe
     return create_group(group, exec, count, plans, alpha, A, B, beta, C, D, true);
 }
 
+static int execute_group(struct tapp_group* g);
+
 /* TAPP_execute_batched_product (product.h:66-75): one plan, num_batches
  * operand sets, one alpha and beta, run as one persistent launch. */
 TAPP_error TAPP_execute_batched_product(TAPP_tensor_product plan, TAPP_executor exec, TAPP_status* status,
@@ -476,6 +498,14 @@ TAPP_error TAPP_AMD_execute_product_group(TAPP_AMD_product_group group)
 {
     struct tapp_group* g = (struct tapp_group*)group;
     if (!g || g->magic != TAPP_AMD_EXEC_MAGIC) return TAPP_AMD_ERR_INVALID;
+    tk_exec_lock(g->ex);
+    int rc = execute_group(g);
+    tk_exec_unlock(g->ex);
+    return rc;
+}
+
+static int execute_group(struct tapp_group* g)
+{
     struct tapp_exec* ex = g->ex;
     if (hipSetDevice(ex->device) != hipSuccess) return TAPP_AMD_ERR_DEVICE;
     if (ex->timing) hipEventRecord(ex->ev0, ex->stream);
@@ -489,8 +519,8 @@ TAPP_error TAPP_AMD_execute_product_group(TAPP_AMD_product_group group)
     }
     if (ex->timing) hipEventRecord(ex->ev1, ex->stream);
     for (int r = 0; r < g->nrest; r++) {
-        int rc = tk_execute(g->rest_plan[r], ex, g->rest_alpha[r], g->rest_A[r], g->rest_B[r], g->rest_beta[r],
-                            g->rest_C[r], g->rest_D[r]);
+        int rc = tk_execute(g->rest_plan[r], ex, g->rest_scalar[r][0], g->rest_A[r], g->rest_B[r],
+                            g->rest_scalar[r][1], g->rest_C[r], g->rest_D[r]);
         if (rc) return rc;
     }
     if (hipStreamSynchronize(ex->stream) != hipSuccess) return TAPP_AMD_ERR_DEVICE;
@@ -507,7 +537,7 @@ TAPP_error TAPP_AMD_destroy_product_group(TAPP_AMD_product_group group)
 {
     struct tapp_group* g = (struct tapp_group*)group;
     if (!g) return 0;
-    if (g->ex && g->ex->stream) hipStreamSynchronize(g->ex->stream);
+    if (g->ex && g->ex->have_stream) hipStreamSynchronize(g->ex->stream);
     g->magic = 0;
     group_free(g);
     return 0;

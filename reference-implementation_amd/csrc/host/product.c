@@ -12,6 +12,11 @@
  *   storage conversion) -> GETT kernel -> optional scatter to D's storage
  *   type -> D staged back if it is host memory.  The call returns when D is
  *   written, as the reference's does.
+ *
+ * Threads: the executor's context (stream, workspace) is locked for the
+ * call and the plan while its tables are built and read, so concurrent calls
+ * are safe -- on one executor, including the shared default context of bare
+ * int executors, they take turns.
  */
 #include <stdlib.h>
 #include <string.h>
@@ -44,6 +49,7 @@ TAPP_error TAPP_create_tensor_product(TAPP_tensor_product* plan, TAPP_handle han
         if (n > 0) memcpy(p->idx[x], idx[x], sizeof(int64_t) * (size_t)n);
     }
     p->prec = prec;
+    tk_mutex_init_recursive(&p->mu);
     *plan = (TAPP_tensor_product)p;
     return 0;
 }
@@ -54,6 +60,7 @@ TAPP_error TAPP_destroy_tensor_product(TAPP_tensor_product plan)
     if (!p) return 0;
     for (int x = 0; x < 4; x++) free(p->idx[x]);
     tk_plan_cache_free(p->cache);
+    pthread_mutex_destroy(&p->mu);
     free(p);
     return 0;
 }
@@ -88,8 +95,12 @@ TAPP_error TAPP_execute_product(TAPP_tensor_product plan, TAPP_executor exec, TA
     /* kind 12: every product computed twice and compared (check.c); the
      * kind is the int the executor points at, for this library's executors
      * their first field */
-    if (ex && *(const int*)exec == 12) return tk_execute_checked(p, ex, alpha, A, B, beta, C, D);
-    return tk_execute(p, ex, alpha, A, B, beta, C, D);
+    int rc;
+    tk_exec_lock(ex);
+    if (ex && *(const int*)exec == 12) rc = tk_execute_checked(p, ex, alpha, A, B, beta, C, D);
+    else rc = tk_execute(p, ex, alpha, A, B, beta, C, D);
+    tk_exec_unlock(ex);
+    return rc;
 }
 
 TAPP_error TAPP_AMD_plan_info(TAPP_tensor_product plan, TAPP_executor exec, const void* beta, const void* C,

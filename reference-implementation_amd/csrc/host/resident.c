@@ -152,7 +152,18 @@ TAPP_error TAPP_AMD_update_resident(TAPP_handle handle, const void* host)
     if (h) {
         struct resident* r = (struct resident*)h->res;
         for (int i = 0; i < h->nres; i++)
-            if (r[i].host == (const char*)host) { rc = copy_in(h->device, r[i].dev, host, r[i].bytes); break; }
+            if (r[i].host == (const char*)host) {
+                /* kernels already queued on any stream (executors, serving
+                 * slots) may still read the copy: let them finish first, as
+                 * release does, so none sees a half-updated operand */
+                if (hipSetDevice(h->device) != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
+                    (void)hipGetLastError();
+                    rc = TAPP_AMD_ERR_DEVICE;
+                    break;
+                }
+                rc = copy_in(h->device, r[i].dev, host, r[i].bytes);
+                break;
+            }
     }
     pthread_mutex_unlock(&mu);
     return rc;

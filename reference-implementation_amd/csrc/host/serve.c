@@ -355,7 +355,7 @@ TAPP_error TAPP_AMD_create_server(TAPP_AMD_server* server, TAPP_executor exec, s
     sv->large_flop = (int64_t)1 << 31;
     sv->heavy_bytes = (size_t)4 << 20;
     sv->paused = (flags & TAPP_AMD_SERVER_PAUSED) != 0;
-    if (!base->stream || hipDeviceGetAttribute(&sv->cus, hipDeviceAttributeMultiprocessorCount, base->device) !=
+    if (!base->have_stream || hipDeviceGetAttribute(&sv->cus, hipDeviceAttributeMultiprocessorCount, base->device) !=
                              hipSuccess || sv->cus <= 0) {
         (void)hipGetLastError();
         sv->cus = 256;
@@ -371,7 +371,7 @@ TAPP_error TAPP_AMD_create_server(TAPP_AMD_server* server, TAPP_executor exec, s
         e->cus = base->cus;
         e->split_bf16 = base->split_bf16;
         e->force_kernel = base->force_kernel;
-        if (base->stream) {
+        if (base->have_stream) {
             if (hipSetDevice(base->device) != hipSuccess ||
                 hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess ||
                 hipEventCreateWithFlags(&sv->slot[i].done, hipEventDisableTiming) != hipSuccess ||
@@ -382,9 +382,10 @@ TAPP_error TAPP_AMD_create_server(TAPP_AMD_server* server, TAPP_executor exec, s
                 return TAPP_AMD_ERR_DEVICE;
             }
             e->own_stream = true;
+            e->have_stream = true;
         }
     }
-    if (!base->stream) sv->paused = true;   /* no device: queue-only (admission logic) */
+    if (!base->have_stream) sv->paused = true;   /* no device: queue-only (admission logic) */
     if (pthread_create(&sv->thread, NULL, dispatcher, sv) != 0) {
         TAPP_AMD_destroy_server((TAPP_AMD_server)sv);
         return TAPP_AMD_ERR_NOMEM;

@@ -40,7 +40,7 @@ def assert_parity(got, want, case, tol=None):
 
 
 def run_both(lib, ex, oracle, case, f64_oracle=False):
-    rc_o, d_o = oracle.run(case, f64=f64_oracle)
+    rc_o, d_o = oracle.run(case, f64=f64_oracle, gpu=True)
     rc_g, d_g = T.run_lib(lib, case, ex)
     assert rc_g == rc_o, (case.name, rc_g, rc_o, lib.explain(rc_g))
     return rc_o, d_g, d_o
@@ -206,7 +206,7 @@ def test_low_precision_kernel_load_modes(lib, oracle, mode):
                 _, exact = oracle.run(c, f64=True)
                 assert T.normwise_error(got, exact, T.written_mask(c.D)) < SPLIT_TOL[mode], (M, N, K, tA, tB, pad)
             else:
-                _, want = oracle.run(c)
+                _, want = oracle.run(c, gpu=True)
                 assert_parity(got, want, c, tol=1e-5)
     lib.destroy_executor(ex)
 
@@ -415,6 +415,54 @@ def test_low_precision_forms(lib, gpu_exec, oracle, prec):
         assert T.normwise_error(got, exact, mask) <= 8 * bound
 
 
+def _abs_case(case):
+    """The case with |alpha|, |beta| and every element replaced by its
+    magnitude (complex: modulus) in binary64: its FP64 result is
+    |beta| |C| + sum_k |alpha| |sum_A| |sum_B| elementwise, the scale of a
+    componentwise error bound."""
+    c = case.copy()
+    for o in c.operands():
+        v = np.abs(o.buf.astype(np.float32) if o.type == tc.F16 else
+                   ((o.buf.astype(np.uint32) << 16).view(np.float32) if o.type == tc.BF16 else o.buf))
+        o.buf = v.astype(np.float64)
+        o.type = tc.F64
+        o.op = 0
+    c.alpha, c.beta = abs(c.alpha), abs(c.beta)
+    c.prec = tc.F64F64_ACCUM_F64
+    return c
+
+
+@pytest.mark.parametrize("prec", [tc.F16F16_ACCUM_F32, tc.F16F16_ACCUM_F16])
+@pytest.mark.parametrize("scenario", ["contraction", "unary", "everything", "hadamard_contraction"])
+def test_f16_precisions_against_reference_semantics(lib, gpu_exec, oracle, prec, scenario):
+    """TAPP_F16F16_ACCUM_F32 / _F16 on the GPU against the oracle's
+    restatement of the reference (bit-exact to its clang F16 build), which
+    rounds every alpha * sum_A * sum_B to binary16 and, for ACCUM_F16,
+    accumulates in binary16.  The GPU rounds the inputs and the unary sums
+    exactly as the reference does (checked at 1e-5 against the oracle's GPU
+    form) but keeps products and sums in binary32, which datatype.h:85-87
+    allows.  Elementwise bound: |D_gpu - D_ref| <= 2 (2 u16 + gamma_K(u_acc)
+    + gamma_K(u32)) (|beta C| + sum_k |alpha sum_A sum_B|), u_acc the
+    reference's accumulator unit."""
+    u16, u32 = 2.0 ** -11, 2.0 ** -24
+    for seed in range(3):
+        c = T.random_case(7100 + 17 * seed, scenario, tc.F32, prec=prec, max_ext=6, min_ext=2, beta_zero=seed == 2)
+        rc, got = T.run_lib(lib, c, gpu_exec)
+        assert rc == 0
+        _, gpu_form = oracle.run(c, gpu=True)
+        _, ref_form = oracle.run(c)
+        mask = T.written_mask(c.D)
+        assert T.normwise_error(got, gpu_form, mask) <= 1e-5, c.name
+        _, mag = oracle.run(_abs_case(c), f64=True)
+        K = max(1, int(np.prod([e for e, l in zip(c.A.extents, c.A.idx) if l in c.B.idx and l not in c.D.idx]
+                                or [1])))
+        gam = lambda u: K * u / (1 - K * u)
+        u_acc = u16 if prec == tc.F16F16_ACCUM_F16 else u32
+        bound = 2 * (2 * u16 + gam(u_acc) + gam(u32)) * np.asarray(mag)[mask] + 1e-30
+        diff = np.abs(np.asarray(got)[mask].astype(np.float64) - np.asarray(ref_form)[mask].astype(np.float64))
+        assert np.all(diff <= bound), (c.name, float(np.max(diff / bound)))
+
+
 def test_split_bf16_accuracy(lib, oracle):
     _, ex = lib.create_executor()
     assert lib.attr_set_int(ex, tc.ATTR_SPLIT_BF16, 1) == 0
@@ -445,7 +493,7 @@ def test_batched_product_equals_single_calls(lib, gpu_exec, oracle):
                              be.ctypes.data, [b.C.ptr() for b in bufs], [b.D.ptr() for b in bufs])
     assert rc == 0
     for b, c in zip(bufs, cases):
-        _, want = oracle.run(c)
+        _, want = oracle.run(c, gpu=True)
         assert_parity(b.D.buf, want, c)
     lib.destroy_plan(plan)
     for h in infos:
@@ -503,7 +551,7 @@ def test_layout_change_after_plan_creation(lib, gpu_exec, oracle):
         rc = lib.execute(plan, gpu_exec, al.ctypes.data, cc.A.ptr(), cc.B.ptr(), be.ctypes.data, cc.C.ptr(),
                          cc.D.ptr())
         assert rc == 0
-        _, want = oracle.run(cc)
+        _, want = oracle.run(cc, gpu=True)
         assert_parity(cc.D.buf, want, cc)
     lib.destroy_plan(plan)
     for h in infos:
@@ -516,7 +564,7 @@ def test_reference_style_executor_int(lib, oracle):
     c = T.random_case(3, "contraction", tc.F32)
     kind = ctypes.c_int(1)
     rc, got = T.run_lib(lib, c, ctypes.addressof(kind))
-    _, want = oracle.run(c)
+    _, want = oracle.run(c, gpu=True)
     assert rc == 0
     assert_parity(got, want, c)
     bad = ctypes.c_int(5)
@@ -566,7 +614,7 @@ def test_beta_zero_ignores_nan_in_c_in_group(lib, gpu_exec, oracle):
     assert rc == 0
     assert lib.execute_group(g) == 0
     for d in dcs:
-        _, want = oracle.run(d.case)
+        _, want = oracle.run(d.case, gpu=True)
         got = d.d()
         assert np.all(np.isfinite(got[T.written_mask(d.case.D)]))
         assert_parity(got, want, d.case)

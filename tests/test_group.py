@@ -79,7 +79,7 @@ def test_group_matches_single_calls_and_oracle(lib, gpu_exec, oracle):
         d.reset_d()
         assert d.run_single(gpu_exec) == 0
         single = d.d()
-        _, want = oracle.run(d.case)
+        _, want = oracle.run(d.case, gpu=True)
         assert_parity(got, want, d.case)
         assert_parity(single, want, d.case)
     lib.destroy_group(g)
@@ -106,7 +106,7 @@ def test_group_graph_replays(lib, gpu_exec, oracle):
         torch.cuda.synchronize()
         assert lib.execute_group(g) == 0
         for d in dcs:
-            _, want = oracle.run(d.case)
+            _, want = oracle.run(d.case, gpu=True)
             assert_parity(d.d(), want, d.case)
     lib.destroy_group(g)
     for d in dcs:
@@ -121,7 +121,7 @@ def test_group_bf16_products(lib, gpu_exec, oracle):
     assert rc == 0
     assert lib.execute_group(g) == 0
     for d in dcs:
-        _, want = oracle.run(d.case)
+        _, want = oracle.run(d.case, gpu=True)
         assert_parity(d.d(), want, d.case, tol=1e-5)
     lib.destroy_group(g)
     for d in dcs:
@@ -157,7 +157,7 @@ def test_batched_product_device_pointers(lib, gpu_exec, oracle):
     assert rc == 0
     assert lib.attr_get_int(gpu_exec, tc.ATTR_LAST_KERNEL)[1] == tc.KERNEL_GROUP
     for d in sets:
-        _, want = oracle.run(d.case)
+        _, want = oracle.run(d.case, gpu=True)
         assert_parity(d.d(), want, d.case)
     for d in sets:
         d.close()

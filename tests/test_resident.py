@@ -71,7 +71,7 @@ def test_resident_operand_is_read_from_its_copy(lib, gpu_exec, oracle):
     # overlapping range on the same handle
     assert lib.make_resident(h, A.buf.ctypes.data + 4, 8) == tc.AMD_ERR_INVALID
 
-    _, want_old = oracle.run(hp.case)
+    _, want_old = oracle.run(hp.case, gpu=True)
     assert hp.run(gpu_exec) == 0
     got = hp.case.D.buf.copy()
     assert T.normwise_error(got, want_old, T.written_mask(hp.case.D)) <= 1e-5
@@ -82,7 +82,7 @@ def test_resident_operand_is_read_from_its_copy(lib, gpu_exec, oracle):
     assert np.array_equal(hp.case.D.buf, got), "resident copy must be read, not the host buffer"
 
     assert lib.update_resident(h, A.buf.ctypes.data) == 0  # now the copy follows
-    _, want_new = oracle.run(hp.case)
+    _, want_new = oracle.run(hp.case, gpu=True)
     assert hp.run(gpu_exec) == 0
     new = hp.case.D.buf.copy()
     assert T.normwise_error(new, want_new, T.written_mask(hp.case.D)) <= 1e-5
@@ -110,7 +110,7 @@ def test_resident_sub_range_and_plan_without_handle(lib, gpu_exec, oracle):
     for handle in (h, 0):
         hp = HostPlan(lib, case, handle, copy=False)
         hp.d0 = hp.case.D.buf.copy()
-        rc_o, want = oracle.run(hp.case)
+        rc_o, want = oracle.run(hp.case, gpu=True)
         assert hp.run(gpu_exec) == rc_o == 0
         assert T.normwise_error(hp.case.D.buf, want, T.written_mask(hp.case.D)) <= 1e-12
         hp.close()
@@ -141,7 +141,7 @@ def test_resident_operands_in_group_and_server(lib, gpu_exec, oracle):
     assert lib.group_info(g)[1][0] == len(hps), "resident operands run in the persistent launch"
     assert lib.execute_group(g) == 0
     for hp, dt in zip(hps, dts):
-        _, want = oracle.run(hp.case)
+        _, want = oracle.run(hp.case, gpu=True)
         got = dt.cpu().numpy().view(hp.case.D.buf.dtype)
         assert T.normwise_error(got, want, T.written_mask(hp.case.D)) <= 1e-5
     assert lib.destroy_group(g) == 0
@@ -163,7 +163,7 @@ def test_resident_operands_in_group_and_server(lib, gpu_exec, oracle):
         assert lib.wait(srv, st) == (0, 0)
         lib.destroy_status(st)
     for hp, dt in zip(hps, dts):
-        _, want = oracle.run(hp.case)
+        _, want = oracle.run(hp.case, gpu=True)
         got = dt.cpu().numpy().view(hp.case.D.buf.dtype)
         assert T.normwise_error(got, want, T.written_mask(hp.case.D)) <= 1e-5
     assert lib.destroy_server(srv) == 0

@@ -84,7 +84,7 @@ def test_results_same_alone_and_in_the_mix(lib, gpu_exec, oracle):
     rc, stats = lib.server_stats(srv)
     assert stats["completed"] == len(dcs) and stats["persistent_products"] > 0, stats
     for d, a in zip(dcs, alone):
-        _, want = oracle.run(d.case)
+        _, want = oracle.run(d.case, gpu=True)
         assert_parity(d.d(), want, d.case)
         assert_parity(d.d(), a, d.case)
     lib.destroy_server(srv)
@@ -119,7 +119,7 @@ def test_budget_stalls_then_completes(lib, gpu_exec, oracle):
     rc, stats = lib.server_stats(srv)
     assert stats["completed"] == 8 and stats["stalls"] > 0, stats
     for c2, plan, infos, al, be in keep:
-        _, want = oracle.run(c2)
+        _, want = oracle.run(c2, gpu=True)
         assert_parity(c2.D.buf, want, c2)
         _free(lib, plan, infos)
     lib.destroy_server(srv)
@@ -195,7 +195,7 @@ def test_heavy_requests_leave_a_stream_to_small_ones(lib, gpu_exec, oracle, slot
     rc, stats = lib.server_stats(srv)
     assert stats["completed"] == len(cases) and stats["batches"] == len(cases), stats
     for c2, plan, infos, al, be in keep:
-        _, want = oracle.run(c2)
+        _, want = oracle.run(c2, gpu=True)
         assert_parity(c2.D.buf, want, c2)
         _free(lib, plan, infos)
     lib.destroy_server(srv)
