@@ -23,7 +23,8 @@ enum {
     TAPP_AMD_ERR_INVALID = 104,     /* invalid handle, NULL argument, label count mismatch */
     TAPP_AMD_ERR_KEY = 105,         /* unknown attribute key */
     TAPP_AMD_ERR_BUDGET = 106,      /* request larger than the serving memory budget */
-    TAPP_AMD_ERR_BUSY = 107         /* serving queue full */
+    TAPP_AMD_ERR_BUSY = 107,        /* serving queue full */
+    TAPP_AMD_ERR_SCRATCH = 108      /* product needs more than the allocation-free entry point's scratch */
 };
 
 /* Threads and ordering.
@@ -200,6 +201,32 @@ TAPP_EXPORT TAPP_error TAPP_AMD_update_resident(TAPP_handle handle, const void* 
 TAPP_EXPORT TAPP_error TAPP_AMD_release_resident(TAPP_handle handle, const void* host);
 /* info[0] resident buffers, info[1] their bytes. */
 TAPP_EXPORT TAPP_error TAPP_AMD_resident_info(TAPP_handle handle, int64_t info[2]);
+
+/* --- allocation-free execution ---------------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): TAPP_execute_product with the same arguments, codes and
+ * results, whose device memory -- the plan's offset tables, pre-pass buffers,
+ * split-K partial sums -- is one static device array of 64 MiB per GPU
+ * (allocated when the library's code object loads, held until the process
+ * ends).  It calls no hipMalloc / hipFree; host memory for a plan's table
+ * contents is allocated on the first call with a given layout.
+ *   - A, B, C and D must be device memory (or host ranges resident on the
+ *     plan's handle): other host operands return TAPP_AMD_ERR_INVALID.
+ *   - A product whose buffers exceed the scratch first drops split-K; if it
+ *     still does not fit, TAPP_AMD_ERR_SCRATCH (108), and
+ *     TAPP_AMD_last_error_message gives the size it needed.
+ *   - exec must come from TAPP_create_executor (a bare int executor would
+ *     need the default context created: TAPP_AMD_ERR_INVALID); the
+ *     self-checking kind 12 is not available here (TAPP_AMD_ERR_UNSUPPORTED).
+ *     The autotuner uses cached choices only.
+ *   - Every call on one GPU uses that GPU's one array: calls from several
+ *     threads take turns (a per-device lock held until the call's kernels
+ *     have finished); the array keeps the last call's contents.
+ * Synchronous: D is written on return. */
+TAPP_EXPORT TAPP_error TAPP_AMD_execute_product_noalloc(TAPP_tensor_product plan, TAPP_executor exec,
+                                                        TAPP_status* status, const void* alpha, const void* A,
+                                                        const void* B, const void* beta, const void* C, void* D);
 
 /* Text of the last device error of the calling thread (which HIP call
  * failed and why), with TAPP_explain_error's maxlen contract. */

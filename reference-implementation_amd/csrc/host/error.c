@@ -42,6 +42,7 @@ static const char* message_for(TAPP_error e)
     case TAPP_AMD_ERR_KEY: return "Unknown attribute key.";
     case TAPP_AMD_ERR_BUDGET: return "Request exceeds the memory budget.";
     case TAPP_AMD_ERR_BUSY: return "Queue full.";
+    case TAPP_AMD_ERR_SCRATCH: return "Product needs more device memory than the allocation-free scratch holds.";
     default: return "Unknown error code.";
     }
 }

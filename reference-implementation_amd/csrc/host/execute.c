@@ -39,6 +39,12 @@ static int device_error(const char* what, int err, int line)
         if (e_ != 0) return device_error(#call, e_, __LINE__);                  \
     } while (0)
 
+#define LAUNCHCHK_UNLOCK(call)                                                  \
+    do {                                                                        \
+        int e_ = (call);                                                        \
+        if (e_ != 0) rc = device_error(#call, e_, __LINE__);                    \
+    } while (0)
+
 size_t TAPP_AMD_last_error_message(char* message, size_t maxlen)
 {
     size_t len = strlen(last_error);
@@ -147,14 +153,11 @@ static void prep_order(const tk_group* rows, const tk_group* K, const tk_group* 
     for (int i = 0; i < 3; i++) { dense_step[ord[i]] = step; step *= size[ord[i]]; }
 }
 
-static int build_tables(struct tapp_plan* plan, const tk_problem* P, int device, struct tapp_plan_cache** out)
+/* The host side of a plan's tables: offsets, sizes and contents. */
+static struct tapp_plan_cache* host_tables(const tk_problem* P, uint64_t key)
 {
-    uint64_t key = problem_key(P);
-    struct tapp_plan_cache* c = plan->cache;
-    if (c && c->key == key && c->device == device && c->dev) { *out = c; return 0; }
-    if (c) { tk_plan_cache_free(c); plan->cache = NULL; }
-    c = calloc(1, sizeof *c);
-    if (!c) return TAPP_AMD_ERR_NOMEM;
+    struct tapp_plan_cache* c = calloc(1, sizeof *c);
+    if (!c) return NULL;
     const int64_t M = P->nM, N = P->nN, K = P->nK, L = P->nL;
     const int64_t len[T_COUNT] = {
         [T_mA] = M, [T_mC] = M, [T_mD] = M, [T_nB] = N, [T_nC] = N, [T_nD] = N,
@@ -169,8 +172,9 @@ static int build_tables(struct tapp_plan* plan, const tk_problem* P, int device,
         total = (total + 31) & ~(size_t)31;   /* 256-byte aligned sub-tables */
     }
     c->total = total;
+    c->key = key;
     c->host = malloc(total * sizeof(int64_t));
-    if (!c->host) { free(c); return TAPP_AMD_ERR_NOMEM; }
+    if (!c->host) { free(c); return NULL; }
     int64_t* h = c->host;
     fill_table(h + c->off[T_mA], &P->M, 0, M);
     fill_table(h + c->off[T_mC], &P->M, 2, M);
@@ -202,6 +206,18 @@ static int build_tables(struct tapp_plan* plan, const tk_problem* P, int device,
     fill_linear(h + c->off[T_dnD], N, M);
     fill_linear(h + c->off[T_dlD], L, M * N);
     h[c->off[T_zero]] = 0;
+    return c;
+}
+
+static int build_tables(struct tapp_plan* plan, const tk_problem* P, int device, struct tapp_plan_cache** out)
+{
+    uint64_t key = problem_key(P);
+    struct tapp_plan_cache* c = plan->cache;
+    if (c && c->key == key && c->device == device && c->dev) { *out = c; return 0; }
+    if (c) { tk_plan_cache_free(c); plan->cache = NULL; }
+    c = host_tables(P, key);
+    if (!c) return TAPP_AMD_ERR_NOMEM;
+    const size_t total = c->total;
     hipError_t e = hipMalloc((void**)&c->dev, total * sizeof(int64_t));
     if (e != hipSuccess) {
         (void)hipGetLastError();
@@ -214,9 +230,23 @@ static int build_tables(struct tapp_plan* plan, const tk_problem* P, int device,
         tk_plan_cache_free(c);
         return device_error("hipMemcpy(tables)", (int)e, __LINE__);
     }
-    c->key = key;
     c->device = device;
     plan->cache = c;
+    *out = c;
+    return 0;
+}
+
+/* Host tables only (no device copy), kept in the plan's second cache slot:
+ * the allocation-free path uploads them into its scratch each call, and
+ * never frees device memory when a layout changes. */
+static int build_host_tables(struct tapp_plan* plan, const tk_problem* P, struct tapp_plan_cache** out)
+{
+    uint64_t key = problem_key(P);
+    struct tapp_plan_cache* c = plan->hcache;
+    if (c && c->key == key) { *out = c; return 0; }
+    if (c) { tk_plan_cache_free(c); plan->hcache = NULL; }
+    if (!(c = host_tables(P, key))) return TAPP_AMD_ERR_NOMEM;
+    plan->hcache = c;
     *out = c;
     return 0;
 }
@@ -456,8 +486,16 @@ int tk_execute(struct tapp_plan* p, struct tapp_exec* ex, const void* alpha, con
     return tk_execute_on(p, ex, alpha, A, B, beta, C, D, true);
 }
 
+/* Device memory of one call: NULL = the plan's cached tables and the
+ * executor's workspace; otherwise one caller-provided region that holds the
+ * tables and every buffer. */
+typedef struct {
+    char* base;
+    size_t bytes;
+} region;
+
 static int execute_on(struct tapp_plan* p, struct tapp_exec* ex, const void* alpha, const void* A, const void* B,
-                      const void* beta, const void* C, void* D, bool sync);
+                      const void* beta, const void* C, void* D, bool sync, const region* scr);
 
 /* With sync == false every step stays ordered on ex->stream and the call
  * returns after enqueueing: host operands must stay valid, and D is written,
@@ -467,13 +505,39 @@ int tk_execute_on(struct tapp_plan* p, struct tapp_exec* ex, const void* alpha, 
                   const void* beta, const void* C, void* D, bool sync)
 {
     pthread_mutex_lock(&p->mu);
-    int rc = execute_on(p, ex, alpha, A, B, beta, C, D, sync);
+    int rc = execute_on(p, ex, alpha, A, B, beta, C, D, sync, NULL);
     pthread_mutex_unlock(&p->mu);
     return rc;
 }
 
+/* One lock per device: calls of the allocation-free path share the device's
+ * scratch array, so each holds it until its kernels have finished. */
+#define NOALLOC_DEVICES 64
+static pthread_mutex_t noalloc_mu[NOALLOC_DEVICES] = {[0 ... NOALLOC_DEVICES - 1] = PTHREAD_MUTEX_INITIALIZER};
+
+int tk_execute_noalloc(struct tapp_plan* p, struct tapp_exec* ex, const void* alpha, const void* A, const void* B,
+                       const void* beta, const void* C, void* D)
+{
+    if (!ex || ex->device < 0 || ex->device >= NOALLOC_DEVICES) return TAPP_AMD_ERR_INVALID;
+    pthread_mutex_lock(&noalloc_mu[ex->device]);
+    region scr = {NULL, 0};
+    int rc = 0;
+    if (ex->have_stream) {
+        void* base = NULL;
+        LAUNCHCHK_UNLOCK(tk_scratch_get(ex->device, &base, &scr.bytes));
+        scr.base = (char*)base;
+    }
+    if (!rc) {
+        pthread_mutex_lock(&p->mu);
+        rc = execute_on(p, ex, alpha, A, B, beta, C, D, true, &scr);
+        pthread_mutex_unlock(&p->mu);
+    }
+    pthread_mutex_unlock(&noalloc_mu[ex->device]);
+    return rc;
+}
+
 static int execute_on(struct tapp_plan* p, struct tapp_exec* ex, const void* alpha, const void* A, const void* B,
-                      const void* beta, const void* C, void* D, bool sync)
+                      const void* beta, const void* C, void* D, bool sync, const region* scr)
 {
     tk_problem P;
     int rc = tk_plan_problem(p, ex, alpha, beta, C, &P);
@@ -484,8 +548,16 @@ static int execute_on(struct tapp_plan* p, struct tapp_exec* ex, const void* alp
     HIPCHK(hipSetDevice(ex->device));
 
     struct tapp_plan_cache* tc;
-    if ((rc = build_tables(p, &P, ex->device, &tc))) return rc;
-    const int64_t* T = tc->dev;
+    const int64_t* T;
+    size_t tab_bytes = 0;
+    if (scr) {
+        if ((rc = build_host_tables(p, &P, &tc))) return rc;
+        tab_bytes = ALIGN256(tc->total * sizeof(int64_t));
+        T = (const int64_t*)scr->base;
+    } else {
+        if ((rc = build_tables(p, &P, ex->device, &tc))) return rc;
+        T = tc->dev;
+    }
     const int64_t* H = tc->host;
 #define DT(t) (T + tc->off[t])
 #define HT(t) (H + tc->off[t])
@@ -510,6 +582,11 @@ static int execute_on(struct tapp_plan* p, struct tapp_exec* ex, const void* alp
         if (r) { src[x] = r; continue; }
         on_dev[x] = false;
         if (P.hi[x] >= P.lo[x]) span_bytes[x] = (size_t)(P.hi[x] - P.lo[x] + 1) * (size_t)es;
+        if (scr) {
+            snprintf(last_error, sizeof last_error,
+                     "allocation-free execute: operand %c is host memory (device or resident operands only)", "ABCD"[x]);
+            return TAPP_AMD_ERR_INVALID;
+        }
     }
     const bool needA = P.nUA > 1 || !compatible(P.type[0], ct) || (P.op_A && P.nUA > 1);
     const bool needB = P.nUB > 1 || !compatible(P.type[1], ct);
@@ -557,17 +634,34 @@ static int execute_on(struct tapp_plan* p, struct tapp_exec* ex, const void* alp
         }
     }
 
-    size_t off_stage[4], off_tmp[4], off_part, total = 0;
-    for (int x = 0; x < 4; x++) { off_stage[x] = total; total = ALIGN256(total + span_bytes[x]); }
+    size_t off_stage[4], off_tmp[4], off_part, off_amax, total;
     const size_t tmp_sz[4] = {szA, szB, szC, szD};
-    for (int x = 0; x < 4; x++) { off_tmp[x] = total; total = ALIGN256(total + tmp_sz[x]); }
-    off_part = total;
-    total = ALIGN256(total + part_bytes);
-    const size_t off_amax = total;
-    if (mode == TK_MODE_SPLIT_F16) total = ALIGN256(total + 2 * sizeof(uint32_t));
-    if (total && (rc = ensure_ws(ex, total))) return rc;
-    char* ws = (char*)ex->ws;
+    for (int pass = 0;; pass++) {
+        total = 0;
+        for (int x = 0; x < 4; x++) { off_stage[x] = total; total = ALIGN256(total + span_bytes[x]); }
+        for (int x = 0; x < 4; x++) { off_tmp[x] = total; total = ALIGN256(total + tmp_sz[x]); }
+        off_part = total;
+        total = ALIGN256(total + part_bytes);
+        off_amax = total;
+        if (mode == TK_MODE_SPLIT_F16) total = ALIGN256(total + 2 * sizeof(uint32_t));
+        /* scratch: split-K gives way first (its partials are the only
+         * optional buffer), then the call is refused */
+        if (!scr || tab_bytes + total <= scr->bytes) break;
+        if (pass == 0 && part_bytes) { ks = 1; kchunk = 0; part_bytes = 0; continue; }
+        snprintf(last_error, sizeof last_error,
+                 "allocation-free execute: needs %zu bytes of scratch, the device scratch holds %zu",
+                 tab_bytes + total, scr->bytes);
+        return TAPP_AMD_ERR_SCRATCH;
+    }
     hipStream_t st = ex->stream;
+    char* ws;
+    if (scr) {
+        ws = scr->base + tab_bytes;
+        HIPCHK(hipMemcpyAsync(scr->base, tc->host, tc->total * sizeof(int64_t), hipMemcpyHostToDevice, st));
+    } else {
+        if (total && (rc = ensure_ws(ex, total))) return rc;
+        ws = (char*)ex->ws;
+    }
 
     /* operands on the device: element (0, ..., 0) addresses */
     dptr d[4];
@@ -695,7 +789,7 @@ static int execute_on(struct tapp_plan* p, struct tapp_exec* ex, const void* alp
         const uint64_t key = tk_tune_key(f, (int)(sizeof f / sizeof f[0]));
         int cfg = 0;
         bool have = tk_tune_get(ex->device, key, &cfg);
-        if (!have && ex->autotune == 1 && sync && tune_safe(&P, src, needC)) {
+        if (!have && ex->autotune == 1 && sync && !scr && tune_safe(&P, src, needC)) {
             if ((rc = tune_gett(ex, &g, ks, kchunk, (float*)(ws + off_part), outs, &cfg))) return rc;
             char desc[160];
             snprintf(desc, sizeof desc, "gett ct=%d mode=%d M=%lld N=%lld K=%lld L=%lld load=%d,%d beta0=%d",

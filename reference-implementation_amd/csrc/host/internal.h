@@ -37,6 +37,7 @@ struct tapp_plan {
     int nidx[4];               /* label count copied at creation */
     TAPP_prectype prec;
     struct tapp_plan_cache* cache;
+    struct tapp_plan_cache* hcache;   /* host-only tables of the allocation-free path */
     /* recursive; held while a call builds or reads the cache and enqueues
      * work that reads it, so calls on one plan from several threads are safe */
     pthread_mutex_t mu;
@@ -156,6 +157,11 @@ int tk_execute(struct tapp_plan* p, struct tapp_exec* ex, const void* alpha, con
                const void* B, const void* beta, const void* C, void* D);
 int tk_execute_on(struct tapp_plan* p, struct tapp_exec* ex, const void* alpha, const void* A,
                   const void* B, const void* beta, const void* C, void* D, bool sync);
+/* Synchronous execute whose device memory (tables, pre-pass buffers,
+ * split-K partials) is the device scratch array (TK_SCRATCH_BYTES); device or
+ * resident operands only.  Calls on one device take turns. */
+int tk_execute_noalloc(struct tapp_plan* p, struct tapp_exec* ex, const void* alpha, const void* A,
+                       const void* B, const void* beta, const void* C, void* D);
 int tk_kernel_for(const tk_problem* P, int force);
 /* self-checking execution (executor kind 12, check.c) */
 int tk_execute_checked(struct tapp_plan* p, struct tapp_exec* ex, const void* alpha, const void* A,

@@ -60,6 +60,7 @@ TAPP_error TAPP_destroy_tensor_product(TAPP_tensor_product plan)
     if (!p) return 0;
     for (int x = 0; x < 4; x++) free(p->idx[x]);
     tk_plan_cache_free(p->cache);
+    tk_plan_cache_free(p->hcache);
     pthread_mutex_destroy(&p->mu);
     free(p);
     return 0;
@@ -99,6 +100,30 @@ TAPP_error TAPP_execute_product(TAPP_tensor_product plan, TAPP_executor exec, TA
     tk_exec_lock(ex);
     if (ex && *(const int*)exec == 12) rc = tk_execute_checked(p, ex, alpha, A, B, beta, C, D);
     else rc = tk_execute(p, ex, alpha, A, B, beta, C, D);
+    tk_exec_unlock(ex);
+    return rc;
+}
+
+TAPP_error TAPP_AMD_execute_product_noalloc(TAPP_tensor_product plan, TAPP_executor exec, TAPP_status* status,
+                                            const void* alpha, const void* A, const void* B, const void* beta,
+                                            const void* C, void* D)
+{
+    (void)status;
+    struct tapp_plan* p = (struct tapp_plan*)plan;
+    if (!p) return TAPP_AMD_ERR_INVALID;
+    /* validation first, with the reference's codes; a caller's bare int
+     * executor is only read (resolving it would create the default context) */
+    struct tapp_exec shadow;
+    memset(&shadow, 0, sizeof shadow);
+    struct tapp_exec* ex = exec ? tk_exec_lookup(exec) : NULL;
+    if (exec && !ex) shadow.kind = *(const int*)exec;
+    tk_problem P;
+    int rc = tk_plan_problem(p, ex ? ex : (exec ? &shadow : NULL), alpha, beta, C, &P);
+    if (rc) return rc;
+    if (!ex) return TAPP_AMD_ERR_INVALID;
+    if (ex->kind == 12) return TAPP_AMD_ERR_UNSUPPORTED;
+    tk_exec_lock(ex);
+    rc = tk_execute_noalloc(p, ex, alpha, A, B, beta, C, D);
     tk_exec_unlock(ex);
     return rc;
 }

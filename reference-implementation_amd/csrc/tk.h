@@ -180,6 +180,12 @@ typedef struct tk_compare {
 
 int tk_launch_compare(const tk_compare* c, void* stream);
 
+/* Device scratch of the allocation-free entry point: a file-scope device
+ * array of TK_SCRATCH_BYTES per GPU (tk_scratch.hip).  *ptr receives its
+ * device address on `device` (looked up once per device, then cached). */
+#define TK_SCRATCH_BYTES ((size_t)64 << 20)
+int tk_scratch_get(int device, void** ptr, size_t* bytes);
+
 /* Number of kernel variants compiled in, for diagnostics. */
 int tk_kernel_count(void);
 

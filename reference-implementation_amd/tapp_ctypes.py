@@ -30,6 +30,7 @@ ATTR_TIMING = 0x1006
 ATTR_AUTOTUNE = 0x1007
 ATTR_SELF_CHECK = 0x1008
 # this library's error codes (tapp_amd.h)
+AMD_ERR_SCRATCH = 108
 AMD_ERR_UNSUPPORTED, AMD_ERR_DEVICE, AMD_ERR_NOMEM, AMD_ERR_INVALID, AMD_ERR_KEY, AMD_ERR_BUDGET, AMD_ERR_BUSY = \
     range(101, 108)
 KERNEL_AUTO, KERNEL_SIMPLE, KERNEL_MFMA, KERNEL_SPLITK, KERNEL_GROUP = 0, 1, 2, 3, 4
@@ -77,7 +78,7 @@ class TappLib:
         "TAPP_AMD_group_info", "TAPP_AMD_create_server", "TAPP_AMD_server_submit", "TAPP_AMD_status_wait",
         "TAPP_AMD_server_cancel", "TAPP_AMD_server_set_paused", "TAPP_AMD_server_drain", "TAPP_AMD_server_stats",
         "TAPP_AMD_destroy_server", "TAPP_AMD_make_resident", "TAPP_AMD_update_resident",
-        "TAPP_AMD_release_resident", "TAPP_AMD_resident_info",
+        "TAPP_AMD_release_resident", "TAPP_AMD_resident_info", "TAPP_AMD_execute_product_noalloc",
     )
 
     def __init__(self, path=None):
@@ -147,6 +148,7 @@ class TappLib:
             L.TAPP_AMD_update_resident.argtypes = [_H, ctypes.c_void_p]
             L.TAPP_AMD_release_resident.argtypes = [_H, ctypes.c_void_p]
             L.TAPP_AMD_resident_info.argtypes = [_H, _i64p]
+            L.TAPP_AMD_execute_product_noalloc.argtypes = [_H, _H, ctypes.c_void_p] + [ctypes.c_void_p] * 6
 
     # --- objects ---------------------------------------------------------
     def create_info(self, dtype, extents, strides):
@@ -188,6 +190,17 @@ class TappLib:
     def execute(self, plan, exec_, alpha_ptr, a_ptr, b_ptr, beta_ptr, c_ptr, d_ptr, status_ptr=None):
         return self.lib.TAPP_execute_product(plan, exec_, status_ptr, alpha_ptr, a_ptr, b_ptr, beta_ptr,
                                              c_ptr, d_ptr)
+
+    def execute_noalloc(self, plan, exec_, alpha_ptr, a_ptr, b_ptr, beta_ptr, c_ptr, d_ptr):
+        """TAPP_AMD_execute_product_noalloc: TAPP_execute_product on the
+        device scratch array (no device allocation)."""
+        return self.lib.TAPP_AMD_execute_product_noalloc(plan, exec_, None, alpha_ptr, a_ptr, b_ptr, beta_ptr,
+                                                         c_ptr, d_ptr)
+
+    def last_error_message(self):
+        buf = ctypes.create_string_buffer(512)
+        self.lib.TAPP_AMD_last_error_message(buf, 512)
+        return buf.value.decode()
 
     def execute_batched(self, plan, exec_, alpha_ptr, a_ptrs, b_ptrs, beta_ptr, c_ptrs, d_ptrs):
         n = len(a_ptrs)
