@@ -63,7 +63,7 @@ enum {
      * products, run in FP32 with value 3.  Opt-in; default 0. */
     TAPP_AMD_ATTR_SPLIT_BF16 = 0x1001,
     /* Force a kernel family: 0 auto, 1 one-lane-per-output FMA, 2 MFMA,
-     * 3 split-K.  Default 0. */
+     * 3 split-K, 5 small-tile MFMA (FP32; others run 2).  Default 0. */
     TAPP_AMD_ATTR_KERNEL = 0x1002,
     /* Read-only: kernel family used by the last execute (int). */
     TAPP_AMD_ATTR_LAST_KERNEL = 0x1003,
@@ -105,7 +105,9 @@ enum {
     TAPP_AMD_KERNEL_SIMPLE = 1,
     TAPP_AMD_KERNEL_MFMA = 2,
     TAPP_AMD_KERNEL_SPLITK = 3,
-    TAPP_AMD_KERNEL_GROUP = 4       /* persistent launch of a product group */
+    TAPP_AMD_KERNEL_GROUP = 4,      /* persistent launch of a product group */
+    TAPP_AMD_KERNEL_MFMA_SMALL = 5  /* FP32 16 x 16 MFMA tiles, K split over a workgroup's
+                                       waves: few outputs, moderate K, one launch */
 };
 
 /* --- product groups ---------------------------------------------------------

@@ -1000,6 +1000,69 @@ __global__ __launch_bounds__(256, (lp_waves<AL, BL, MODE>())) void k_gett_lp(tk_
     }
 }
 
+// ---------------------------------------------------------------------------
+// k_small_f32: FP32 products with few outputs and a moderate K (configs[0]'s
+// 256 x 256 outputs at K = 256 gives 128 x 128 tiles only 4 workgroups).
+// One workgroup per 16 x 16 output tile, so the grid has M N L / 256
+// workgroups; its 4 waves each take a quarter of K with
+// v_mfma_f32_16x16x4f32 (lane l supplies row l % 16 at k = l / 16 of a
+// 4-deep step).  Operands go from global memory straight into that lane
+// layout: a chunk's K-table reads, then its data reads, are all in flight
+// together.  The four partial tiles are summed through LDS in wave order
+// (deterministic) and each lane writes one output.  One launch, no partial
+// buffer and no second kernel.
+// ---------------------------------------------------------------------------
+template <int CH>
+__global__ __launch_bounds__(256) void k_small_f32(tk_gett g)
+{
+    __shared__ float part[4][256];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t ntm = (g.M + 15) / 16, ntn = (g.N + 15) / 16;
+    const int64_t tile = xcd_tile(blockIdx.x, ntm * ntn);
+    const int64_t m0 = (tile / ntn) * 16, n0 = (tile % ntn) * 16;
+    const int64_t m = m0 + (lane & 15), n = n0 + (lane & 15);
+    const int kq = lane >> 4;
+    const bool m_in = m < g.M, n_in = n < g.N;
+    const int64_t ma = m_in ? g.mA[m] : 0, nb = n_in ? g.nB[n] : 0;
+    // this wave's quarter of K, in 4-deep steps
+    const int64_t steps = (g.K + 3) / 4, sq = (steps + 3) / 4;
+    const int64_t s_begin = wave * sq, s_end = s_begin + sq < steps ? s_begin + sq : steps;
+
+    for (int64_t l = blockIdx.y; l < g.L; l += gridDim.y) {
+        const float* Ab = (const float*)g.A + g.lA[l] + ma;
+        const float* Bb = (const float*)g.B + g.lB[l] + nb;
+        f32x4 acc = f32x4{};
+        for (int64_t s0 = s_begin; s0 < s_end; s0 += CH) {
+            int64_t ka[CH], kb[CH];
+            bool in[CH];
+#pragma unroll
+            for (int j = 0; j < CH; ++j) {
+                const int64_t k = 4 * (s0 + j) + kq;
+                in[j] = s0 + j < s_end && k < g.K;
+                ka[j] = in[j] ? g.kA[k] : 0;
+                kb[j] = in[j] ? g.kB[k] : 0;
+            }
+            float a[CH], b[CH];
+#pragma unroll
+            for (int j = 0; j < CH; ++j) {
+                a[j] = (in[j] && m_in) ? Ab[ka[j]] : 0.0f;
+                b[j] = (in[j] && n_in) ? Bb[kb[j]] : 0.0f;
+            }
+#pragma unroll
+            for (int j = 0; j < CH; ++j) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(b[j], a[j], acc, 0, 0, 0);
+        }
+        // lane holds out[n0 + 4 (lane / 16) + r][m0 + lane % 16]
+        __syncthreads();   // the previous batch index's reads of part are done
+#pragma unroll
+        for (int r = 0; r < 4; ++r) part[wave][(4 * kq + r) * 16 + (lane & 15)] = acc[r];
+        __syncthreads();
+        const int t = threadIdx.x;
+        const float sum = ((part[0][t] + part[1][t]) + part[2][t]) + part[3][t];
+        const int64_t mo = m0 + (t & 15), no = n0 + (t >> 4);
+        if (mo < g.M && no < g.N) epilogue<float, false>(g, l, mo, no, sum, 0.0f);
+    }
+}
+
 // Sums the split-K partials of k_gett_f32<..., true> in split order
 // (deterministic) and applies the epilogue.  Position p of a split's partial
 // block is (l, tile, wave, slot, lane, element) with element fastest, the
@@ -1120,6 +1183,12 @@ extern "C" int tk_launch_gett(const tk_gett* gp, void* stream)
     case TK_F32:
         if (kern == TK_KERNEL_SIMPLE) return launch_simple<float, false>(g, st);
         if (kern == TK_KERNEL_SPLITK) return launch_splitk<float, false>(g, st);
+        if (kern == TK_KERNEL_SMALL && g.mode == TK_MODE_PLAIN) {
+            const int64_t tiles = ((g.M + 15) / 16) * ((g.N + 15) / 16);
+            const int64_t ly = g.L < 65535 ? g.L : 65535;
+            hipLaunchKernelGGL((k_small_f32<16>), dim3((unsigned)tiles, (unsigned)ly), dim3(256), 0, st, g);
+            return (int)hipGetLastError();
+        }
         return launch_gett_f32(g, st);
     case TK_F64:
         if (kern == TK_KERNEL_SIMPLE) return launch_simple<double, false>(g, st);
@@ -1146,4 +1215,4 @@ extern "C" int tk_launch_gett(const tk_gett* gp, void* stream)
     }
 }
 
-extern "C" int tk_kernel_count(void) { return 12; }
+extern "C" int tk_kernel_count(void) { return 13; }

@@ -23,7 +23,7 @@ TAPP_error TAPP_attr_set(TAPP_attr attr, TAPP_key key, void* value)
     }
     case TAPP_AMD_ATTR_KERNEL: {
         int k = *(int*)value;
-        if (k < 0 || k > 3) return TAPP_AMD_ERR_INVALID;
+        if (k < 0 || k > 5 || k == 4) return TAPP_AMD_ERR_INVALID;
         e->force_kernel = k;
         return 0;
     }

@@ -334,12 +334,14 @@ static int launch_gett_ctx(void* c)
 static int tune_gett(struct tapp_exec* ex, tk_gett* g, int64_t ks, int64_t kchunk, float* partial, int64_t outs,
                      int* best)
 {
-    int cand[6], nc = 0;
+    int cand[7], nc = 0;
     const double macs = (double)outs * (double)g->K;
     if (macs <= (double)(1 << 30)) cand[nc++] = TK_KERNEL_SIMPLE;
     if (g->ctype == TK_F32) {
         cand[nc++] = TK_KERNEL_MFMA;
         if (ks > 1) cand[nc++] = TK_KERNEL_MFMA | (1 << 8);
+        const int64_t t16 = ((g->M + 15) / 16) * ((g->N + 15) / 16) * g->L;
+        if (g->mode == TK_MODE_PLAIN && g->K <= 8192 && t16 <= 16 * (int64_t)ex->cus) cand[nc++] = TK_KERNEL_SMALL;
     } else {
         cand[nc++] = TK_KERNEL_MFMA | (1 << 4);
         cand[nc++] = TK_KERNEL_MFMA | (2 << 4);
@@ -618,6 +620,15 @@ static int execute_on(struct tapp_plan* p, struct tapp_exec* ex, const void* alp
         if (kern != TK_KERNEL_MFMA || ct != TK_F32) mode = TK_MODE_PLAIN;
         else if (!sync || !tune_safe(&P, src, needC)) mode = TK_MODE_SPLIT6_BF16;
     }
+    /* few outputs and a moderate K (configs[0]: 4 tiles of 128 x 128): 16 x
+     * 16 tiles with K split over each workgroup's waves, in one launch,
+     * instead of split-K's partial sums and second kernel */
+    if (kern == TK_KERNEL_MFMA && ct == TK_F32 && mode == TK_MODE_PLAIN && ex->force_kernel == TK_KERNEL_AUTO) {
+        const int64_t t128 = ((M + 127) / 128) * ((N + 127) / 128) * L;
+        const int64_t t16 = ((M + 15) / 16) * ((N + 15) / 16) * L;
+        if (t128 < 2 * (int64_t)ex->cus && K <= 1024 && t16 <= 4 * (int64_t)ex->cus) kern = TK_KERNEL_SMALL;
+    }
+    if (kern == TK_KERNEL_SMALL && (ct != TK_F32 || mode != TK_MODE_PLAIN)) kern = TK_KERNEL_MFMA;
     int64_t ks = 1, kchunk = 0;
     size_t part_bytes = 0;
     if ((kern == TK_KERNEL_MFMA || may_tune) && ct == TK_F32 && K >= 64) {

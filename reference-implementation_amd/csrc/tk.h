@@ -81,7 +81,8 @@ enum {
     TK_KERNEL_AUTO = 0,
     TK_KERNEL_SIMPLE = 1,    /* one lane per output element, FMA over K */
     TK_KERNEL_MFMA = 2,      /* LDS-tiled MFMA GETT */
-    TK_KERNEL_SPLITK = 3     /* one workgroup per output, K split across lanes */
+    TK_KERNEL_SPLITK = 3,    /* one workgroup per output, K split across lanes */
+    TK_KERNEL_SMALL = 5      /* FP32: 16 x 16 MFMA tiles, K split over the workgroup's waves */
 };
 
 /* Pre-pass: reduce unary modes, apply conj and input rounding, convert the

@@ -33,7 +33,7 @@ ATTR_SELF_CHECK = 0x1008
 AMD_ERR_SCRATCH = 108
 AMD_ERR_UNSUPPORTED, AMD_ERR_DEVICE, AMD_ERR_NOMEM, AMD_ERR_INVALID, AMD_ERR_KEY, AMD_ERR_BUDGET, AMD_ERR_BUSY = \
     range(101, 108)
-KERNEL_AUTO, KERNEL_SIMPLE, KERNEL_MFMA, KERNEL_SPLITK, KERNEL_GROUP = 0, 1, 2, 3, 4
+KERNEL_AUTO, KERNEL_SIMPLE, KERNEL_MFMA, KERNEL_SPLITK, KERNEL_GROUP, KERNEL_MFMA_SMALL = 0, 1, 2, 3, 4, 5
 
 TYPE_SIZE = {F32: 4, F64: 8, C32: 8, C64: 16, F16: 2, BF16: 2}
 

@@ -131,6 +131,29 @@ def test_configs0_against_reference_golden(lib, gpu_exec):
     assert_parity(got, g["D"], c)
 
 
+def test_small_tile_kernel_by_rule_and_batches(lib, gpu_exec, oracle):
+    """Few outputs and a moderate K take the 16 x 16 small-tile kernel by
+    rule (configs[0] itself; a Hadamard batch; ragged edges; K not a
+    multiple of 4; beta != 0 with a transposed A), each against the oracle,
+    and the same product run twice gives the same bits."""
+    cases = [T.configs0_case(seed=3),
+             T.four_index_case(5, (9, 7, 13, 6, 11, 5)),
+             _gemm_case(tc.F32, 100, 90, 301, 21, transA=True),
+             _gemm_case(tc.F32, 17, 250, 1000, 22, transB=True, beta=0.0)]
+    A = T.make_operand(tc.F32, [40, 37, 5], "mkl", 31)
+    B = T.make_operand(tc.F32, [37, 24, 5], "knl", 32)
+    C = T.make_operand(tc.F32, [40, 24, 5], "mnl", 33)
+    D = T.make_operand(tc.F32, [40, 24, 5], "mnl", 34, pad=1)
+    cases.append(T.Case(A, B, C, D, tc.DEFAULT_PREC, 0.5, -1.25, name="small_batch"))
+    for c in cases:
+        rc, got, want = run_both(lib, gpu_exec, oracle, c)
+        assert rc == 0
+        assert lib.attr_get_int(gpu_exec, tc.ATTR_LAST_KERNEL)[1] == tc.KERNEL_MFMA_SMALL, c.name
+        assert_parity(got, want, c)
+        rc, again = T.run_lib(lib, c, gpu_exec)
+        assert rc == 0 and np.array_equal(again.view(np.uint32), got.view(np.uint32)), c.name
+
+
 def test_configs1_full_size(lib, gpu_exec, oracle):
     c = T.configs1_case(seed=1)
     rc, got, want = run_both(lib, gpu_exec, oracle, c)
@@ -161,7 +184,7 @@ def _gemm_case(dtype, M, N, K, seed, transA=False, transB=False, ldpad=0, conj=(
 
 
 @pytest.mark.parametrize("dtype", [tc.F32, tc.F64, tc.C32, tc.C64])
-@pytest.mark.parametrize("kernel", [tc.KERNEL_SIMPLE, tc.KERNEL_MFMA, tc.KERNEL_SPLITK])
+@pytest.mark.parametrize("kernel", [tc.KERNEL_SIMPLE, tc.KERNEL_MFMA, tc.KERNEL_SPLITK, tc.KERNEL_MFMA_SMALL])
 def test_forced_kernels_on_gemm_shapes(lib, oracle, dtype, kernel):
     _, ex = lib.create_executor()
     assert lib.attr_set_int(ex, tc.ATTR_KERNEL, kernel) == 0
@@ -175,7 +198,8 @@ def test_forced_kernels_on_gemm_shapes(lib, oracle, dtype, kernel):
             rc, got, want = run_both(lib, ex, oracle, c)
             assert rc == 0
             assert_parity(got, want, c)
-            assert lib.attr_get_int(ex, tc.ATTR_LAST_KERNEL)[1] == kernel
+            ran = kernel if kernel != tc.KERNEL_MFMA_SMALL or dtype == tc.F32 else tc.KERNEL_MFMA
+            assert lib.attr_get_int(ex, tc.ATTR_LAST_KERNEL)[1] == ran
     lib.destroy_executor(ex)
 
 
@@ -295,15 +319,20 @@ def test_split_f16_falls_back_bitwise(lib):
         assert rc == 0 and rc2 == 0
         assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), case.name
 
+    # the fallback is the FP32 MFMA GETT (plain FP32 by rule would take the
+    # small-tile kernel at these sizes)
+    ex32m = _with_split(lib, 0)
+    assert lib.attr_set_int(ex32m, tc.ATTR_KERNEL, tc.KERNEL_MFMA) == 0
     c = _gemm_case(tc.F32, 136, 72, 512, 950)
     c.A.buf[c.A.offset + 5] = np.inf
-    same(c, ex32)
+    same(c, ex32m)
     c = _gemm_case(tc.F32, 136, 72, 512, 951)
     c.B.buf[c.B.offset + 7] = np.nan
-    same(c, ex32)
+    same(c, ex32m)
     c = _gemm_case(tc.F32, 136, 72, 512, 952)
     c.A.buf *= np.float32(1e-32)
-    same(c, ex32)
+    same(c, ex32m)
+    lib.destroy_executor(ex32m)
     c = _gemm_case(tc.F32, 3, 700, 96, 953, beta=0.25)
     same(c, ex32)
     assert lib.attr_get_int(ex16, tc.ATTR_LAST_KERNEL)[1] == tc.KERNEL_SIMPLE
@@ -358,6 +387,8 @@ def test_split_k_few_tiles(lib, oracle, mode):
     again gives the same bits."""
     prec = {"bf16": tc.BF16BF16_ACCUM_F32, "f16": tc.F16F16_ACCUM_F32}.get(mode, tc.DEFAULT_PREC)
     _, ex = lib.create_executor()
+    # forced: by rule these shapes take the small-tile kernel in plain FP32
+    assert lib.attr_set_int(ex, tc.ATTR_KERNEL, tc.KERNEL_MFMA) == 0
     if mode.startswith("split"):
         assert lib.attr_set_int(ex, tc.ATTR_SPLIT_BF16, SPLIT_ATTR[mode]) == 0
     for i, (M, N, K, L) in enumerate([(100, 90, 301, 3), (256, 256, 256, 1), (40, 200, 1000, 2)]):

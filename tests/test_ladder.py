@@ -57,7 +57,7 @@ def test_ladder_fp32_against_oracle(lib, gpu_exec, oracle, beta):
     c = T.ladder_case(seed=3, beta=beta)
     rc, got = T.run_lib(lib, c, gpu_exec)
     assert rc == 0
-    assert lib.attr_get_int(gpu_exec, tc.ATTR_LAST_KERNEL)[1] == tc.KERNEL_MFMA
+    assert lib.attr_get_int(gpu_exec, tc.ATTR_LAST_KERNEL)[1] in (tc.KERNEL_MFMA, tc.KERNEL_MFMA_SMALL)
     _, want = oracle.run(c)
     assert T.normwise_error(got, want, T.written_mask(c.D)) <= 1e-5
 
