@@ -1039,17 +1039,22 @@ __global__ __launch_bounds__(256) void k_small_f32(tk_gett g)
             for (int j = 0; j < CH; ++j) {
                 const int64_t k = 4 * (s0 + j) + kq;
                 in[j] = s0 + j < s_end && k < g.K;
-                ka[j] = in[j] ? g.kA[k] : 0;
-                kb[j] = in[j] ? g.kB[k] : 0;
+                const int64_t kk = in[j] ? k : 0;        // table entry 0 is valid whenever K > 0
+                ka[j] = g.kA[kk];
+                kb[j] = g.kB[kk];
             }
+            // unconditional loads at valid addresses (offset 0 when out of
+            // range), masked afterwards: no branch separates them
             float a[CH], b[CH];
 #pragma unroll
             for (int j = 0; j < CH; ++j) {
-                a[j] = (in[j] && m_in) ? Ab[ka[j]] : 0.0f;
-                b[j] = (in[j] && n_in) ? Bb[kb[j]] : 0.0f;
+                a[j] = Ab[ka[j]];
+                b[j] = Bb[kb[j]];
             }
 #pragma unroll
-            for (int j = 0; j < CH; ++j) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(b[j], a[j], acc, 0, 0, 0);
+            for (int j = 0; j < CH; ++j)
+                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(in[j] && n_in ? b[j] : 0.0f, in[j] && m_in ? a[j] : 0.0f,
+                                                            acc, 0, 0, 0);
         }
         // lane holds out[n0 + 4 (lane / 16) + r][m0 + lane % 16]
         __syncthreads();   // the previous batch index's reads of part are done

@@ -13,10 +13,10 @@
 //             k or g has unit stride, so global reads are coalesced); each lane
 //             then owns one column g and accumulates S outputs with FMA.  This
 //             is the HBM-streaming shape of Tucker mode products.
-//  mfma tile  64 x 64 x 32 per step, 4 waves each one 32 x 32 block of
-//             v_mfma_f32_32x32x2f32; operands through LDS; row offsets in
-//             registers and K offsets staged in LDS per 256-wide chunk, so a
-//             step's loads are one memory round trip, issued a step ahead.
+//  mfma tile  128 x 128 outputs, four independent 64 x 64 wave blocks over
+//             the whole K, 2 x 2 MFMA blocks each (FP32, or BF16 / F16
+//             inputs for those precisions) fed straight from global memory
+//             (K offsets staged in LDS per 256-deep chunk).
 #include "tk_device.hpp"
 
 namespace tk {
@@ -27,11 +27,9 @@ constexpr int GT = 256;                   // thin tile columns
 constexpr int KC = 16;                    // thin K chunk (small LDS: six workgroups per CU)
 constexpr int YP = GT + 1;                // LDS pitch of the Y chunk
 constexpr int SMAX = 16;
-constexpr int MT = 64, MK = 32, MP = MT + 4;
-constexpr int NBC = MT / 16;              // B columns per lane when B is read along K
+constexpr int MT = TK_GROUP_MT;           // mfma tile edge
 constexpr int KO = 256;                   // K offsets staged in LDS per chunk (mfma tile)
-// the mfma tile's A and B steps share the thin tile's LDS array
-static_assert(2 * MK * MP <= KC * YP + SMAX * KC, "mfma tile staging must fit the shared LDS array");
+static_assert(MT == 128, "the mfma tile is four 64 x 64 wave blocks");
 
 // Pointers read from a descriptor in memory are generic to the compiler, and
 // generic (FLAT) loads wait conservatively; the operands and tables all live
@@ -81,9 +79,24 @@ __device__ __forceinline__ void group_store(const GView& p, int64_t l, int64_t s
     p.D[p.lD[l] + p.sD[s] + p.gD[g]] = out;
 }
 
-template <int SB>
+template <int RM>
+__device__ __forceinline__ float rnd_m(float x)
+{
+    if (RM == TK_MODE_BF16) return round_bf16(x);
+    if (RM == TK_MODE_F16) return round_f16(x);
+    return x;
+}
+
+// Thin tile.  X's chunk (this lane: s = tid / KC, k = tid % KC) and Y's
+// chunk go through LDS.  Dense Y (K consecutive, columns K apart: the tile is
+// one contiguous block) is prefetched: a chunk's loads, unconditional at
+// valid addresses and masked when stored, are issued into registers right
+// after the previous chunk reached LDS, so they are in flight during its
+// arithmetic.  Other Y layouts load each chunk when it is needed.
+template <int SB, int RM>
 __device__ __forceinline__ void thin_tile(const GView& p, int64_t l, int64_t g0, float* lds, int64_t* goff)
 {
+    constexpr int NY = KC * GT / 256;           // Y elements per lane and chunk
     float* Yl = lds;               // [KC][YP]
     float* Xl = lds + KC * YP;     // [SMAX][KC]
     const int tid = threadIdx.x;
@@ -92,37 +105,51 @@ __device__ __forceinline__ void thin_tile(const GView& p, int64_t l, int64_t g0,
     for (int s = 0; s < SB; ++s) acc[s] = 0.0f;
     const int64_t lx = p.lX[l], ly = p.lY[l];
     const int gn = (int)(p.G - g0 < GT ? p.G - g0 : GT);
-    // column offsets of this tile, once (the K loop reuses them)
+    // column offsets of this tile, once (the K loop reuses them); column 0's
+    // offset stands in for columns past the end
     __syncthreads();
-    goff[tid] = tid < gn ? p.gY[g0 + tid] : 0;
-    // dense Y (K consecutive, columns K apart): the tile is one contiguous block
+    goff[tid] = p.gY[g0 + (tid < gn ? tid : 0)];
+    // this lane's X row (fixed for the tile)
+    const int xs = tid / KC, xk = tid % KC;
+    const bool x_row = tid < SB * KC && xs < p.S;
+    const int64_t xrow = lx + p.sX[x_row ? xs : 0];
+    __syncthreads();
     const bool dense = p.y_kfast == 2;
+    const int64_t g0off = goff[0];
+    float yv[NY], xv = 0.0f;
+    // dense chunk loads: element (k, gg) at ly + g0off + gg K + k0 + k, one
+    // base per chunk and 32-bit offsets (the host takes this path only when
+    // K < 2^21)
+    auto fetch_dense = [&](int64_t k0) {
+        const int kc = (int)(p.K - k0 < KC ? p.K - k0 : KC);
+        xv = p.X[xrow + p.kX[k0 + (xk < kc ? xk : 0)]];
+        gf yb = p.Y + ly + g0off + k0;
+        const uint32_t kk = (uint32_t)(xk < kc ? xk : 0);
+#pragma unroll
+        for (int i = 0; i < NY; ++i) {
+            const int gg = tid / KC + (256 / KC) * i;
+            yv[i] = yb[(uint32_t)(gg < gn ? gg : 0) * (uint32_t)p.K + kk];
+        }
+    };
+    if (dense && p.K > 0) fetch_dense(0);
     for (int64_t k0 = 0; k0 < p.K; k0 += KC) {
         const int kc = (int)(p.K - k0 < KC ? p.K - k0 : KC);
-        __syncthreads();
-        for (int e = tid; e < SB * KC; e += 256) {
-            const int s = e / KC, k = e % KC;
-            float v = 0.0f;
-            if (s < p.S && k < kc) v = rnd_in(p.X[lx + p.sX[s] + p.kX[k0 + k]], p.round);
-            Xl[s * KC + k] = v;
-        }
+        __syncthreads();                          // the previous chunk's LDS reads are done
+        if (!dense) xv = p.X[xrow + p.kX[k0 + (xk < kc ? xk : 0)]];
+        if (tid < SB * KC) Xl[tid] = (x_row && xk < kc) ? rnd_m<RM>(xv) : 0.0f;
         if (dense) {
-            // element (k, gg) at ly + goff[0] + gg * K + k0 + k
-            gf base = p.Y + ly + goff[0] + k0;
-#pragma unroll 8
-            for (int e = tid; e < KC * GT; e += 256) {
-                const int k = e % KC, gg = e / KC;
-                float v = 0.0f;
-                if (k < kc && gg < gn) v = rnd_in(base[(int64_t)gg * p.K + k], p.round);
-                Yl[k * YP + gg] = v;
+#pragma unroll
+            for (int i = 0; i < NY; ++i) {
+                const int gg = tid / KC + (256 / KC) * i;
+                Yl[xk * YP + gg] = (xk < kc && gg < gn) ? rnd_m<RM>(yv[i]) : 0.0f;
             }
         } else if (p.y_kfast) {
             const int k = tid % KC;
-            const int64_t ko = k < kc ? p.kY[k0 + k] : 0;
+            const int64_t ko = p.kY[k0 + (k < kc ? k : 0)];
 #pragma unroll 8
             for (int gg = tid / KC; gg < GT; gg += 256 / KC) {
                 float v = 0.0f;
-                if (k < kc && gg < gn) v = rnd_in(p.Y[ly + ko + goff[gg]], p.round);
+                if (k < kc && gg < gn) v = rnd_m<RM>(p.Y[ly + ko + goff[gg]]);
                 Yl[k * YP + gg] = v;
             }
         } else {
@@ -130,11 +157,14 @@ __device__ __forceinline__ void thin_tile(const GView& p, int64_t l, int64_t g0,
 #pragma unroll 8
             for (int k = 0; k < KC; ++k) {
                 float v = 0.0f;
-                if (k < kc && gg < gn) v = rnd_in(p.Y[ly + p.kY[k0 + k] + goff[gg]], p.round);
+                if (k < kc && gg < gn) v = rnd_m<RM>(p.Y[ly + p.kY[k0 + k] + goff[gg]]);
                 Yl[k * YP + gg] = v;
             }
         }
         __syncthreads();
+        // the registers are free again: the next dense chunk's loads go out
+        // now and are in flight during this chunk's arithmetic
+        if (dense && k0 + KC < p.K) fetch_dense(k0 + KC);
         for (int k = 0; k < kc; ++k) {
             const float y = Yl[k * YP + tid];
 #pragma unroll
@@ -152,100 +182,217 @@ __device__ __forceinline__ void thin_tile(const GView& p, int64_t l, int64_t g0,
     }
 }
 
-__device__ __forceinline__ void mfma_tile(const GView& p, int64_t l, int64_t m0, int64_t n0, float* lds,
-                                          int64_t* kox, int64_t* koy)
+// MFMA tile: 128 x 128 outputs, four independent 64 x 64 wave blocks (wave
+// w: rows 64 (w & 1), columns 64 (w >> 1)), each over the whole K with no
+// exchange between the waves.  A wave feeds its 2 x 2 MFMA blocks with
+// operands loaded from global memory straight into the MFMA lane layout --
+// two rows of A and two columns of B per lane and K step, so one load per
+// MFMA -- a batch of loads in flight at a time, unconditional at valid
+// addresses and masked afterwards.  The K offsets, shared by the four waves,
+// are staged in LDS per 256-deep chunk.  Gathered loads are the bound of
+// these tiles (the texture addresser takes one 64-lane address set per load
+// instruction), so the tile is as large as the accumulators allow.
+//   FP32 (RM = 0): v_mfma_f32_32x32x2f32, lane l supplies row l % 32 at
+//     k = 2 j + l / 32;
+//   BF16 / F16 inputs (RM = TK_MODE_BF16 / TK_MODE_F16, FP32 accumulate):
+//     v_mfma_f32_32x32x16_{bf16,f16}, lane l supplies row l % 32 at the 8
+//     consecutive k = 16 j + 8 (l / 32) + 0..7, each rounded once (RNE).
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+
+template <int RM>
+__device__ __forceinline__ uint32_t pack16(float lo, float hi)
 {
-    float* As = lds;               // [MK][MP]
-    float* Bs = lds + MK * MP;     // [MK][MP]
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = wave & 1, wn = wave >> 1, il = lane & 31, kl = lane >> 5;
-    const int64_t lx = p.lX[l], ly = p.lY[l];
-    // fixed rows of this lane: A element (tid % 64, tid / 64 + 4 t).  B
-    // element (n, k) with the unit-stride index across lanes: along n, lane
-    // column tid % 64 and k = tid / 64 + 4 t; along k, 16 lanes take 16
-    // consecutive k of a column, so a lane touches NBC columns
-    // (n = tid / 16 + 16 c) at k = tid % 16 + 16 j.
-    const int am = tid % MT;
-    const bool arow_ok = m0 + am < p.S;
-    const int64_t arow = arow_ok ? lx + p.sX[m0 + am] : 0;
-    const bool bk = p.y_kfast != 0;
-    int64_t bcol[NBC];
-    bool bok[NBC];                      // offsets may be negative: validity kept apart
-#pragma unroll
-    for (int c = 0; c < NBC; ++c) {
-        const int n = bk ? tid / 16 + 16 * c : tid % MT;
-        bok[c] = n0 + n < p.G;
-        bcol[c] = bok[c] ? ly + p.gY[n0 + n] : 0;
+    if (RM == TK_MODE_BF16) {
+        const uint32_t a = __float_as_uint(round_bf16(lo)) >> 16, b = __float_as_uint(round_bf16(hi)) & 0xffff0000u;
+        return a | b;
     }
-    // B element t of this lane: column slot, n and k
-    auto bslot = [&](int t) { return bk ? t % NBC : 0; };
-    auto bn_of = [&](int t) { return bk ? tid / 16 + 16 * (t % NBC) : tid % MT; };
-    auto bk_of = [&](int t) { return bk ? tid % 16 + 16 * (t / NBC) : tid / MT + (256 / MT) * t; };
-    float ra[(MT * MK) / 256], rb[(MT * MK) / 256];
-    // loads of one K step: offsets of K come from LDS (staged per KO-chunk)
-    auto fetch = [&](int64_t k0) {
-        const int64_t kb = k0 - (k0 / KO) * KO;
+    const _Float16 x = (_Float16)lo, y = (_Float16)hi;
+    return (uint32_t)__builtin_bit_cast(uint16_t, x) | ((uint32_t)__builtin_bit_cast(uint16_t, y) << 16);
+}
+
+template <int RM>
+__device__ __forceinline__ void mfma_chunk(const GView& p, const int64_t (&arow)[2], const int64_t (&bcol)[2],
+                                           const bool (&a_ok)[2], const bool (&b_ok)[2], const int64_t* kox,
+                                           const int64_t* koy, int kc, int kl, f32x16 (&acc)[2][2])
+{
+    if (RM == 0) {
+        // two register sets: the loads of the next JB steps are in flight
+        // while the MFMAs of the current ones run
+        constexpr int JB = 4;
+        float a0[JB][2], b0[JB][2], a1[JB][2], b1[JB][2];
+        auto issue = [&](int j0, float (&a)[JB][2], float (&b)[JB][2]) {
 #pragma unroll
-        for (int t = 0; t < (MT * MK) / 256; ++t) {
-            const int k = tid / MT + (256 / MT) * t;
-            ra[t] = (arow_ok && k0 + k < p.K) ? rnd_in(p.X[arow + kox[kb + k]], p.round) : 0.0f;
-        }
+            for (int j = 0; j < JB; ++j) {
+                const int k = j0 + 2 * j + kl;
+                const int64_t ka = kox[k < kc ? k : 0], kb = koy[k < kc ? k : 0];
 #pragma unroll
-        for (int t = 0; t < (MT * MK) / 256; ++t) {
-            const int k = bk_of(t), c = bslot(t);
-            rb[t] = (bok[c] && k0 + k < p.K) ? rnd_in(p.Y[bcol[c] + koy[kb + k]], p.round) : 0.0f;
-        }
-    };
-    auto stage = [&](int64_t k0) {      // K offsets for k0 .. k0 + KO
-        __syncthreads();
-        for (int k = tid; k < KO; k += 256) {
-            const bool in = k0 + k < p.K;
-            kox[k] = in ? p.kX[k0 + k] : 0;
-            koy[k] = in ? p.kY[k0 + k] : 0;
-        }
-        __syncthreads();
-    };
-    f32x16 acc = {};
-    if (p.K > 0) {
-        stage(0);
-        fetch(0);
-    }
-    for (int64_t k0 = 0; k0 < p.K; k0 += MK) {
-        __syncthreads();
-#pragma unroll
-        for (int t = 0; t < (MT * MK) / 256; ++t) As[(tid / MT + (256 / MT) * t) * MP + am] = ra[t];
-#pragma unroll
-        for (int t = 0; t < (MT * MK) / 256; ++t) Bs[bk_of(t) * MP + bn_of(t)] = rb[t];
-        __syncthreads();
-        const int64_t kn = k0 + MK;
-        if (kn < p.K) {
-            if (kn % KO == 0) {
-                // the offsets of the next chunk are needed before its first loads
-                stage(kn);
+                for (int i = 0; i < 2; ++i) {
+                    a[j][i] = p.X[arow[i] + ka];
+                    b[j][i] = p.Y[bcol[i] + kb];
+                }
             }
-            fetch(kn);                  // next step's loads in flight during the MFMAs
-        }
+        };
+        auto compute = [&](int j0, const float (&a)[JB][2], const float (&b)[JB][2]) {
 #pragma unroll
-        for (int kk = 0; kk < MK; kk += 2) {
-            const float a = As[(kk + kl) * MP + wm * 32 + il];
-            const float b = Bs[(kk + kl) * MP + wn * 32 + il];
-            acc = __builtin_amdgcn_mfma_f32_32x32x2f32(b, a, acc, 0, 0, 0);
+            for (int j = 0; j < JB; ++j) {
+                const bool in = j0 + 2 * j + kl < kc;
+#pragma unroll
+                for (int ib = 0; ib < 2; ++ib)
+#pragma unroll
+                    for (int ia = 0; ia < 2; ++ia)
+                        acc[ib][ia] = __builtin_amdgcn_mfma_f32_32x32x2f32((in && b_ok[ib]) ? b[j][ib] : 0.0f,
+                                                                           (in && a_ok[ia]) ? a[j][ia] : 0.0f,
+                                                                           acc[ib][ia], 0, 0, 0);
+            }
+        };
+        issue(0, a0, b0);
+        for (int j0 = 0; j0 < kc; j0 += 4 * JB) {
+            const int j1 = j0 + 2 * JB, j2 = j1 + 2 * JB;
+            if (j1 < kc) issue(j1, a1, b1);
+            compute(j0, a0, b0);
+            if (j1 >= kc) break;
+            if (j2 < kc) issue(j2, a0, b0);
+            compute(j1, a1, b1);
+        }
+    } else {
+        // 16-deep steps: A's 16 loads packed, then B's (a second register
+        // set for the next step's loads would spill beside 64 accumulators)
+        for (int j0 = 0; j0 < kc; j0 += 16) {
+            uint4 ua[2], ub[2];
+            auto load_pack = [&](const float TK_GLOBAL* base, const int64_t (&off)[2], const bool (&ok)[2],
+                                 const int64_t* ko, uint4 (&out)[2]) {
+                float v[2][8];
+#pragma unroll
+                for (int t = 0; t < 8; ++t) {
+                    const int k = j0 + 8 * kl + t;
+                    const int64_t kk = ko[k < kc ? k : 0];
+#pragma unroll
+                    for (int i = 0; i < 2; ++i) v[i][t] = base[off[i] + kk];
+                }
+#pragma unroll
+                for (int i = 0; i < 2; ++i) {
+                    uint32_t pk[4];
+#pragma unroll
+                    for (int t = 0; t < 8; t += 2) {
+                        const int k = j0 + 8 * kl + t;
+                        pk[t / 2] = pack16<RM>((ok[i] && k < kc) ? v[i][t] : 0.0f, (ok[i] && k + 1 < kc) ? v[i][t + 1] : 0.0f);
+                    }
+                    out[i] = make_uint4(pk[0], pk[1], pk[2], pk[3]);
+                }
+            };
+            load_pack(p.X, arow, a_ok, kox, ua);
+            load_pack(p.Y, bcol, b_ok, koy, ub);
+#pragma unroll
+            for (int ib = 0; ib < 2; ++ib)
+#pragma unroll
+                for (int ia = 0; ia < 2; ++ia) {
+                    if (RM == TK_MODE_BF16)
+                        acc[ib][ia] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                            __builtin_bit_cast(bf16x8, ub[ib]), __builtin_bit_cast(bf16x8, ua[ia]), acc[ib][ia], 0, 0, 0);
+                    else
+                        acc[ib][ia] = __builtin_amdgcn_mfma_f32_32x32x16_f16(
+                            __builtin_bit_cast(f16x8, ub[ib]), __builtin_bit_cast(f16x8, ua[ia]), acc[ib][ia], 0, 0, 0);
+                }
         }
     }
-    const int64_t m = m0 + wm * 32 + il;
-    if (m < p.S) {
-        float cv[16];
+}
+
+template <int RM>
+__device__ __forceinline__ void mfma_tile(const GView& p, int64_t l, int64_t m0, int64_t n0, int64_t* kox,
+                                          int64_t* koy)
+{
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int il = lane & 31, kl = lane >> 5;
+    const int64_t lx = p.lX[l], ly = p.lY[l];
+    const int64_t wm0 = m0 + 64 * (wave & 1), wn0 = n0 + 64 * (wave >> 1);
+    // a wave whose block lies past the edge only helps stage the K offsets
+    const bool active = wm0 < p.S && wn0 < p.G;
+    int64_t arow[2], bcol[2];
+    bool a_ok[2], b_ok[2];
 #pragma unroll
-        for (int r = 0; r < 16; ++r) {
-            const int64_t n = n0 + wn * 32 + 8 * (r >> 2) + 4 * kl + (r & 3);
-            cv[r] = n < p.G ? group_c(p, l, m, n) : 0.0f;
+    for (int i = 0; i < 2; ++i) {
+        const int64_t m = wm0 + 32 * i + il, n = wn0 + 32 * i + il;
+        a_ok[i] = m < p.S;
+        b_ok[i] = n < p.G;
+        arow[i] = lx + p.sX[a_ok[i] ? m : 0];
+        bcol[i] = ly + p.gY[b_ok[i] ? n : 0];
+    }
+    f32x16 acc[2][2];
+#pragma unroll
+    for (int ib = 0; ib < 2; ++ib)
+#pragma unroll
+        for (int ia = 0; ia < 2; ++ia) acc[ib][ia] = f32x16{};
+    for (int64_t k0 = 0; k0 < p.K; k0 += KO) {
+        const int kc = (int)(p.K - k0 < KO ? p.K - k0 : KO);
+        __syncthreads();                         // the previous chunk's offsets are no longer read
+        for (int k = tid; k < kc; k += 256) {
+            kox[k] = p.kX[k0 + k];
+            koy[k] = p.kY[k0 + k];
         }
+        __syncthreads();
+        if (active) mfma_chunk<RM>(p, arow, bcol, a_ok, b_ok, kox, koy, kc, kl, acc);
+    }
+    // the tile's row and column offsets of C and D, staged once in LDS (the
+    // K offsets are no longer read): kox = C rows | D rows, koy = C columns
+    // | D columns; an output then costs one C load and one D store
+    __syncthreads();
+    if (tid < MT) {
+        const int64_t m = m0 + tid < p.S ? m0 + tid : 0, n = n0 + tid < p.G ? n0 + tid : 0;
+        kox[tid] = p.beta_zero ? 0 : p.sC[m];
+        kox[MT + tid] = p.sD[m];
+        koy[tid] = p.beta_zero ? 0 : p.gC[n];
+        koy[MT + tid] = p.gD[n];
+    }
+    __syncthreads();
+    if (!active) return;
+    const int64_t lc = p.beta_zero ? 0 : p.lC[l], ld = p.lD[l];
+    // acc[ib][ia][r] of lane holds out[wn0 + 32 ib + 8 (r / 4) + 4 kl + r % 4][wm0 + 32 ia + il];
+    // per 32 x 32 block, the lane's C values are read before its D stores
 #pragma unroll
-        for (int r = 0; r < 16; ++r) {
-            const int64_t n = n0 + wn * 32 + 8 * (r >> 2) + 4 * kl + (r & 3);
-            if (n < p.G) group_store(p, l, m, n, acc[r], cv[r]);
+    for (int ia = 0; ia < 2; ++ia) {
+        if (!a_ok[ia]) continue;
+        const int mt = (int)(wm0 - m0) + 32 * ia + il;
+        const int64_t rc = lc + kox[mt], rd = ld + kox[MT + mt];
+#pragma unroll
+        for (int ib = 0; ib < 2; ++ib) {
+            float cv[16];
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int nt = (int)(wn0 - n0) + 32 * ib + 8 * (r >> 2) + 4 * kl + (r & 3);
+                float c = 0.0f;
+                if (!p.beta_zero && n0 + nt < p.G) {
+                    c = p.C[rc + koy[nt]];
+                    if (p.round_C) c = rnd_in(c, p.round);
+                }
+                cv[r] = c;
+            }
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int nt = (int)(wn0 - n0) + 32 * ib + 8 * (r >> 2) + 4 * kl + (r & 3);
+                if (n0 + nt < p.G) {
+                    float out = p.alpha * acc[ib][ia][r];
+                    if (!p.beta_zero) out += p.beta * cv[r];
+                    p.D[rd + koy[MT + nt]] = out;
+                }
+            }
         }
     }
+}
+
+template <int RM>
+__device__ __forceinline__ void thin_dispatch(const GView& p, int64_t l, int64_t g0, float* lds, int64_t* goff)
+{
+    if (p.S <= 4) thin_tile<4, RM>(p, l, g0, lds, goff);
+    else if (p.S <= 8) thin_tile<8, RM>(p, l, g0, lds, goff);
+    else thin_tile<16, RM>(p, l, g0, lds, goff);
+}
+
+__device__ __forceinline__ int64_t uniform64(int64_t v)
+{
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)((uint64_t)v >> 32));
+    return (int64_t)(((uint64_t)hi << 32) | lo);
 }
 
 // Tiles come from `nq` queues: queue q holds tiles [qbeg[q], qbeg[q + 1])
@@ -271,7 +418,9 @@ __global__ __launch_bounds__(256, TK_GROUP_WG_PER_CU) void k_group(const tk_prob
         for (;;) {
             if (threadIdx.x == 0) next = qb + (int64_t)atomicAdd(counter + 32 * q, 1ull);
             __syncthreads();
-            const int64_t t = next;
+            // uniform over the workgroup: said so, so the descriptor below is
+            // read with scalar loads into scalar registers
+            const int64_t t = uniform64(next);
             __syncthreads();
             if (t >= qe) break;
             int lo;
@@ -285,6 +434,7 @@ __global__ __launch_bounds__(256, TK_GROUP_WG_PER_CU) void k_group(const tk_prob
                 }
                 lo = a;
             }
+            lo = __builtin_amdgcn_readfirstlane(lo);
             const tk_prob& pd = probs[lo];
             const GView p(pd);
             const int64_t local = t - (lo ? tile_end[lo - 1] : 0);
@@ -294,11 +444,13 @@ __global__ __launch_bounds__(256, TK_GROUP_WG_PER_CU) void k_group(const tk_prob
             const int64_t l = local / per_l, rem = local % per_l;
             const int64_t ts = rem / tiles_g, tg = rem % tiles_g;
             if (cls == TK_GROUP_THIN) {
-                if (p.S <= 4) thin_tile<4>(p, l, tg * GT, lds, goff);
-                else if (p.S <= 8) thin_tile<8>(p, l, tg * GT, lds, goff);
-                else thin_tile<16>(p, l, tg * GT, lds, goff);
+                if (p.round == TK_MODE_BF16) thin_dispatch<TK_MODE_BF16>(p, l, tg * GT, lds, goff);
+                else if (p.round == TK_MODE_F16) thin_dispatch<TK_MODE_F16>(p, l, tg * GT, lds, goff);
+                else thin_dispatch<0>(p, l, tg * GT, lds, goff);
             } else {
-                mfma_tile(p, l, ts * MT, tg * MT, lds, kox, koy);
+                if (p.round == TK_MODE_BF16) mfma_tile<TK_MODE_BF16>(p, l, ts * MT, tg * MT, kox, koy);
+                else if (p.round == TK_MODE_F16) mfma_tile<TK_MODE_F16>(p, l, ts * MT, tg * MT, kox, koy);
+                else mfma_tile<0>(p, l, ts * MT, tg * MT, kox, koy);
             }
         }
     }

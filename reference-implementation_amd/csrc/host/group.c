@@ -54,6 +54,7 @@ struct tapp_group {
 static int y_access(const int64_t* kt, int64_t K, const int64_t* gt, int64_t G)
 {
     if (K <= 1) return 0;
+    if (K >= ((int64_t)1 << 21)) return kt[1] - kt[0] == 1 ? 1 : 0;   /* the dense path's 32-bit tile offsets */
     for (int64_t k = 0; k < K; k++)
         if (kt[k] != kt[0] + k) return kt[1] - kt[0] == 1 ? 1 : 0;
     for (int64_t g = 0; g < G; g++)
@@ -132,14 +133,14 @@ bool tk_group_describe(struct tapp_plan* plan, struct tapp_exec* ex, const tk_pr
     out->round = P->mode;
     out->round_C = P->round_C;
     if (out->cls == TK_GROUP_THIN) { out->tiles_s = 1; out->tiles_g = (out->G + 255) / 256; }
-    else { out->tiles_s = (out->S + 63) / 64; out->tiles_g = (out->G + 63) / 64; }
+    else { out->tiles_s = (out->S + TK_GROUP_MT - 1) / TK_GROUP_MT; out->tiles_g = (out->G + TK_GROUP_MT - 1) / TK_GROUP_MT; }
     return true;
 }
 
 static double tile_cost(const tk_prob* d)
 {
     const double per_tile = d->cls == TK_GROUP_THIN ? 256.0 * (double)d->K * (double)(d->S + 1)
-                                                    : 64.0 * 64.0 * (double)d->K;
+                                                    : (double)TK_GROUP_MT * TK_GROUP_MT * (double)d->K;
     return per_tile + 1.0;
 }
 

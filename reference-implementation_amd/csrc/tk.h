@@ -115,16 +115,20 @@ typedef struct tk_scatter {
  * device operands).  Classes:
  *   TK_GROUP_THIN: D[l, s, g] = sum_k X[l, s, k] Y[l, k, g] with S <= 16; a
  *     workgroup takes 256 columns g and every s, streaming Y through LDS.
- *   TK_GROUP_MFMA: D[l, m, n] over 64 x 64 MFMA tiles (X = rows m, Y = cols n).
+ *   TK_GROUP_MFMA: D[l, m, n] over TK_GROUP_MT x TK_GROUP_MT MFMA tiles (X = rows m,
+ *     Y = cols n).
  * tiles = L * tiles_s * tiles_g; tile_end is the inclusive prefix over the
  * group's problem order. */
 enum { TK_GROUP_THIN = 0, TK_GROUP_MFMA = 1 };
 
 /* Persistent group workgroups per CU: the group kernel's occupancy bound
- * (launch bounds) and the grid the host launches.  Six 256-lane workgroups
- * (24 KB of LDS each) give each CU six independent tiles in flight, which
- * is what hides the tiles' dependent table and data round trips. */
-#define TK_GROUP_WG_PER_CU 6
+ * (launch bounds) and the grid the host launches.  Four 256-lane workgroups
+ * (24 KB of LDS, 128 VGPRs each): the MFMA tile's four 64 x 64 wave blocks
+ * hold 64 accumulators per lane, and the thin tile's prefetch needs the
+ * rest. */
+#define TK_GROUP_WG_PER_CU 3
+/* Edge of a TK_GROUP_MFMA output tile. */
+#define TK_GROUP_MT 128
 
 typedef struct tk_prob {
     int64_t S, G, K, L;
