@@ -24,35 +24,55 @@ namespace tk {
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 constexpr int GT = 256;                   // thin tile columns
-constexpr int KC = 16;                    // thin K chunk (small LDS: six workgroups per CU)
+constexpr int KC = 16;                    // thin K chunk (its X chunk is one element per lane: SMAX KC <= 256)
 constexpr int YP = GT + 1;                // LDS pitch of the Y chunk
 constexpr int SMAX = 16;
 constexpr int MT = TK_GROUP_MT;           // mfma tile edge
 constexpr int KO = 256;                   // K offsets staged in LDS per chunk (mfma tile)
 static_assert(MT == 128, "the mfma tile is four 64 x 64 wave blocks");
+static_assert(SMAX * KC <= 256, "a thin tile loads its X chunk one element per lane");
 
 // Pointers read from a descriptor in memory are generic to the compiler, and
 // generic (FLAT) loads wait conservatively; the operands and tables all live
 // in global memory, so they are accessed through address-space-1 pointers.
 #define TK_GLOBAL __attribute__((address_space(1)))
+#define TK_CONST __attribute__((address_space(4)))
 typedef const float TK_GLOBAL* gf;
 typedef float TK_GLOBAL* gfw;
 typedef const int64_t TK_GLOBAL* gi;
 
+// A product's descriptor, read field by field where a field is used (scalar
+// loads through the constant address space) rather than copied whole into
+// registers for the tile: the persistent loop has too few scalar registers
+// to hold every table pointer at once.
 struct GView {
-    int64_t S, G, K;
-    gf X, Y, C;
-    gfw D;
-    gi sX, kX, lX, gY, kY, lY, sD, gD, lD, sC, gC, lC;
-    float alpha, beta;
-    int beta_zero, round, round_C, y_kfast;
-    __device__ __forceinline__ explicit GView(const tk_prob& p)
-        : S(p.S), G(p.G), K(p.K), X((gf)p.X), Y((gf)p.Y), C((gf)p.C), D((gfw)p.D), sX((gi)p.sX), kX((gi)p.kX),
-          lX((gi)p.lX), gY((gi)p.gY), kY((gi)p.kY), lY((gi)p.lY), sD((gi)p.sD), gD((gi)p.gD), lD((gi)p.lD),
-          sC((gi)p.sC), gC((gi)p.gC), lC((gi)p.lC), alpha(p.alpha), beta(p.beta), beta_zero(p.beta_zero),
-          round(p.round), round_C(p.round_C), y_kfast(p.y_kfast)
-    {
-    }
+    const tk_prob TK_CONST* d;
+    __device__ __forceinline__ explicit GView(const tk_prob TK_CONST* p) : d(p) {}
+    __device__ __forceinline__ int64_t S() const { return d->S; }
+    __device__ __forceinline__ int64_t G() const { return d->G; }
+    __device__ __forceinline__ int64_t K() const { return d->K; }
+    __device__ __forceinline__ gf X() const { return (gf)d->X; }
+    __device__ __forceinline__ gf Y() const { return (gf)d->Y; }
+    __device__ __forceinline__ gf C() const { return (gf)d->C; }
+    __device__ __forceinline__ gfw D() const { return (gfw)d->D; }
+    __device__ __forceinline__ gi sX() const { return (gi)d->sX; }
+    __device__ __forceinline__ gi kX() const { return (gi)d->kX; }
+    __device__ __forceinline__ gi lX() const { return (gi)d->lX; }
+    __device__ __forceinline__ gi gY() const { return (gi)d->gY; }
+    __device__ __forceinline__ gi kY() const { return (gi)d->kY; }
+    __device__ __forceinline__ gi lY() const { return (gi)d->lY; }
+    __device__ __forceinline__ gi sD() const { return (gi)d->sD; }
+    __device__ __forceinline__ gi gD() const { return (gi)d->gD; }
+    __device__ __forceinline__ gi lD() const { return (gi)d->lD; }
+    __device__ __forceinline__ gi sC() const { return (gi)d->sC; }
+    __device__ __forceinline__ gi gC() const { return (gi)d->gC; }
+    __device__ __forceinline__ gi lC() const { return (gi)d->lC; }
+    __device__ __forceinline__ float alpha() const { return d->alpha; }
+    __device__ __forceinline__ float beta() const { return d->beta; }
+    __device__ __forceinline__ int beta_zero() const { return d->beta_zero; }
+    __device__ __forceinline__ int round() const { return d->round; }
+    __device__ __forceinline__ int round_C() const { return d->round_C; }
+    __device__ __forceinline__ int y_kfast() const { return d->y_kfast; }
 };
 
 __device__ __forceinline__ float rnd_in(float x, int mode)
@@ -67,16 +87,16 @@ __device__ __forceinline__ float rnd_in(float x, int mode)
 // load issued after a store has to wait for it.
 __device__ __forceinline__ float group_c(const GView& p, int64_t l, int64_t s, int64_t g)
 {
-    if (p.beta_zero) return 0.0f;
-    float c = p.C[p.lC[l] + p.sC[s] + p.gC[g]];
-    return p.round_C ? rnd_in(c, p.round) : c;
+    if (p.beta_zero()) return 0.0f;
+    float c = p.C()[p.lC()[l] + p.sC()[s] + p.gC()[g]];
+    return p.round_C() ? rnd_in(c, p.round()) : c;
 }
 
 __device__ __forceinline__ void group_store(const GView& p, int64_t l, int64_t s, int64_t g, float acc, float c)
 {
-    float out = p.alpha * acc;
-    if (!p.beta_zero) out += p.beta * c;
-    p.D[p.lD[l] + p.sD[s] + p.gD[g]] = out;
+    float out = p.alpha() * acc;
+    if (!p.beta_zero()) out += p.beta() * c;
+    p.D()[p.lD()[l] + p.sD()[s] + p.gD()[g]] = out;
 }
 
 template <int RM>
@@ -103,39 +123,39 @@ __device__ __forceinline__ void thin_tile(const GView& p, int64_t l, int64_t g0,
     float acc[SB];
 #pragma unroll
     for (int s = 0; s < SB; ++s) acc[s] = 0.0f;
-    const int64_t lx = p.lX[l], ly = p.lY[l];
-    const int gn = (int)(p.G - g0 < GT ? p.G - g0 : GT);
+    const int64_t lx = p.lX()[l], ly = p.lY()[l];
+    const int gn = (int)(p.G() - g0 < GT ? p.G() - g0 : GT);
     // column offsets of this tile, once (the K loop reuses them); column 0's
     // offset stands in for columns past the end
     __syncthreads();
-    goff[tid] = p.gY[g0 + (tid < gn ? tid : 0)];
+    goff[tid] = p.gY()[g0 + (tid < gn ? tid : 0)];
     // this lane's X row (fixed for the tile)
     const int xs = tid / KC, xk = tid % KC;
-    const bool x_row = tid < SB * KC && xs < p.S;
-    const int64_t xrow = lx + p.sX[x_row ? xs : 0];
+    const bool x_row = tid < SB * KC && xs < p.S();
+    const int64_t xrow = lx + p.sX()[x_row ? xs : 0];
     __syncthreads();
-    const bool dense = p.y_kfast == 2;
+    const bool dense = p.y_kfast() == 2;
     const int64_t g0off = goff[0];
     float yv[NY], xv = 0.0f;
     // dense chunk loads: element (k, gg) at ly + g0off + gg K + k0 + k, one
     // base per chunk and 32-bit offsets (the host takes this path only when
     // K < 2^21)
     auto fetch_dense = [&](int64_t k0) {
-        const int kc = (int)(p.K - k0 < KC ? p.K - k0 : KC);
-        xv = p.X[xrow + p.kX[k0 + (xk < kc ? xk : 0)]];
-        gf yb = p.Y + ly + g0off + k0;
+        const int kc = (int)(p.K() - k0 < KC ? p.K() - k0 : KC);
+        xv = p.X()[xrow + p.kX()[k0 + (xk < kc ? xk : 0)]];
+        gf yb = p.Y() + ly + g0off + k0;
         const uint32_t kk = (uint32_t)(xk < kc ? xk : 0);
 #pragma unroll
         for (int i = 0; i < NY; ++i) {
             const int gg = tid / KC + (256 / KC) * i;
-            yv[i] = yb[(uint32_t)(gg < gn ? gg : 0) * (uint32_t)p.K + kk];
+            yv[i] = yb[(uint32_t)(gg < gn ? gg : 0) * (uint32_t)p.K() + kk];
         }
     };
-    if (dense && p.K > 0) fetch_dense(0);
-    for (int64_t k0 = 0; k0 < p.K; k0 += KC) {
-        const int kc = (int)(p.K - k0 < KC ? p.K - k0 : KC);
+    if (dense && p.K() > 0) fetch_dense(0);
+    for (int64_t k0 = 0; k0 < p.K(); k0 += KC) {
+        const int kc = (int)(p.K() - k0 < KC ? p.K() - k0 : KC);
         __syncthreads();                          // the previous chunk's LDS reads are done
-        if (!dense) xv = p.X[xrow + p.kX[k0 + (xk < kc ? xk : 0)]];
+        if (!dense) xv = p.X()[xrow + p.kX()[k0 + (xk < kc ? xk : 0)]];
         if (tid < SB * KC) Xl[tid] = (x_row && xk < kc) ? rnd_m<RM>(xv) : 0.0f;
         if (dense) {
 #pragma unroll
@@ -143,13 +163,13 @@ __device__ __forceinline__ void thin_tile(const GView& p, int64_t l, int64_t g0,
                 const int gg = tid / KC + (256 / KC) * i;
                 Yl[xk * YP + gg] = (xk < kc && gg < gn) ? rnd_m<RM>(yv[i]) : 0.0f;
             }
-        } else if (p.y_kfast) {
+        } else if (p.y_kfast()) {
             const int k = tid % KC;
-            const int64_t ko = p.kY[k0 + (k < kc ? k : 0)];
+            const int64_t ko = p.kY()[k0 + (k < kc ? k : 0)];
 #pragma unroll 8
             for (int gg = tid / KC; gg < GT; gg += 256 / KC) {
                 float v = 0.0f;
-                if (k < kc && gg < gn) v = rnd_m<RM>(p.Y[ly + ko + goff[gg]]);
+                if (k < kc && gg < gn) v = rnd_m<RM>(p.Y()[ly + ko + goff[gg]]);
                 Yl[k * YP + gg] = v;
             }
         } else {
@@ -157,14 +177,14 @@ __device__ __forceinline__ void thin_tile(const GView& p, int64_t l, int64_t g0,
 #pragma unroll 8
             for (int k = 0; k < KC; ++k) {
                 float v = 0.0f;
-                if (k < kc && gg < gn) v = rnd_m<RM>(p.Y[ly + p.kY[k0 + k] + goff[gg]]);
+                if (k < kc && gg < gn) v = rnd_m<RM>(p.Y()[ly + p.kY()[k0 + k] + goff[gg]]);
                 Yl[k * YP + gg] = v;
             }
         }
         __syncthreads();
         // the registers are free again: the next dense chunk's loads go out
         // now and are in flight during this chunk's arithmetic
-        if (dense && k0 + KC < p.K) fetch_dense(k0 + KC);
+        if (dense && k0 + KC < p.K()) fetch_dense(k0 + KC);
         for (int k = 0; k < kc; ++k) {
             const float y = Yl[k * YP + tid];
 #pragma unroll
@@ -172,13 +192,13 @@ __device__ __forceinline__ void thin_tile(const GView& p, int64_t l, int64_t g0,
         }
     }
     const int64_t g = g0 + tid;
-    if (g < p.G) {
+    if (g < p.G()) {
         float cv[SB];
 #pragma unroll
-        for (int s = 0; s < SB; ++s) cv[s] = s < p.S ? group_c(p, l, s, g) : 0.0f;
+        for (int s = 0; s < SB; ++s) cv[s] = s < p.S() ? group_c(p, l, s, g) : 0.0f;
 #pragma unroll
         for (int s = 0; s < SB; ++s)
-            if (s < p.S) group_store(p, l, s, g, acc[s], cv[s]);
+            if (s < p.S()) group_store(p, l, s, g, acc[s], cv[s]);
     }
 }
 
@@ -212,9 +232,9 @@ __device__ __forceinline__ uint32_t pack16(float lo, float hi)
 }
 
 template <int RM>
-__device__ __forceinline__ void mfma_chunk(const GView& p, const int64_t (&arow)[2], const int64_t (&bcol)[2],
-                                           const bool (&a_ok)[2], const bool (&b_ok)[2], const int64_t* kox,
-                                           const int64_t* koy, int kc, int kl, f32x16 (&acc)[2][2])
+__device__ __forceinline__ void mfma_chunk(gf Xb, gf Yb, const int32_t (&arow)[2], const int32_t (&bcol)[2],
+                                           const bool (&a_ok)[2], const bool (&b_ok)[2], const int32_t* kox,
+                                           const int32_t* koy, int kc, int kl, f32x16 (&acc)[2][2])
 {
     if (RM == 0) {
         // two register sets: the loads of the next JB steps are in flight
@@ -225,11 +245,11 @@ __device__ __forceinline__ void mfma_chunk(const GView& p, const int64_t (&arow)
 #pragma unroll
             for (int j = 0; j < JB; ++j) {
                 const int k = j0 + 2 * j + kl;
-                const int64_t ka = kox[k < kc ? k : 0], kb = koy[k < kc ? k : 0];
+                const int32_t ka = kox[k < kc ? k : 0], kb = koy[k < kc ? k : 0];
 #pragma unroll
                 for (int i = 0; i < 2; ++i) {
-                    a[j][i] = p.X[arow[i] + ka];
-                    b[j][i] = p.Y[bcol[i] + kb];
+                    a[j][i] = Xb[(uint32_t)(arow[i] + ka)];
+                    b[j][i] = Yb[(uint32_t)(bcol[i] + kb)];
                 }
             }
         };
@@ -260,15 +280,15 @@ __device__ __forceinline__ void mfma_chunk(const GView& p, const int64_t (&arow)
         // set for the next step's loads would spill beside 64 accumulators)
         for (int j0 = 0; j0 < kc; j0 += 16) {
             uint4 ua[2], ub[2];
-            auto load_pack = [&](const float TK_GLOBAL* base, const int64_t (&off)[2], const bool (&ok)[2],
-                                 const int64_t* ko, uint4 (&out)[2]) {
+            auto load_pack = [&](gf base, const int32_t (&off)[2], const bool (&ok)[2], const int32_t* ko,
+                                 uint4 (&out)[2]) {
                 float v[2][8];
 #pragma unroll
                 for (int t = 0; t < 8; ++t) {
                     const int k = j0 + 8 * kl + t;
-                    const int64_t kk = ko[k < kc ? k : 0];
+                    const int32_t kk = ko[k < kc ? k : 0];
 #pragma unroll
-                    for (int i = 0; i < 2; ++i) v[i][t] = base[off[i] + kk];
+                    for (int i = 0; i < 2; ++i) v[i][t] = base[(uint32_t)(off[i] + kk)];
                 }
 #pragma unroll
                 for (int i = 0; i < 2; ++i) {
@@ -281,8 +301,8 @@ __device__ __forceinline__ void mfma_chunk(const GView& p, const int64_t (&arow)
                     out[i] = make_uint4(pk[0], pk[1], pk[2], pk[3]);
                 }
             };
-            load_pack(p.X, arow, a_ok, kox, ua);
-            load_pack(p.Y, bcol, b_ok, koy, ub);
+            load_pack(Xb, arow, a_ok, kox, ua);
+            load_pack(Yb, bcol, b_ok, koy, ub);
 #pragma unroll
             for (int ib = 0; ib < 2; ++ib)
 #pragma unroll
@@ -304,49 +324,55 @@ __device__ __forceinline__ void mfma_tile(const GView& p, int64_t l, int64_t m0,
 {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int il = lane & 31, kl = lane >> 5;
-    const int64_t lx = p.lX[l], ly = p.lY[l];
+    const int64_t lx = p.lX()[l], ly = p.lY()[l];
     const int64_t wm0 = m0 + 64 * (wave & 1), wn0 = n0 + 64 * (wave >> 1);
     // a wave whose block lies past the edge only helps stage the K offsets
-    const bool active = wm0 < p.S && wn0 < p.G;
-    int64_t arow[2], bcol[2];
+    const bool active = wm0 < p.S() && wn0 < p.G();
+    // operands addressed from their lowest readable element with 32-bit
+    // offsets (the host checked every element lies within 2^29 of it)
+    const int64_t xlo = p.d->xlo, ylo = p.d->ylo;
+    gf Xb = p.X() + xlo, Yb = p.Y() + ylo;
+    int32_t arow[2], bcol[2];
     bool a_ok[2], b_ok[2];
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
         const int64_t m = wm0 + 32 * i + il, n = wn0 + 32 * i + il;
-        a_ok[i] = m < p.S;
-        b_ok[i] = n < p.G;
-        arow[i] = lx + p.sX[a_ok[i] ? m : 0];
-        bcol[i] = ly + p.gY[b_ok[i] ? n : 0];
+        a_ok[i] = m < p.S();
+        b_ok[i] = n < p.G();
+        arow[i] = (int32_t)(lx + p.sX()[a_ok[i] ? m : 0] - xlo);
+        bcol[i] = (int32_t)(ly + p.gY()[b_ok[i] ? n : 0] - ylo);
     }
     f32x16 acc[2][2];
 #pragma unroll
     for (int ib = 0; ib < 2; ++ib)
 #pragma unroll
         for (int ia = 0; ia < 2; ++ia) acc[ib][ia] = f32x16{};
-    for (int64_t k0 = 0; k0 < p.K; k0 += KO) {
-        const int kc = (int)(p.K - k0 < KO ? p.K - k0 : KO);
+    int32_t* kx = reinterpret_cast<int32_t*>(kox);
+    int32_t* ky = reinterpret_cast<int32_t*>(koy);
+    for (int64_t k0 = 0; k0 < p.K(); k0 += KO) {
+        const int kc = (int)(p.K() - k0 < KO ? p.K() - k0 : KO);
         __syncthreads();                         // the previous chunk's offsets are no longer read
         for (int k = tid; k < kc; k += 256) {
-            kox[k] = p.kX[k0 + k];
-            koy[k] = p.kY[k0 + k];
+            kx[k] = (int32_t)p.kX()[k0 + k];
+            ky[k] = (int32_t)p.kY()[k0 + k];
         }
         __syncthreads();
-        if (active) mfma_chunk<RM>(p, arow, bcol, a_ok, b_ok, kox, koy, kc, kl, acc);
+        if (active) mfma_chunk<RM>(Xb, Yb, arow, bcol, a_ok, b_ok, kx, ky, kc, kl, acc);
     }
     // the tile's row and column offsets of C and D, staged once in LDS (the
     // K offsets are no longer read): kox = C rows | D rows, koy = C columns
     // | D columns; an output then costs one C load and one D store
     __syncthreads();
     if (tid < MT) {
-        const int64_t m = m0 + tid < p.S ? m0 + tid : 0, n = n0 + tid < p.G ? n0 + tid : 0;
-        kox[tid] = p.beta_zero ? 0 : p.sC[m];
-        kox[MT + tid] = p.sD[m];
-        koy[tid] = p.beta_zero ? 0 : p.gC[n];
-        koy[MT + tid] = p.gD[n];
+        const int64_t m = m0 + tid < p.S() ? m0 + tid : 0, n = n0 + tid < p.G() ? n0 + tid : 0;
+        kox[tid] = p.beta_zero() ? 0 : p.sC()[m];
+        kox[MT + tid] = p.sD()[m];
+        koy[tid] = p.beta_zero() ? 0 : p.gC()[n];
+        koy[MT + tid] = p.gD()[n];
     }
     __syncthreads();
     if (!active) return;
-    const int64_t lc = p.beta_zero ? 0 : p.lC[l], ld = p.lD[l];
+    const int64_t lc = p.beta_zero() ? 0 : p.lC()[l], ld = p.lD()[l];
     // acc[ib][ia][r] of lane holds out[wn0 + 32 ib + 8 (r / 4) + 4 kl + r % 4][wm0 + 32 ia + il];
     // per 32 x 32 block, the lane's C values are read before its D stores
 #pragma unroll
@@ -361,19 +387,19 @@ __device__ __forceinline__ void mfma_tile(const GView& p, int64_t l, int64_t m0,
             for (int r = 0; r < 16; ++r) {
                 const int nt = (int)(wn0 - n0) + 32 * ib + 8 * (r >> 2) + 4 * kl + (r & 3);
                 float c = 0.0f;
-                if (!p.beta_zero && n0 + nt < p.G) {
-                    c = p.C[rc + koy[nt]];
-                    if (p.round_C) c = rnd_in(c, p.round);
+                if (!p.beta_zero() && n0 + nt < p.G()) {
+                    c = p.C()[rc + koy[nt]];
+                    if (p.round_C()) c = rnd_in(c, p.round());
                 }
                 cv[r] = c;
             }
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
                 const int nt = (int)(wn0 - n0) + 32 * ib + 8 * (r >> 2) + 4 * kl + (r & 3);
-                if (n0 + nt < p.G) {
-                    float out = p.alpha * acc[ib][ia][r];
-                    if (!p.beta_zero) out += p.beta * cv[r];
-                    p.D[rd + koy[MT + nt]] = out;
+                if (n0 + nt < p.G()) {
+                    float out = p.alpha() * acc[ib][ia][r];
+                    if (!p.beta_zero()) out += p.beta() * cv[r];
+                    p.D()[rd + koy[MT + nt]] = out;
                 }
             }
         }
@@ -383,8 +409,8 @@ __device__ __forceinline__ void mfma_tile(const GView& p, int64_t l, int64_t m0,
 template <int RM>
 __device__ __forceinline__ void thin_dispatch(const GView& p, int64_t l, int64_t g0, float* lds, int64_t* goff)
 {
-    if (p.S <= 4) thin_tile<4, RM>(p, l, g0, lds, goff);
-    else if (p.S <= 8) thin_tile<8, RM>(p, l, g0, lds, goff);
+    if (p.S() <= 4) thin_tile<4, RM>(p, l, g0, lds, goff);
+    else if (p.S() <= 8) thin_tile<8, RM>(p, l, g0, lds, goff);
     else thin_tile<16, RM>(p, l, g0, lds, goff);
 }
 
@@ -435,21 +461,24 @@ __global__ __launch_bounds__(256, TK_GROUP_WG_PER_CU) void k_group(const tk_prob
                 lo = a;
             }
             lo = __builtin_amdgcn_readfirstlane(lo);
-            const tk_prob& pd = probs[lo];
+            // the descriptor through the constant address space: its scalar
+            // loads are invariant, so they are re-issued where a field is
+            // used instead of holding every field in registers for the tile
+            const tk_prob TK_CONST* pd = (const tk_prob TK_CONST*)probs + lo;
             const GView p(pd);
             const int64_t local = t - (lo ? tile_end[lo - 1] : 0);
-            const int64_t tiles_s = pd.tiles_s, tiles_g = pd.tiles_g;
-            const int cls = pd.cls;
+            const int64_t tiles_s = pd->tiles_s, tiles_g = pd->tiles_g;
+            const int cls = pd->cls;
             const int64_t per_l = tiles_s * tiles_g;
             const int64_t l = local / per_l, rem = local % per_l;
             const int64_t ts = rem / tiles_g, tg = rem % tiles_g;
             if (cls == TK_GROUP_THIN) {
-                if (p.round == TK_MODE_BF16) thin_dispatch<TK_MODE_BF16>(p, l, tg * GT, lds, goff);
-                else if (p.round == TK_MODE_F16) thin_dispatch<TK_MODE_F16>(p, l, tg * GT, lds, goff);
+                if (p.round() == TK_MODE_BF16) thin_dispatch<TK_MODE_BF16>(p, l, tg * GT, lds, goff);
+                else if (p.round() == TK_MODE_F16) thin_dispatch<TK_MODE_F16>(p, l, tg * GT, lds, goff);
                 else thin_dispatch<0>(p, l, tg * GT, lds, goff);
             } else {
-                if (p.round == TK_MODE_BF16) mfma_tile<TK_MODE_BF16>(p, l, ts * MT, tg * MT, kox, koy);
-                else if (p.round == TK_MODE_F16) mfma_tile<TK_MODE_F16>(p, l, ts * MT, tg * MT, kox, koy);
+                if (p.round() == TK_MODE_BF16) mfma_tile<TK_MODE_BF16>(p, l, ts * MT, tg * MT, kox, koy);
+                else if (p.round() == TK_MODE_F16) mfma_tile<TK_MODE_F16>(p, l, ts * MT, tg * MT, kox, koy);
                 else mfma_tile<0>(p, l, ts * MT, tg * MT, kox, koy);
             }
         }

@@ -126,6 +126,14 @@ bool tk_group_describe(struct tapp_plan* plan, struct tapp_exec* ex, const tk_pr
     }
     out->lD = lD; out->lC = lC;
     out->K = P->nK; out->L = P->nL;
+    if (out->cls == TK_GROUP_MFMA) {
+        /* 32-bit offsets from the lowest element of each operand */
+        const int xi = x_is_b ? 1 : 0, yi = 1 - xi;
+        const int64_t lim = (int64_t)1 << 29;
+        if (P->hi[xi] - P->lo[xi] >= lim || P->hi[yi] - P->lo[yi] >= lim) return false;
+        out->xlo = P->lo[xi] - P->pin[xi];
+        out->ylo = P->lo[yi] - P->pin[yi];
+    }
     out->C = c; out->D = d;
     out->alpha = (float)P->alpha[0];
     out->beta = (float)P->beta[0];

@@ -122,10 +122,10 @@ typedef struct tk_scatter {
 enum { TK_GROUP_THIN = 0, TK_GROUP_MFMA = 1 };
 
 /* Persistent group workgroups per CU: the group kernel's occupancy bound
- * (launch bounds) and the grid the host launches.  Four 256-lane workgroups
- * (24 KB of LDS, 128 VGPRs each): the MFMA tile's four 64 x 64 wave blocks
- * hold 64 accumulators per lane, and the thin tile's prefetch needs the
- * rest. */
+ * (launch bounds) and the grid the host launches.  Three 256-lane
+ * workgroups (24 KB of LDS, 168 VGPRs each): the MFMA tile's four 64 x 64
+ * wave blocks hold 64 accumulators per lane beside two batches of loads in
+ * flight, which four per CU (128 VGPRs) would spill. */
 #define TK_GROUP_WG_PER_CU 3
 /* Edge of a TK_GROUP_MFMA output tile. */
 #define TK_GROUP_MT 128
@@ -148,6 +148,9 @@ typedef struct tk_prob {
     int round;          /* 0 or TK_MODE_BF16 / TK_MODE_F16: inputs rounded (FP32 accumulate) */
     int round_C;
     int pad;
+    /* MFMA tiles: X + xlo (Y + ylo) is the lowest element the tile can read,
+     * and every element lies within 2^29 of it, so offsets are 32-bit */
+    int64_t xlo, ylo;
 } tk_prob;
 
 /* All launchers are asynchronous on `stream` (a hipStream_t) and return 0
