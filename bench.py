@@ -105,9 +105,10 @@ def time_form(lib, torch, prec, split, A, B, D, steps, warmup):
     torch.cuda.synchronize()
     wall = (time.perf_counter() - t0) / steps
     kern = lib.attr_get_int(ex, tc.ATTR_LAST_KERNEL)[1]
+    name = lib.last_kernel_name(ex)[1]
     p.close()
     lib.destroy_executor(ex)
-    return wall, float(np.mean(dev_us)), kern
+    return wall, float(np.mean(dev_us)), (kern, name)
 
 
 def fp64_reference(lib, torch, A, B):
@@ -403,6 +404,97 @@ def batching_bench(lib, torch, steps, prec=tc.DEFAULT_PREC):
                           "create_ms_with_tuning": tuned_create_s * 1e3, "max_rel_err_group_vs_loop": tuned_err}}
 
 
+def batched_product_bench(lib, torch, steps, count=8192, shape=(32, 32, 32, 8)):
+    """TAPP_execute_batched_product on one plan -- the Tucker product
+    D[r,j,k] = sum_i A[i,j,k] U[r,i], i = j = k = 32, r = 8, FP32 -- over
+    `count` operand sets: the first call (builds the batch and keeps it on
+    the plan), repeated calls with the same pointers, calls alternating
+    between two operand pools (every descriptor patched and uploaded), and
+    the same products as a product group replayed."""
+    i, j, k, r = shape
+    L = lambda t: [ord(c) for c in t]  # noqa: E731
+    ops = [([i, j, k], "ijk"), ([r, i], "ri"), ([r, j, k], "rjk"), ([r, j, k], "rjk")]
+    sizes = [int(np.prod(e)) for e, _ in ops]
+    per = sum(sizes)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(17)
+    pool = torch.rand(2 * count * per, generator=g, device="cuda", dtype=torch.float32).mul_(2.0).sub_(1.0)
+    torch.cuda.synchronize()   # written on torch's stream before the library reads it
+    base = pool.data_ptr()
+    sets = []
+    for q in range(2):
+        cols = [[], [], [], []]
+        for n in range(count):
+            off = (q * count + n) * per
+            for x in range(4):
+                cols[x].append(base + 4 * off)
+                off += sizes[x]
+        sets.append(cols)
+    infos = []
+    for ext, _ in ops:
+        st, acc = [], 1
+        for x in ext:
+            st.append(acc)
+            acc *= x
+        infos.append(lib.create_info(tc.F32, ext, st)[1])
+    _, plan = lib.create_plan([0, 0, 0, 0], infos, [L(x[1]) for x in ops], tc.DEFAULT_PREC)
+    al, be = np.ones(1, np.float32), np.zeros(1, np.float32)
+    _, ex = lib.create_executor()
+    lib.attr_set_int(ex, tc.ATTR_TIMING, 1)
+
+    arrays = [[lib.pointer_array(col) for col in cols] for cols in sets]   # built once, as a C caller would
+
+    def call(q):
+        a, b, _, d = arrays[q]
+        rc = lib.execute_batched(plan, ex, al.ctypes.data, a, b, be.ctypes.data, None, d)
+        assert rc == 0, lib.explain(rc)
+        return lib.attr_get_int(ex, tc.ATTR_LAST_REUSED)[1], lib.attr_get_double(ex, tc.ATTR_LAST_DEVICE_US)[1]
+
+    t0 = time.perf_counter()
+    reused_first, _ = call(0)
+    first_s = time.perf_counter() - t0
+    out = {"first_call_ms": first_s * 1e3, "first_call_reused": reused_first}
+    for label, seq in (("same_pointers", [0] * steps), ("alternating_pools", [1, 0] * max(1, steps // 2))):
+        call(seq[0])
+        torch.cuda.synchronize()
+        reused, dev = [], []
+        t0 = time.perf_counter()
+        for q in seq:
+            rz, du = call(q)
+            reused.append(rz)
+            dev.append(du)
+        wall = (time.perf_counter() - t0) / len(seq)
+        out[label] = {"call_ms": wall * 1e3, "kernel_ms": float(np.mean(dev)) * 1e-3, "all_reused": all(reused)}
+    D_batched = [pool[(p - base) // 4:(p - base) // 4 + sizes[3]].clone() for p in sets[0][3]]
+    torch.cuda.synchronize()
+    # the same products as a product group, replayed
+    rc, grp = lib.create_group(ex, [plan] * count, [al.ctypes.data] * count, sets[0][0], sets[0][1],
+                               [be.ctypes.data] * count, [None] * count, sets[0][3])
+    assert rc == 0, lib.explain(rc)
+    assert lib.execute_group(grp) == 0
+    dev = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        assert lib.execute_group(grp) == 0
+        dev.append(lib.attr_get_double(ex, tc.ATTR_LAST_DEVICE_US)[1])
+    group_s = (time.perf_counter() - t0) / steps
+    D_group = [pool[(p - base) // 4:(p - base) // 4 + sizes[3]].clone() for p in sets[0][3]]
+    torch.cuda.synchronize()
+    lib.destroy_group(grp)
+    lib.destroy_executor(ex)
+    lib.destroy_plan(plan)
+    for h in infos:
+        lib.destroy_info(h)
+    del pool
+    flop = 2.0 * i * j * k * r * count
+    out.update({"workload": f"{count} operand sets of D[r,j,k] = sum_i A[i,j,k] U[r,i], i=j=k={i}, r={r}, FP32, "
+                            "beta 0, one plan",
+                "group_replay": {"call_ms": group_s * 1e3, "kernel_ms": float(np.mean(dev)) * 1e-3},
+                "batched_equals_group_bitwise": all(torch.equal(x, y) for x, y in zip(D_batched, D_group)),
+                "gflops_repeated_call": flop / (out["same_pointers"]["call_ms"] * 1e-3) / 1e9})
+    return out
+
+
 def serving_bench(lib, torch, rates, nreq, split=False, seed=3):
     """open-loop Poisson arrivals into the serving loop.
     Requests: 5 % large ladder-form products (virtual extent 64:
@@ -572,6 +664,7 @@ def config_cases(lib, torch, steps):
             dev.append(lib.attr_get_double(ex, tc.ATTR_LAST_DEVICE_US)[1])
         wall = (time.perf_counter() - t0) / steps
         d_us = float(np.mean(dev))
+        kname = lib.last_kernel_name(ex)[1]
         torch.cuda.synchronize()
         operands[name] = [b.cpu() for b in bufs]     # D holds the GPU result
         lib.destroy_executor(ex)
@@ -579,7 +672,8 @@ def config_cases(lib, torch, steps):
         for h in infos:
             lib.destroy_info(h)
         ideal = max(flop / (PEAK_FP32_TFLOPS * 1e12 if ctype == tc.F32 else 78.6e12), bytes_ / (PEAK_HBM_GBS * 1e9))
-        out[name] = {"call_us": wall * 1e6, "device_us": d_us, "gflops": flop / (d_us * 1e-6) / 1e9,
+        out[name] = {"call_us": wall * 1e6, "device_us": d_us, "kernel": kname,
+                     "gflops": flop / (d_us * 1e-6) / 1e9,
                      "algorithmic_bytes": bytes_, "achieved_gbs": bytes_ / (d_us * 1e-6) / 1e9,
                      "roofline_frac": ideal / (d_us * 1e-6), "ideal_us": ideal * 1e6}
 
@@ -639,16 +733,19 @@ def cpu_reference_configs(cases):
 
 
 def measured_traffic():
-    """HBM bytes per launch of the FP32 ladder kernel from the committed
+    """HBM bytes per launch of the FP32 ladder kernel, read from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE passes (scaled by the copy-kernel
-    calibration factors recorded beside them)."""
+    calibration factors recorded beside them): not measured by this run.
+    Returns (bytes or None, where the figure comes from)."""
     path = os.path.join(ROOT, "profiles", "ladder_counters.json")
     try:
         with open(path) as f:
             c = json.load(f)["ladder_fp32"]
-        return c["fetch_kib"] * 1024 * c["fetch_factor"] + c["write_kib"] * 1024 * c["write_factor"]
+        b = c["fetch_kib"] * 1024 * c["fetch_factor"] + c["write_kib"] * 1024 * c["write_factor"]
+        return b, (f"read from profiles/ladder_counters.json (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE passes over "
+                   f"{c['kernel']}; see {c['source']}), not measured in this run")
     except (OSError, KeyError, ValueError):
-        return None
+        return None, "profiles/ladder_counters.json not found"
 
 
 def main():
@@ -700,7 +797,8 @@ def main():
                        "f16": (tc.F16F16_ACCUM_F32, 0), "split_bf16x3": (tc.DEFAULT_PREC, 1),
                        "split_bf16x6": (tc.DEFAULT_PREC, 2), "split_f16x3": (tc.DEFAULT_PREC, 3)}[args.only_form]
         w, du, k = time_form(lib, torch, prec, split, A, B, D, args.steps, args.warmup)
-        print(json.dumps({"form": args.only_form, "kernel_us": du, "gflops": flops / w / 1e9, "kernel": k}))
+        print(json.dumps({"form": args.only_form, "kernel_us": du, "gflops": flops / w / 1e9, "kernel_family": k[0],
+                          "kernel": k[1]}))
         return
 
     log(f"bench: FP32 plain, warmup {args.warmup}, steps {args.steps}")
@@ -734,7 +832,8 @@ def main():
             log(f"bench: form {name}")
             w, du, k = time_form(lib, torch, prec, split, A, B, D, max(1, args.steps // 4), 1)
             err = (D.double() - D64).abs().max().item() / den
-            forms[name] = {"gflops": flops / w / 1e9, "kernel_us": du, "kernel": k, "max_rel_err_vs_fp64": err,
+            forms[name] = {"gflops": flops / w / 1e9, "kernel_us": du, "kernel_family": k[0], "kernel": k[1],
+                           "max_rel_err_vs_fp64": err,
                            "elementwise_rel_err_vs_fp64": elementwise(D),
                            "roofline_frac": flops / (du * 1e-6) / (peak_unit[name] * 1e12),
                            "peak_unit_tflops": peak_unit[name],
@@ -762,6 +861,9 @@ def main():
         log("bench: uneven batch, one persistent launch vs one call per product")
         batched["fp32"] = batching_bench(lib, torch, max(2, args.steps // 2))
         batched["bf16"] = batching_bench(lib, torch, max(2, args.steps // 2), prec=tc.BF16BF16_ACCUM_F32)
+        torch.cuda.empty_cache()
+        log("bench: TAPP_execute_batched_product, repeated calls on one plan")
+        batched["batched_product"] = batched_product_bench(lib, torch, max(4, args.steps // 2))
 
     serving = {}
     if not args.no_serve:
@@ -771,6 +873,7 @@ def main():
         serving["fp32"] = serving_bench(lib, torch, rates, args.serve_requests)
         serving["split_bf16"] = serving_bench(lib, torch, [5000], args.serve_requests, split=True)
 
+    traffic, traffic_src = measured_traffic()
     out = {
         "metric": "GFLOP/s + % MFMA/HBM roofline, split-BF16 vs FP32 GPU; p50/p99 req latency",
         "value": value,
@@ -788,8 +891,9 @@ def main():
                                "FP32, TAPP_DEFAULT_PREC",
                    "M": V * V, "N": O * O, "K": V * V, "flop_per_step": flops},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": measured_traffic(),
-                     "kernel": "k_gett_f32<CONTIG_ROW, CONTIG_K> (v_mfma_f32_32x32x2f32)", "kernel_us": dev_us, "kernel_family": kern},
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
+                     "traffic_source": traffic_src,
+                     "kernel": kern[1], "kernel_us": dev_us, "kernel_family": kern[0]},
         "cpu_baseline": cpu,
         "forms": forms,
         "configs": others,

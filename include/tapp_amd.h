@@ -96,7 +96,14 @@ enum {
      * differ.  Setting it writes 12 (or 1) into the executor's kind, so
      * writing 12 into the int the executor points at, as a reference caller
      * does, has the same effect.  Default 0. */
-    TAPP_AMD_ATTR_SELF_CHECK = 0x1008
+    TAPP_AMD_ATTR_SELF_CHECK = 0x1008,
+    /* Read-only: 1 when the last TAPP_execute_batched_product on this
+     * executor reused the plan's prepared batch (descriptors patched, graph
+     * replayed), 0 when it built the batch (int). */
+    TAPP_AMD_ATTR_LAST_REUSED = 0x1009,
+    /* Read-only: name of the kernel template the last execute launched
+     * (const char*, owned by the executor, valid until its next call). */
+    TAPP_AMD_ATTR_LAST_KERNEL_NAME = 0x100A
 };
 
 /* Kernel family identifiers reported by TAPP_AMD_ATTR_LAST_KERNEL. */

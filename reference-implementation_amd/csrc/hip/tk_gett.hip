@@ -1089,6 +1089,27 @@ __global__ __launch_bounds__(256) void k_splitk_reduce(tk_gett g)
     }
 }
 
+// Name of the kernel template the calling thread launched last (read back by
+// the host for TAPP_AMD_ATTR_LAST_KERNEL_NAME).
+static thread_local char last_name[96];
+
+static const char* load_name(int m)
+{
+    return m == TK_CONTIG_ROW ? "CONTIG_ROW" : m == TK_CONTIG_K ? "CONTIG_K" : "GATHER";
+}
+
+static const char* mode_name(int m)
+{
+    switch (m) {
+    case TK_MODE_PLAIN: return "PLAIN";
+    case TK_MODE_BF16: return "BF16";
+    case TK_MODE_F16: return "F16";
+    case TK_MODE_SPLIT_BF16: return "SPLIT_BF16";
+    case TK_MODE_SPLIT_F16: return "SPLIT_F16";
+    default: return "SPLIT6_BF16";
+    }
+}
+
 template <int AL, int BL>
 static hipError_t launch_gett_f32_modes(const tk_gett& g, hipStream_t st)
 {
@@ -1108,6 +1129,8 @@ static hipError_t launch_gett_f32_modes(const tk_gett& g, hipStream_t st)
     default: TK_LAUNCH(k_gett_lp, TK_MODE_SPLIT6_BF16); break;
     }
 #undef TK_LAUNCH
+    snprintf(last_name, sizeof last_name, "%s<%s, %s, %s%s>", g.mode == TK_MODE_PLAIN ? "k_gett_f32" : "k_gett_lp",
+             load_name(AL), load_name(BL), mode_name(g.mode), split ? ", split-K" : "");
     if (g.ksplit > 1) {
         const int64_t total = tiles * g.L * 16384;
         int64_t blocks = (total + 255) / 256;
@@ -1145,6 +1168,8 @@ static hipError_t launch_simple(const tk_gett& g, hipStream_t st)
     const int64_t total = g.M * g.N * g.L;
     int64_t blocks = (total + 255) / 256;
     if (blocks > 65536) blocks = 65536;
+    snprintf(last_name, sizeof last_name, "k_simple<%s, %s>", sizeof(R) == 8 ? "double" : "float",
+             CPLX ? "complex" : "real");
     hipLaunchKernelGGL((k_simple<R, CPLX>), dim3((unsigned)blocks), dim3(256), 0, st, g);
     return hipGetLastError();
 }
@@ -1153,6 +1178,8 @@ template <typename R, bool CPLX>
 static hipError_t launch_splitk(const tk_gett& g, hipStream_t st)
 {
     const int64_t total = g.M * g.N * g.L;
+    snprintf(last_name, sizeof last_name, "k_splitk<%s, %s>", sizeof(R) == 8 ? "double" : "float",
+             CPLX ? "complex" : "real");
     hipLaunchKernelGGL((k_splitk<R, CPLX>), dim3((unsigned)total), dim3(256), 0, st, g);
     return hipGetLastError();
 }
@@ -1170,6 +1197,8 @@ static hipError_t launch_mfma(const tk_gett& g, hipStream_t st)
     constexpr int BM = 2 * FM * S, BN = 2 * FN * S;
     const int64_t tiles = ((g.M + BM - 1) / BM) * ((g.N + BN - 1) / BN);
     const int64_t ly = g.L < 65535 ? g.L : 65535;
+    snprintf(last_name, sizeof last_name, "k_mfma<%s, %s, %d, %d, %d, %d>", sizeof(R) == 8 ? "double" : "float",
+             CPLX ? "complex" : "real", S, FM, FN, BK);
     hipLaunchKernelGGL((k_mfma<R, CPLX, S, FM, FN, BK>), dim3((unsigned)tiles, (unsigned)ly), dim3(256), 0, st, g);
     return hipGetLastError();
 }
@@ -1191,6 +1220,7 @@ extern "C" int tk_launch_gett(const tk_gett* gp, void* stream)
         if (kern == TK_KERNEL_SMALL && g.mode == TK_MODE_PLAIN) {
             const int64_t tiles = ((g.M + 15) / 16) * ((g.N + 15) / 16);
             const int64_t ly = g.L < 65535 ? g.L : 65535;
+            snprintf(last_name, sizeof last_name, "k_small_f32<16>");
             hipLaunchKernelGGL((k_small_f32<16>), dim3((unsigned)tiles, (unsigned)ly), dim3(256), 0, st, g);
             return (int)hipGetLastError();
         }
@@ -1221,3 +1251,5 @@ extern "C" int tk_launch_gett(const tk_gett* gp, void* stream)
 }
 
 extern "C" int tk_kernel_count(void) { return 13; }
+
+extern "C" const char* tk_last_gett_name(void) { return last_name; }

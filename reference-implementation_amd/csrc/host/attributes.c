@@ -65,6 +65,8 @@ TAPP_error TAPP_attr_get(TAPP_attr attr, TAPP_key key, void** value)
     case TAPP_AMD_ATTR_LAST_DEVICE_US: *(double*)value = e->last_device_us; return 0;
     case TAPP_AMD_ATTR_STREAM: *(hipStream_t*)value = e->stream; return 0;
     case TAPP_AMD_ATTR_SELF_CHECK: *(int*)value = e->kind == 12; return 0;
+    case TAPP_AMD_ATTR_LAST_REUSED: *(int*)value = e->last_reused; return 0;
+    case TAPP_AMD_ATTR_LAST_KERNEL_NAME: *(const char**)value = e->last_kernel_name; return 0;
     default: return TAPP_AMD_ERR_KEY;
     }
 }

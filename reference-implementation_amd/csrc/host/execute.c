@@ -82,6 +82,13 @@ void tk_plan_cache_free(struct tapp_plan_cache* c)
     free(c);
 }
 
+const void* tk_plan_tables_id(const struct tapp_plan* plan, uint64_t* key)
+{
+    const struct tapp_plan_cache* c = plan->cache;
+    *key = c ? c->key : 0;
+    return c ? (const void*)c->dev : NULL;
+}
+
 static uint64_t mix(uint64_t h, uint64_t v)
 {
     h ^= v + 0x9e3779b97f4a7c15ull + (h << 6) + (h >> 2);
@@ -820,6 +827,7 @@ static int execute_on(struct tapp_plan* p, struct tapp_exec* ex, const void* alp
         LAUNCHCHK(tk_launch_gett(&g, st));
     }
     ex->last_kernel = kern;
+    snprintf(ex->last_kernel_name, sizeof ex->last_kernel_name, "%s", tk_last_gett_name());
 
     if (needD) {
         tk_scatter sc = {ct, P.type[3], ws + off_tmp[3], (void*)d[3].base, M, N, L,

@@ -37,6 +37,10 @@ struct tapp_group {
     hipGraph_t graph;
     hipGraphExec_t gexec;
     bool have_graph;
+    /* kept for a batched call's reuse: the descriptors in launch order
+     * (pinned host memory) and the operand set each came from */
+    tk_prob* hprobs;
+    int* hsrc;
     /* products executed one by one */
     int nrest;
     struct tapp_plan** rest_plan;
@@ -125,6 +129,7 @@ bool tk_group_describe(struct tapp_plan* plan, struct tapp_exec* ex, const tk_pr
         out->y_kfast = y_access(hkB, P->nK, hnB, N);
     }
     out->lD = lD; out->lC = lC;
+    out->x_is_b = x_is_b;
     out->K = P->nK; out->L = P->nL;
     if (out->cls == TK_GROUP_MFMA) {
         /* 32-bit offsets from the lowest element of each operand */
@@ -318,6 +323,8 @@ static void group_free(struct tapp_group* g)
     if (!g) return;
     if (g->have_graph) { hipGraphExecDestroy(g->gexec); hipGraphDestroy(g->graph); }
     if (g->dmem) hipFree(g->dmem);
+    if (g->hprobs) hipHostFree(g->hprobs);
+    free(g->hsrc);
     free(g->rest_plan); free(g->rest_A); free(g->rest_B);
     free(g->rest_C); free(g->rest_D); free(g->rest_scalar);
     free(g);
@@ -326,7 +333,7 @@ static void group_free(struct tapp_group* g)
 static int create_group_on(TAPP_AMD_product_group* group, struct tapp_exec* ex, int count,
                            const TAPP_tensor_product* plans, const void* const* alpha, const void* const* A,
                            const void* const* B, const void* const* beta, const void* const* C, void* const* D,
-                           bool capture);
+                           bool capture, bool keep_host);
 
 static int create_group(TAPP_AMD_product_group* group, TAPP_executor exec, int count,
                         const TAPP_tensor_product* plans, const void* const* alpha, const void* const* A,
@@ -340,7 +347,7 @@ static int create_group(TAPP_AMD_product_group* group, TAPP_executor exec, int c
                       ? tk_default_exec() : NULL;
     if (!ex) return 33;
     tk_exec_lock(ex);
-    int rc = create_group_on(group, ex, count, plans, alpha, A, B, beta, C, D, capture);
+    int rc = create_group_on(group, ex, count, plans, alpha, A, B, beta, C, D, capture, false);
     tk_exec_unlock(ex);
     return rc;
 }
@@ -348,7 +355,7 @@ static int create_group(TAPP_AMD_product_group* group, TAPP_executor exec, int c
 static int create_group_on(TAPP_AMD_product_group* group, struct tapp_exec* ex, int count,
                            const TAPP_tensor_product* plans, const void* const* alpha, const void* const* A,
                            const void* const* B, const void* const* beta, const void* const* C, void* const* D,
-                           bool capture)
+                           bool capture, bool keep_host)
 {
     if (!ex->have_stream) return TAPP_AMD_ERR_DEVICE;
     struct tapp_group* g = calloc(1, sizeof *g);
@@ -403,7 +410,11 @@ static int create_group_on(TAPP_AMD_product_group* group, struct tapp_exec* ex, 
         tk_prob* hp = malloc(pb);
         int64_t* he = malloc(tb);
         int* bin = malloc(sizeof(int) * (size_t)n);
-        if (!hp || !he || !bin) { free(hp); free(he); free(bin); free(list); group_free(g); return TAPP_AMD_ERR_NOMEM; }
+        int* src = malloc(sizeof(int) * (size_t)n);
+        if (!hp || !he || !bin || !src) {
+            free(hp); free(he); free(bin); free(src); free(list); group_free(g);
+            return TAPP_AMD_ERR_NOMEM;
+        }
         /* one tile queue per XCD: each product goes whole to the queue with
          * the least work so far (products in cost order), so its tiles share
          * one L2; queues are concatenated, each in cost order */
@@ -420,6 +431,7 @@ static int create_group_on(TAPP_AMD_product_group* group, struct tapp_exec* ex, 
             for (int i = 0; i < n; i++) {
                 if (bin[i] != q) continue;
                 hp[k] = list[i].p;
+                src[k] = list[i].idx;
                 total += hp[k].L * hp[k].tiles_s * hp[k].tiles_g;
                 he[k++] = total;
             }
@@ -437,7 +449,7 @@ static int create_group_on(TAPP_AMD_product_group* group, struct tapp_exec* ex, 
         size_t o_q = (o_map + (hm ? mb : 0) + 255) & ~(size_t)255;
         size_t o_cnt = (o_q + sizeof qbeg + 255) & ~(size_t)255;
         hipError_t e = hipMalloc(&g->dmem, o_cnt + 256 * GROUP_QUEUES);
-        if (e != hipSuccess) { (void)hipGetLastError(); free(hp); free(he); free(hm); free(list); group_free(g); return TAPP_AMD_ERR_NOMEM; }
+        if (e != hipSuccess) { (void)hipGetLastError(); free(hp); free(he); free(hm); free(src); free(list); group_free(g); return TAPP_AMD_ERR_NOMEM; }
         g->dprobs = (tk_prob*)g->dmem;
         g->dtile_end = (int64_t*)((char*)g->dmem + o_end);
         g->dtile_prob = hm ? (int*)((char*)g->dmem + o_map) : NULL;
@@ -447,7 +459,17 @@ static int create_group_on(TAPP_AMD_product_group* group, struct tapp_exec* ex, 
         if (e == hipSuccess) e = hipMemcpy(g->dtile_end, he, tb, hipMemcpyHostToDevice);
         if (e == hipSuccess && hm) e = hipMemcpy(g->dtile_prob, hm, mb, hipMemcpyHostToDevice);
         if (e == hipSuccess) e = hipMemcpy(g->dqbeg, qbeg, sizeof qbeg, hipMemcpyHostToDevice);
-        free(hp); free(he); free(hm);
+        if (e == hipSuccess && keep_host) {
+            if (hipHostMalloc((void**)&g->hprobs, pb, hipHostMallocDefault) == hipSuccess) {
+                memcpy(g->hprobs, hp, pb);
+                g->hsrc = src;
+                src = NULL;
+            } else {
+                (void)hipGetLastError();
+                g->hprobs = NULL;
+            }
+        }
+        free(hp); free(he); free(hm); free(src);
         if (e != hipSuccess) { free(list); group_free(g); return TAPP_AMD_ERR_DEVICE; }
         int cus = 0;
         if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ex->device) != hipSuccess || cus <= 0)
@@ -480,8 +502,175 @@ TAPP_error TAPP_AMD_create_product_group(TAPP_AMD_product_group* group, TAPP_exe
 
 static int execute_group(struct tapp_group* g);
 
-/* TAPP_execute_batched_product (product.h:66-75): one plan, num_batches
- * operand sets, one alpha and beta, run as one persistent launch. */
+/* --- TAPP_execute_batched_product ------------------------------------------
+ * One plan, num_batches operand sets, one alpha and beta (product.h:66-75),
+ * run as one persistent launch.  The group built for a call is kept on the
+ * plan when every operand set ran in the persistent launch from its own
+ * device pointers; the next call with the same batch count on the same
+ * device reuses it: the plan is validated once, only operand pointers that
+ * changed are classified, their descriptors patched and uploaded (one copy
+ * from pinned memory), and the captured graph replayed.  Anything else --
+ * another batch count or device, a changed layout, precision or beta == 0,
+ * a host or resident operand, a NULL pointer -- builds the group again,
+ * with the codes and behaviour of the first call. */
+
+struct tapp_batch_cache {
+    int device, num;
+    int mode, beta_zero, round_C;
+    bool c_null;                 /* the call passed no C array */
+    uint64_t key;                /* layout key and device base of the offset tables */
+    const void* tables;          /* the descriptors point into */
+    struct tapp_group* g;        /* n == num products, nothing one by one */
+    const void **A, **B, **C;    /* operand pointers of the last call, per set */
+    void** D;
+};
+
+void tk_batch_cache_free(struct tapp_batch_cache* b)
+{
+    if (!b) return;
+    group_free(b->g);
+    free(b->A); free(b->B); free(b->C); free(b->D);
+    free(b);
+}
+
+/* Device allocations met during one call, so that operand pointers inside
+ * one already classified are not looked up again. */
+typedef struct {
+    const char* lo[8];
+    const char* hi[8];
+    int n, next;
+} range_memo;
+
+static bool memo_is_device(range_memo* m, const void* p)
+{
+    const char* c = (const char*)p;
+    for (int i = 0; i < m->n; i++)
+        if (c >= m->lo[i] && c < m->hi[i]) return true;
+    if (!tk_is_device_ptr(p)) return false;
+    void* base = NULL;
+    size_t size = 0;
+    if (hipMemGetAddressRange(&base, &size, (void*)p) == hipSuccess && size > 0) {
+        const int s = m->n < 8 ? m->n++ : (m->next++ & 7);
+        m->lo[s] = (const char*)base;
+        m->hi[s] = (const char*)base + size;
+    } else {
+        (void)hipGetLastError();
+    }
+    return true;
+}
+
+static const char* shift(const void* now, const void* was, const void* field)
+{
+    return (const char*)field + ((const char*)now - (const char*)was);
+}
+
+/* Reuse of the plan's prepared batch; returns -1 when it does not apply. */
+static int batched_reuse(struct tapp_plan* p, struct tapp_exec* ex, int num, const void* alpha,
+                         const void** A, const void** B, const void* beta, const void** C, void** D)
+{
+    struct tapp_batch_cache* b = p->batch;
+    if (!b || b->device != ex->device || b->num != num || b->c_null != (C == NULL)) return -1;
+    tk_problem P;
+    if (tk_plan_problem(p, ex, alpha, beta, C ? C[0] : NULL, &P) || P.empty) return -1;
+    if (P.mode != b->mode || P.beta_zero != b->beta_zero || P.round_C != b->round_C) return -1;
+    const int64_t *t[12];
+    if (tk_problem_tables(p, &P, ex->device, &t[0], &t[1], &t[2], &t[3], &t[4], &t[5], &t[6], &t[7], &t[8], &t[9],
+                          &t[10], &t[11]))
+        return -1;
+    uint64_t key;
+    if (tk_plan_tables_id(p, &key) != b->tables || key != b->key) return -1;
+    /* every changed pointer must be device memory (checked before anything
+     * is patched, so a refusal leaves the cache as it was) */
+    range_memo memo = {{0}, {0}, 0, 0};
+    bool changed = false;
+    for (int i = 0; i < num; i++) {
+        if (!A[i] || !B[i] || !D[i] || (!P.beta_zero && !C[i])) return -1;
+        if (A[i] != b->A[i]) { if (!memo_is_device(&memo, A[i])) return -1; changed = true; }
+        if (B[i] != b->B[i]) { if (!memo_is_device(&memo, B[i])) return -1; changed = true; }
+        if (D[i] != b->D[i]) { if (!memo_is_device(&memo, D[i])) return -1; changed = true; }
+        if (!P.beta_zero && C[i] != b->C[i]) { if (!memo_is_device(&memo, C[i])) return -1; changed = true; }
+    }
+    struct tapp_group* g = b->g;
+    const float al = (float)P.alpha[0], be = (float)P.beta[0];
+    if (g->n > 0 && (al != g->hprobs[0].alpha || be != g->hprobs[0].beta)) changed = true;
+    if (changed) {
+        for (int k = 0; k < g->n; k++) {
+            tk_prob* d = &g->hprobs[k];
+            const int i = g->hsrc[k];
+            const void *xa = d->x_is_b ? B[i] : A[i], *xw = d->x_is_b ? b->B[i] : b->A[i];
+            const void *ya = d->x_is_b ? A[i] : B[i], *yw = d->x_is_b ? b->A[i] : b->B[i];
+            d->X = (const float*)shift(xa, xw, d->X);
+            d->Y = (const float*)shift(ya, yw, d->Y);
+            d->D = (float*)shift(D[i], b->D[i], d->D);
+            if (!P.beta_zero) d->C = (const float*)shift(C[i], b->C[i], d->C);
+            d->alpha = al;
+            d->beta = be;
+        }
+        for (int i = 0; i < num; i++) {
+            b->A[i] = A[i]; b->B[i] = B[i]; b->D[i] = D[i];
+            b->C[i] = C ? C[i] : NULL;
+        }
+        if (hipSetDevice(ex->device) != hipSuccess ||
+            hipMemcpyAsync(g->dprobs, g->hprobs, sizeof(tk_prob) * (size_t)g->n, hipMemcpyHostToDevice,
+                           ex->stream) != hipSuccess) {
+            (void)hipGetLastError();
+            p->batch = NULL;          /* device copy unknown: build again next time */
+            tk_batch_cache_free(b);
+            return TAPP_AMD_ERR_DEVICE;
+        }
+    }
+    g->ex = ex;
+    return execute_group(g);
+}
+
+/* Keeps `g` on the plan for the next call when every operand set runs in
+ * the persistent launch from its caller's pointers; otherwise frees it. */
+static void batched_keep(struct tapp_plan* p, struct tapp_exec* ex, struct tapp_group* g, int num,
+                         const void* alpha, const void** A, const void** B, const void* beta, const void** C,
+                         void** D)
+{
+    tk_problem P;
+    uint64_t key = 0;
+    const void* tables = tk_plan_tables_id(p, &key);
+    bool ok = g->n == num && g->nrest == 0 && g->hprobs && g->hsrc && tables &&
+              !tk_plan_problem(p, ex, alpha, beta, C ? C[0] : NULL, &P);
+    for (int k = 0; ok && k < g->n; k++) {
+        /* not a resident copy: the descriptor's pointers are the caller's */
+        const tk_prob* d = &g->hprobs[k];
+        const int i = g->hsrc[k], xi = d->x_is_b ? 1 : 0;
+        const void* ops[2] = {A[i], B[i]};
+        ok = d->X == (const float*)ops[xi] + P.pin[xi] && d->Y == (const float*)ops[1 - xi] + P.pin[1 - xi] &&
+             d->D == (float*)D[i] + P.pin[3] && (P.beta_zero || d->C == (const float*)C[i] + P.pin[2]);
+    }
+    struct tapp_batch_cache* b = ok ? calloc(1, sizeof *b) : NULL;
+    if (b) {
+        b->A = malloc(sizeof(void*) * (size_t)num);
+        b->B = malloc(sizeof(void*) * (size_t)num);
+        b->C = malloc(sizeof(void*) * (size_t)num);
+        b->D = malloc(sizeof(void*) * (size_t)num);
+    }
+    if (!b || !b->A || !b->B || !b->C || !b->D) {
+        if (b) { free(b->A); free(b->B); free(b->C); free(b->D); free(b); }
+        group_free(g);
+        return;
+    }
+    b->device = ex->device;
+    b->num = num;
+    b->mode = P.mode;
+    b->beta_zero = P.beta_zero;
+    b->round_C = P.round_C;
+    b->c_null = C == NULL;
+    b->key = key;
+    b->tables = tables;
+    b->g = g;
+    for (int i = 0; i < num; i++) {
+        b->A[i] = A[i]; b->B[i] = B[i]; b->D[i] = D[i];
+        b->C[i] = C ? C[i] : NULL;
+    }
+    tk_batch_cache_free(p->batch);
+    p->batch = b;
+}
+
 TAPP_error TAPP_execute_batched_product(TAPP_tensor_product plan, TAPP_executor exec, TAPP_status* status,
                                         int num_batches, const void* alpha, const void** A, const void** B,
                                         const void* beta, const void** C, void** D)
@@ -489,17 +678,37 @@ TAPP_error TAPP_execute_batched_product(TAPP_tensor_product plan, TAPP_executor 
     (void)status;
     if (!plan || num_batches < 0 || (num_batches > 0 && (!A || !B || !D))) return TAPP_AMD_ERR_INVALID;
     if (num_batches == 0) return 0;
-    size_t n = (size_t)num_batches;
-    TAPP_tensor_product* plans = malloc(sizeof(TAPP_tensor_product) * n);
-    const void** al = malloc(sizeof(void*) * n);
-    const void** be = malloc(sizeof(void*) * n);
-    if (!plans || !al || !be) { free(plans); free(al); free(be); return TAPP_AMD_ERR_NOMEM; }
-    for (size_t i = 0; i < n; i++) { plans[i] = plan; al[i] = alpha; be[i] = beta; }
-    TAPP_AMD_product_group g = 0;
-    int rc = create_group(&g, exec, num_batches, plans, al, A, B, be, C, D, false);
-    if (!rc) rc = TAPP_AMD_execute_product_group(g);
-    if (g) TAPP_AMD_destroy_product_group(g);
-    free(plans); free(al); free(be);
+    struct tapp_plan* p = (struct tapp_plan*)plan;
+    struct tapp_exec* ex = tk_exec_lookup(exec);
+    if (!ex) ex = (exec && (*(const int*)exec == 1 || *(const int*)exec == 2 || *(const int*)exec == 12))
+                      ? tk_default_exec() : NULL;
+    if (!ex) return 33;
+    tk_exec_lock(ex);
+    pthread_mutex_lock(&p->mu);
+    int rc = batched_reuse(p, ex, num_batches, alpha, A, B, beta, C, D);
+    ex->last_reused = rc >= 0;
+    if (rc < 0) {
+        size_t n = (size_t)num_batches;
+        TAPP_tensor_product* plans = malloc(sizeof(TAPP_tensor_product) * n);
+        const void** al = malloc(sizeof(void*) * n);
+        const void** be = malloc(sizeof(void*) * n);
+        if (!plans || !al || !be) {
+            rc = TAPP_AMD_ERR_NOMEM;
+        } else {
+            for (size_t i = 0; i < n; i++) { plans[i] = plan; al[i] = alpha; be[i] = beta; }
+            TAPP_AMD_product_group g = 0;
+            rc = create_group_on(&g, ex, num_batches, plans, al, A, B, be, C, D, true, true);
+            if (!rc) rc = execute_group((struct tapp_group*)g);
+            if (g && !rc) {
+                batched_keep(p, ex, (struct tapp_group*)g, num_batches, alpha, A, B, beta, C, D);
+            } else if (g) {
+                group_free((struct tapp_group*)g);
+            }
+        }
+        free(plans); free(al); free(be);
+    }
+    pthread_mutex_unlock(&p->mu);
+    tk_exec_unlock(ex);
     return rc;
 }
 
@@ -539,6 +748,8 @@ static int execute_group(struct tapp_group* g)
         ex->last_device_us = ms * 1000.0;
     }
     ex->last_kernel = TAPP_AMD_KERNEL_GROUP;
+    snprintf(ex->last_kernel_name, sizeof ex->last_kernel_name, "k_group (%d persistent, %d one by one)", g->n,
+             g->nrest);
     return 0;
 }
 

@@ -25,6 +25,7 @@ struct tapp_info {
 };
 
 struct tapp_plan_cache;
+struct tapp_batch_cache;
 
 /* Product plan (reference: struct plan, ref_impl.h:21-37).  Layouts are held
  * by handle, so later TAPP_set_* calls on an info change the plan, as in the
@@ -38,6 +39,7 @@ struct tapp_plan {
     TAPP_prectype prec;
     struct tapp_plan_cache* cache;
     struct tapp_plan_cache* hcache;   /* host-only tables of the allocation-free path */
+    struct tapp_batch_cache* batch;   /* the last TAPP_execute_batched_product call, prepared */
     /* recursive; held while a call builds or reads the cache and enqueues
      * work that reads it, so calls on one plan from several threads are safe */
     pthread_mutex_t mu;
@@ -70,8 +72,10 @@ struct tapp_exec {
     int split_bf16;
     int force_kernel;
     int autotune;              /* 0 rules, 1 measure unknown shape classes, 2 cache only */
+    int last_reused;           /* the last batched call reused its plan's prepared batch */
     int timing;
     int last_kernel;
+    char last_kernel_name[96];  /* kernel template of the last execute */
     double last_device_us;
 };
 
@@ -167,6 +171,10 @@ int tk_kernel_for(const tk_problem* P, int force);
 int tk_execute_checked(struct tapp_plan* p, struct tapp_exec* ex, const void* alpha, const void* A,
                        const void* B, const void* beta, const void* C, void* D);
 void tk_plan_cache_free(struct tapp_plan_cache* c);
+/* Device base and layout key of the plan's current offset tables (NULL
+ * before the first build). */
+const void* tk_plan_tables_id(const struct tapp_plan* plan, uint64_t* key);
+void tk_batch_cache_free(struct tapp_batch_cache* b);
 int tk_problem_tables(struct tapp_plan* plan, const tk_problem* P, int device, const int64_t** mA,
                       const int64_t** kA, const int64_t** lA, const int64_t** nB, const int64_t** kB,
                       const int64_t** lB, const int64_t** mD, const int64_t** nD, const int64_t** lD,

@@ -61,6 +61,7 @@ TAPP_error TAPP_destroy_tensor_product(TAPP_tensor_product plan)
     for (int x = 0; x < 4; x++) free(p->idx[x]);
     tk_plan_cache_free(p->cache);
     tk_plan_cache_free(p->hcache);
+    tk_batch_cache_free(p->batch);
     pthread_mutex_destroy(&p->mu);
     free(p);
     return 0;

@@ -147,7 +147,7 @@ typedef struct tk_prob {
     int y_kfast;        /* 0: load Y along g, 1: along K (unit K step), 2: dense K x G block */
     int round;          /* 0 or TK_MODE_BF16 / TK_MODE_F16: inputs rounded (FP32 accumulate) */
     int round_C;
-    int pad;
+    int x_is_b;         /* X is B (host bookkeeping; not read by the kernel) */
     /* MFMA tiles: X + xlo (Y + ylo) is the lowest element the tile can read,
      * and every element lies within 2^29 of it, so offsets are 32-bit */
     int64_t xlo, ylo;
@@ -196,6 +196,9 @@ int tk_scratch_get(int device, void** ptr, size_t* bytes);
 
 /* Number of kernel variants compiled in, for diagnostics. */
 int tk_kernel_count(void);
+/* Name of the kernel template the calling thread last launched through
+ * tk_launch_gett. */
+const char* tk_last_gett_name(void);
 
 #ifdef __cplusplus
 }

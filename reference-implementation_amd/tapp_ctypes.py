@@ -29,6 +29,8 @@ ATTR_STREAM = 0x1005
 ATTR_TIMING = 0x1006
 ATTR_AUTOTUNE = 0x1007
 ATTR_SELF_CHECK = 0x1008
+ATTR_LAST_REUSED = 0x1009
+ATTR_LAST_KERNEL_NAME = 0x100A
 # this library's error codes (tapp_amd.h)
 AMD_ERR_SCRATCH = 108
 AMD_ERR_UNSUPPORTED, AMD_ERR_DEVICE, AMD_ERR_NOMEM, AMD_ERR_INVALID, AMD_ERR_KEY, AMD_ERR_BUDGET, AMD_ERR_BUSY = \
@@ -202,9 +204,15 @@ class TappLib:
         self.lib.TAPP_AMD_last_error_message(buf, 512)
         return buf.value.decode()
 
+    @staticmethod
+    def pointer_array(ptrs):
+        """A C array of pointers (reusable across calls)."""
+        return (ctypes.c_void_p * max(1, len(ptrs)))(*ptrs)
+
     def execute_batched(self, plan, exec_, alpha_ptr, a_ptrs, b_ptrs, beta_ptr, c_ptrs, d_ptrs):
+        """Pointer lists or arrays made by pointer_array."""
         n = len(a_ptrs)
-        arr = lambda ps: (ctypes.c_void_p * max(1, n))(*ps)  # noqa: E731
+        arr = lambda ps: ps if isinstance(ps, ctypes.Array) else (ctypes.c_void_p * max(1, n))(*ps)  # noqa: E731
         c_arr = arr(c_ptrs) if c_ptrs is not None else None
         return self.lib.TAPP_execute_batched_product(plan, exec_, None, n, alpha_ptr, arr(a_ptrs), arr(b_ptrs),
                                                      beta_ptr, c_arr, arr(d_ptrs))
@@ -234,6 +242,11 @@ class TappLib:
         v = ctypes.c_double(0)
         rc = self.lib.TAPP_attr_get(exec_, key, ctypes.byref(v))
         return rc, v.value
+
+    def last_kernel_name(self, exec_):
+        v = ctypes.c_char_p()
+        rc = self.lib.TAPP_attr_get(exec_, ATTR_LAST_KERNEL_NAME, ctypes.byref(v))
+        return rc, (v.value or b"").decode()
 
     def create_group(self, exec_, plans, alphas, a_ptrs, b_ptrs, betas, c_ptrs, d_ptrs):
         n = len(plans)

@@ -188,3 +188,143 @@ def test_group_with_split_forms(lib, oracle, split):
     for d in dcs:
         d.close()
     lib.destroy_executor(ex)
+
+
+class BatchSets:
+    """`n` operand sets of one layout on the device, executed through one
+    plan with TAPP_execute_batched_product."""
+
+    def __init__(self, lib, base, n, seed):
+        self.lib = lib
+        self.sets = []
+        for s in range(n):
+            c = base.copy()
+            for o in c.operands():
+                o.buf = T.fill(o.type, len(o.buf), seed + 13 * s + o.type)
+            self.sets.append(DevCase(lib, c))
+        self.plan = self.sets[0].plan
+
+    def call(self, ex, alpha, beta, pick=None, c_null=False, n=None, a_ptr=None):
+        """D of set i from A, B of set pick[i] and C of set i; returns
+        (rc, reused, expected cases)."""
+        n = len(self.sets) if n is None else n
+        pick = list(range(n)) if pick is None else pick
+        d_type = self.sets[0].case.D.type
+        al, be = T.scalar_bytes(alpha, d_type), T.scalar_bytes(beta, d_type)
+        for d in self.sets[:n]:
+            d.reset_d()
+        a = [self.sets[pick[i]].ptr["A"] for i in range(n)]
+        if a_ptr:
+            for i, p in a_ptr.items():
+                a[i] = p
+        rc = self.lib.execute_batched(self.plan, ex, al.ctypes.data, a, [self.sets[pick[i]].ptr["B"] for i in range(n)],
+                                      be.ctypes.data, None if c_null else [d.ptr["C"] for d in self.sets[:n]],
+                                      [d.ptr["D"] for d in self.sets[:n]])
+        want = []
+        for i in range(n):
+            c = self.sets[i].case.copy()
+            c.A, c.B = self.sets[pick[i]].case.A.copy(), self.sets[pick[i]].case.B.copy()
+            c.alpha, c.beta, c.c_null = alpha, beta, c_null
+            want.append(c)
+        return rc, self.lib.attr_get_int(ex, tc.ATTR_LAST_REUSED)[1], want
+
+    def check(self, oracle, want):
+        for d, c in zip(self.sets, want):
+            _, w = oracle.run(c, gpu=True)
+            assert_parity(d.d(), w, c)
+
+    def close(self):
+        for d in self.sets:
+            d.close()
+
+
+@pytest.mark.parametrize("shape", ["c0form", "tucker"])
+def test_batched_product_reuses_prepared_batch(lib, oracle, shape):
+    """Repeated TAPP_execute_batched_product calls on one plan: the second
+    call with the same batch count replays the prepared batch (descriptors
+    patched for changed pointers, scalars refreshed) and every D still
+    equals the oracle; beta switching to 0, a host operand and another batch
+    count build the batch again."""
+    import torch
+    _, ex = lib.create_executor()
+    base = T.four_index_case(5, (6, 5, 7, 4, 3, 5)) if shape == "c0form" else T.tucker_case(6, 24, 9, 7, 5)
+    bs = BatchSets(lib, base, 6, 300)
+    rc, reused, want = bs.call(ex, 1.5, 0.5)
+    assert (rc, reused) == (0, 0)
+    assert lib.attr_get_int(ex, tc.ATTR_LAST_KERNEL)[1] == tc.KERNEL_GROUP
+    bs.check(oracle, want)
+    # same pointers, new data in A of set 2: read, not cached
+    s2 = bs.sets[2]
+    s2.case.A.buf[:] = T.fill(s2.case.A.type, len(s2.case.A.buf), 999)
+    s2.t["A"].copy_(torch.from_numpy(s2.case.A.buf.view(np.uint8).copy()).cuda())
+    torch.cuda.synchronize()
+    rc, reused, want = bs.call(ex, 1.5, 0.5)
+    assert (rc, reused) == (0, 1)
+    bs.check(oracle, want)
+    # operand sets permuted, new scalars
+    rc, reused, want = bs.call(ex, -0.75, 2.0, pick=[5, 3, 4, 0, 1, 2])
+    assert (rc, reused) == (0, 1)
+    bs.check(oracle, want)
+    # beta = 0 without C: built again, then reused
+    for expect in (0, 1):
+        rc, reused, want = bs.call(ex, 2.0, 0.0, c_null=True)
+        assert (rc, reused) == (0, expect)
+        bs.check(oracle, want)
+    # a host operand: staged through the general path, not reused
+    host_a = bs.sets[1].case.A.ptr()
+    rc, reused, want = bs.call(ex, 2.0, 0.0, c_null=True, a_ptr={1: host_a})
+    assert (rc, reused) == (0, 0)
+    bs.check(oracle, want)
+    # another batch count
+    rc, reused, want = bs.call(ex, 1.0, 1.0, n=4)
+    assert (rc, reused) == (0, 0)
+    bs.check(oracle, want[:4])
+    rc, reused, want = bs.call(ex, 1.0, 1.0, n=4, pick=[1, 0, 3, 2])
+    assert (rc, reused) == (0, 1)
+    bs.check(oracle, want[:4])
+    # a NULL operand is reported as on the first call
+    d_type = bs.sets[0].case.D.type
+    al = T.scalar_bytes(1.0, d_type)
+    ptrs = [d.ptr["A"] for d in bs.sets[:4]]
+    ptrs[2] = None
+    rc = lib.execute_batched(bs.plan, ex, al.ctypes.data, ptrs, [d.ptr["B"] for d in bs.sets[:4]], al.ctypes.data,
+                             [d.ptr["C"] for d in bs.sets[:4]], [d.ptr["D"] for d in bs.sets[:4]])
+    rc_single = lib.execute(bs.plan, ex, al.ctypes.data, None, bs.sets[0].ptr["B"], al.ctypes.data,
+                            bs.sets[0].ptr["C"], bs.sets[0].ptr["D"])
+    assert rc == rc_single != 0
+    bs.close()   # frees the prepared batch with the plan
+    lib.destroy_executor(ex)
+
+
+def test_batched_product_reuse_follows_layout_change(lib, oracle):
+    """A layout changed through TAPP_set_strides after a batched call: the
+    next call rebuilds (the plan's offset tables changed) and reads the new
+    layout."""
+    import ctypes
+    from test_gpu_parity import _gemm_case
+    _, ex = lib.create_executor()
+    base = _gemm_case(tc.F32, 40, 30, 20, 9)
+    bs = BatchSets(lib, base, 3, 500)
+    rc, reused, want = bs.call(ex, 1.0, 0.5)
+    assert (rc, reused) == (0, 0)
+    bs.check(oracle, want)
+    rc, reused, _ = bs.call(ex, 1.0, 0.5)
+    assert (rc, reused) == (0, 1)
+    # A stored transposed in every set, announced through the plan's info
+    import torch
+    M, K = base.A.extents
+    for d in bs.sets:
+        a2 = d.case.A.buf[:M * K].reshape(K, M).T.copy().reshape(-1).astype(np.float32)
+        d.case.A.buf = a2
+        d.case.A.strides = [K, 1]
+        d.case.A.offset = 0
+        d.t["A"] = torch.from_numpy(a2.view(np.uint8).copy()).cuda()
+        d.ptr["A"] = d.t["A"].data_ptr()
+    torch.cuda.synchronize()
+    s = (ctypes.c_int64 * 2)(K, 1)
+    assert lib.lib.TAPP_set_strides(bs.sets[0].infos[0], s) == 0
+    rc, reused, want = bs.call(ex, 1.0, 0.5)
+    assert (rc, reused) == (0, 0)
+    bs.check(oracle, want)
+    bs.close()
+    lib.destroy_executor(ex)
