@@ -495,15 +495,19 @@ def batched_product_bench(lib, torch, steps, count=8192, shape=(32, 32, 32, 8)):
     return out
 
 
-def serving_bench(lib, torch, rates, nreq, split=False, seed=3):
+def serving_bench(lib, torch, rates, nreq, split=False, seed=3, budget=64 << 30, sequential=False):
     """open-loop Poisson arrivals into the serving loop.
     Requests: 5 % large ladder-form products (virtual extent 64:
     D[a,b,i,j] = sum A[a,b,c,d] B[c,i,d,j], a..d = 64, i,j = 32), 20 %
     configs[0] products (extents 16), 75 % drawn from 256 configs[3]-mix shapes.
     70 % of requests pass a host A that is resident on the plans' handle
     (TAPP_AMD_make_resident: read from its HBM copy, no transfer); 30 % pass
-    another pinned host A that the server stages (counted against the 64 GB
-    budget).  B, C and D are device tensors (D from a ring per shape)."""
+    another pinned host A that the server stages.  B, C and D are device
+    tensors (D from a ring per shape).  The server's device memory --
+    workspaces (staging, pre-pass buffers, split-K partials), descriptor
+    buffers, tables being built -- is admitted against `budget`.  With
+    `sequential`, the last rate's trace is also run as one
+    TAPP_execute_product call per request, back to back on one executor."""
     rng = np.random.default_rng(seed)
     L = lambda t: [ord(c) for c in t]  # noqa: E731
 
@@ -549,7 +553,7 @@ def serving_bench(lib, torch, rates, nreq, split=False, seed=3):
     rc, ex = lib.create_executor()
     if split:
         lib.attr_set_int(ex, tc.ATTR_SPLIT_BF16, 1)
-    rc, srv = lib.create_server(ex, 64 << 30, 4, 64)
+    rc, srv = lib.create_server(ex, budget, 4, 64)
     assert rc == 0
     # warm-up: one request of every shape (resident and staged A), so that
     # no kernel is launched for the first time inside a measured rate
@@ -601,12 +605,24 @@ def serving_bench(lib, torch, rates, nreq, split=False, seed=3):
                 log(f"bench: serving {rate} req/s: {s1} of {nreq} requests done")
         _, st = lib.server_stats(srv)
         assert failed == 0 and all(r == 0 for r in rcs), (failed, [r for r in rcs if r][:5])
-        results.append({"rate_req_s": rate, "requests": nreq, "p50_us": st["p50_us"], "p99_us": st["p99_us"],
-                        "mean_us": st["mean_us"], "sustained_gflops": st["gflops"],
-                        "offered_gflops": flop / (arrivals[-1] * 1e-6) / 1e9, "admission_stalls": st["stalls"],
-                        "max_in_flight": st["max_inflight"], "batches": st["batches"],
-                        "persistent_products": st["persistent_products"], "wall_s": wall,
-                        "mean_submit_late_us": late / nreq})
+        row = {"rate_req_s": rate, "requests": nreq, "p50_us": st["p50_us"], "p99_us": st["p99_us"],
+               "mean_us": st["mean_us"], "sustained_gflops": st["gflops"],
+               "sustained_req_s": nreq / wall, "offered_gflops": flop / (arrivals[-1] * 1e-6) / 1e9,
+               "admission_stalls": st["stalls"], "workspace_reclaims": st["reclaims"],
+               "max_device_bytes_held": st["max_held_bytes"], "budget_bytes": budget,
+               "max_in_flight": st["max_inflight"], "batches": st["batches"],
+               "persistent_products": st["persistent_products"], "wall_s": wall,
+               "mean_submit_late_us": late / nreq}
+        if sequential and rate == rates[-1]:
+            # the same requests, one TAPP_execute_product call at a time
+            seq_ex = lib.create_executor()[1]
+            f, r, el = tc.trace_sequential(lib, seq_ex, plans, als, As, Bs, bes, Cs, Ds)
+            assert f == 0, [x for x in r if x][:5]
+            lib.destroy_executor(seq_ex)
+            row["one_call_at_a_time"] = {"wall_s": el, "req_s": nreq / el, "gflops": flop / el / 1e9}
+            row["served_vs_one_call_at_a_time_req_s"] = (nreq / wall) / (nreq / el)
+            row["served_vs_one_call_at_a_time_gflops"] = st["gflops"] / (flop / el / 1e9)
+        results.append(row)
         log(f"bench: serving {'split-BF16' if split else 'FP32'} {rate} req/s: p50 {st['p50_us']:.0f} us "
             f"p99 {st['p99_us']:.0f} us, {st['gflops']:.0f} GFLOP/s, stalls {st['stalls']:.0f}")
     lib.destroy_server(srv)
@@ -870,8 +886,13 @@ def main():
         torch.cuda.empty_cache()
         log("bench: serving loop, open-loop Poisson arrivals")
         rates = [1000, 5000, 20000, 50000]
-        serving["fp32"] = serving_bench(lib, torch, rates, args.serve_requests)
+        serving["requests_per_rate"] = args.serve_requests
+        serving["requests_note"] = (f"{args.serve_requests} requests per arrival rate (this run's --serve-requests; "
+                                    "the workload's full trace is 200000 requests)")
+        serving["fp32"] = serving_bench(lib, torch, rates, args.serve_requests, sequential=True)
         serving["split_bf16"] = serving_bench(lib, torch, [5000], args.serve_requests, split=True)
+        log("bench: serving loop with a 128 MiB device-memory budget")
+        serving["fp32_budget_128MiB"] = serving_bench(lib, torch, [20000], args.serve_requests, budget=128 << 20)
 
     traffic, traffic_src = measured_traffic()
     out = {

@@ -149,11 +149,20 @@ TAPP_EXPORT TAPP_error TAPP_AMD_group_info(TAPP_AMD_product_group group, int64_t
 /* --- serving -----------------------------------------------------------------
  * A request loop with continuous batching on one GPU.  A dispatcher thread
  * owns `slots` HIP streams; whenever one is idle it admits queued requests
- * (FIFO) while their staging footprint -- host operands plus pre-pass
- * buffers; device operands cost nothing -- fits `hbm_budget_bytes`, launches
- * them as one batch (small FP32 products in one persistent launch, the rest
- * on the general path) and keeps admitting while batches run on other
- * streams.  A heavy request (above 2^31 flop, or a footprint above 4 MiB)
+ * (FIFO) while the device memory the server holds fits `hbm_budget_bytes`,
+ * launches them as one batch (small FP32 products in one persistent launch,
+ * the rest on the general path) and keeps admitting while batches run on
+ * other streams.  Held memory: each stream's workspace (sized to the largest
+ * general-path request of its batch -- staged host operands, pre-pass and
+ * conversion buffers, split-K partial sums -- and kept between batches),
+ * and the offset tables that batches in flight build for their plans
+ * (afterwards the plan keeps them, outside the budget); the persistent
+ * launch's descriptor buffers (232 bytes per request of a batch) are not
+ * counted.
+ * When a request does not fit, idle streams' workspaces are freed first;
+ * if it still does not fit it waits (an admission stall).  Device operands
+ * and resident tensors cost nothing.  A heavy request (above 2^31 flop, or
+ * a workspace above 4 MiB)
  * is passed over while every other stream holds one, so that one stream
  * stays free for the small requests queued behind it.  The process needs a
  * hardware queue per stream it uses (HIP's GPU_MAX_HW_QUEUES, default 4,
@@ -172,8 +181,8 @@ enum { TAPP_AMD_SERVER_PAUSED = 1 };   /* create paused: queue and admission onl
 
 TAPP_EXPORT TAPP_error TAPP_AMD_create_server(TAPP_AMD_server* server, TAPP_executor exec, size_t hbm_budget_bytes,
                                               int slots, int max_batch, int flags);
-/* Validates like TAPP_execute_product; TAPP_AMD_ERR_BUDGET when the request
- * alone exceeds the budget.  alpha/beta are copied. */
+/* Validates like TAPP_execute_product; TAPP_AMD_ERR_BUDGET when the request's
+ * own workspace and tables exceed the budget.  alpha/beta are copied. */
 TAPP_EXPORT TAPP_error TAPP_AMD_server_submit(TAPP_AMD_server server, TAPP_tensor_product plan, const void* alpha,
                                               const void* A, const void* B, const void* beta, const void* C,
                                               void* D, TAPP_status* status);
@@ -183,10 +192,11 @@ TAPP_EXPORT TAPP_error TAPP_AMD_server_cancel(TAPP_AMD_server server, TAPP_statu
 TAPP_EXPORT TAPP_error TAPP_AMD_server_set_paused(TAPP_AMD_server server, int paused);
 /* Waits until nothing is queued (or the server is paused) and no batch runs. */
 TAPP_EXPORT TAPP_error TAPP_AMD_server_drain(TAPP_AMD_server server);
-/* stats: completed, p50 us, p99 us, mean us, admission stalls, max in flight,
- * batches, products run in persistent launches, GFLOP/s, queued, budget bytes
- * in use.  Latency = submit to observed completion. */
-TAPP_EXPORT TAPP_error TAPP_AMD_server_stats(TAPP_AMD_server server, double stats[11], int reset);
+/* stats: completed, p50 us, p99 us, mean us, requests that waited for memory, max in flight,
+ * batches, products run in persistent launches, GFLOP/s, queued, device
+ * bytes held now, most device bytes held, workspace reclaims.  Latency =
+ * submit to observed completion. */
+TAPP_EXPORT TAPP_error TAPP_AMD_server_stats(TAPP_AMD_server server, double stats[13], int reset);
 TAPP_EXPORT TAPP_error TAPP_AMD_destroy_server(TAPP_AMD_server server);
 
 /* --- resident tensors ----------------------------------------------------

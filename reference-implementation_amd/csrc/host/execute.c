@@ -160,11 +160,10 @@ static void prep_order(const tk_group* rows, const tk_group* K, const tk_group* 
     for (int i = 0; i < 3; i++) { dense_step[ord[i]] = step; step *= size[ord[i]]; }
 }
 
-/* The host side of a plan's tables: offsets, sizes and contents. */
-static struct tapp_plan_cache* host_tables(const tk_problem* P, uint64_t key)
+/* Element offset of each table in a plan's arena (off may be NULL); returns
+ * the arena's length in elements. */
+static size_t table_layout(const tk_problem* P, size_t* off)
 {
-    struct tapp_plan_cache* c = calloc(1, sizeof *c);
-    if (!c) return NULL;
     const int64_t M = P->nM, N = P->nN, K = P->nK, L = P->nL;
     const int64_t len[T_COUNT] = {
         [T_mA] = M, [T_mC] = M, [T_mD] = M, [T_nB] = N, [T_nC] = N, [T_nD] = N,
@@ -174,10 +173,20 @@ static struct tapp_plan_cache* host_tables(const tk_problem* P, uint64_t key)
         [T_dmD] = M, [T_dnD] = N, [T_dlD] = L, [T_zero] = 1};
     size_t total = 0;
     for (int t = 0; t < T_COUNT; t++) {
-        c->off[t] = total;
+        if (off) off[t] = total;
         total += (size_t)(len[t] > 0 ? len[t] : 1);
         total = (total + 31) & ~(size_t)31;   /* 256-byte aligned sub-tables */
     }
+    return total;
+}
+
+/* The host side of a plan's tables: offsets, sizes and contents. */
+static struct tapp_plan_cache* host_tables(const tk_problem* P, uint64_t key)
+{
+    struct tapp_plan_cache* c = calloc(1, sizeof *c);
+    if (!c) return NULL;
+    const int64_t M = P->nM, N = P->nN, K = P->nK, L = P->nL;
+    const size_t total = table_layout(P, c->off);
     c->total = total;
     c->key = key;
     c->host = malloc(total * sizeof(int64_t));
@@ -545,6 +554,172 @@ int tk_execute_noalloc(struct tapp_plan* p, struct tapp_exec* ex, const void* al
     return rc;
 }
 
+/* Where a call's operands live and what device memory it needs beyond
+ * them: host operands staged over their spans, pre-pass / conversion
+ * buffers, split-K partial sums and the split-F16 range word, laid out in
+ * one workspace.  Shared by execute_on and the serving loop's admission. */
+typedef struct {
+    const void* src[4];          /* device address of each operand (resident copies resolved) */
+    bool on_dev[4];
+    size_t span[4];              /* bytes staged from host memory */
+    bool need[4];                /* pre-pass (A, B, C) or conversion (D) buffer */
+    size_t tmp[4];
+    int kern, mode;
+    bool may_tune;
+    int64_t ks, kchunk;
+    size_t part;                 /* split-K partials */
+    size_t off_stage[4], off_tmp[4], off_part, off_amax, total;
+} work_layout;
+
+static void work_offsets(work_layout* w)
+{
+    size_t total = 0;
+    for (int x = 0; x < 4; x++) { w->off_stage[x] = total; total = ALIGN256(total + w->span[x]); }
+    for (int x = 0; x < 4; x++) { w->off_tmp[x] = total; total = ALIGN256(total + w->tmp[x]); }
+    w->off_part = total;
+    total = ALIGN256(total + w->part);
+    w->off_amax = total;
+    if (w->mode == TK_MODE_SPLIT_F16) total = ALIGN256(total + 2 * sizeof(uint32_t));
+    w->total = total;
+}
+
+static int plan_work(const struct tapp_plan* p, const struct tapp_exec* ex, const tk_problem* Pp, const void* A,
+                     const void* B, const void* C, void* D, bool sync, bool scratch, work_layout* w)
+{
+    const tk_problem P = *Pp;
+    const int64_t M = P.nM, N = P.nN, K = P.nK, L = P.nL;
+    const int ct = P.ctype;
+    const size_t cs = ctype_size(ct);
+    memset(w, 0, sizeof *w);
+    /* device operands are used in place, host ranges resident on the plan's
+     * handle are read from their copies (not D: it is written through the
+     * caller's pointer), other host operands are staged over their span */
+    const void* ops[4] = {A, B, P.beta_zero ? NULL : C, D};
+    for (int x = 0; x < 4; x++) {
+        w->src[x] = ops[x];
+        w->on_dev[x] = true;
+        if (!ops[x]) continue;
+        const int64_t es = (int64_t)tk_type_size(P.type[x]);
+        const void* r = x < 3 ? tk_resolve_operand(p->handle, ops[x], P.lo[x] * es, (P.hi[x] + 1) * es)
+                              : (tk_is_device_ptr(ops[x]) ? ops[x] : NULL);
+        if (r) { w->src[x] = r; continue; }
+        w->on_dev[x] = false;
+        if (P.hi[x] >= P.lo[x]) w->span[x] = (size_t)(P.hi[x] - P.lo[x] + 1) * (size_t)es;
+        if (scratch) {
+            snprintf(last_error, sizeof last_error,
+                     "allocation-free execute: operand %c is host memory (device or resident operands only)", "ABCD"[x]);
+            return TAPP_AMD_ERR_INVALID;
+        }
+    }
+    w->need[0] = P.nUA > 1 || !compatible(P.type[0], ct) || (P.op_A && P.nUA > 1);
+    w->need[1] = P.nUB > 1 || !compatible(P.type[1], ct);
+    w->need[2] = !P.beta_zero && !compatible(P.type[2], ct);
+    w->need[3] = !compatible(P.type[3], ct);
+    w->tmp[0] = w->need[0] ? (size_t)(M * K * L) * cs : 0;
+    w->tmp[1] = w->need[1] ? (size_t)(N * K * L) * cs : 0;
+    w->tmp[2] = w->need[2] ? (size_t)(M * N * L) * cs : 0;
+    w->tmp[3] = w->need[3] ? (size_t)(M * N * L) * cs : 0;
+
+    /* kernel choice (symmetric in M and N, so it holds after the orientation
+     * swap in execute_on) */
+    int kern = tk_kernel_for(&P, ex->force_kernel);
+    w->may_tune = ex->autotune && ex->force_kernel == TK_KERNEL_AUTO && K > 0 && P.mode != TK_MODE_SPLIT_F16;
+
+    /* split-F16 runs on the real FP32 MFMA GETT only: the other kernels,
+     * and complex products, compute in FP32, which is as accurate.  Its
+     * range check may ask for a second launch, which needs a synchronous
+     * call whose C is still intact after the first (tune_safe); otherwise
+     * the three-piece split-BF16 gives the same accuracy in one launch. */
+    int mode = P.mode;
+    if (mode == TK_MODE_SPLIT_F16) {
+        if (kern != TK_KERNEL_MFMA || ct != TK_F32) mode = TK_MODE_PLAIN;
+        else if (!sync || !tune_safe(&P, w->src, w->need[2])) mode = TK_MODE_SPLIT6_BF16;
+    }
+    /* few outputs and a moderate K (configs[0]: 4 tiles of 128 x 128): 16 x
+     * 16 tiles with K split over each workgroup's waves, in one launch,
+     * instead of split-K's partial sums and second kernel */
+    if (kern == TK_KERNEL_MFMA && ct == TK_F32 && mode == TK_MODE_PLAIN && ex->force_kernel == TK_KERNEL_AUTO) {
+        const int64_t t128 = ((M + 127) / 128) * ((N + 127) / 128) * L;
+        const int64_t t16 = ((M + 15) / 16) * ((N + 15) / 16) * L;
+        if (t128 < 2 * (int64_t)ex->cus && K <= 1024 && t16 <= 4 * (int64_t)ex->cus) kern = TK_KERNEL_SMALL;
+    }
+    if (kern == TK_KERNEL_SMALL && (ct != TK_F32 || mode != TK_MODE_PLAIN)) kern = TK_KERNEL_MFMA;
+    /* split-K for the FP32-data MFMA kernel when its 128 x 128 tiles cannot
+     * fill the GPU: workgroups take K slices (multiples of its 32-deep K
+     * tile) and store FP32 partial sums of whole tiles, which a second
+     * kernel adds in a fixed order */
+    int64_t ks = 1, kchunk = 0;
+    size_t part = 0;
+    if ((kern == TK_KERNEL_MFMA || w->may_tune) && ct == TK_F32 && K >= 64) {
+        const int64_t tiles = ((M + 127) / 128) * ((N + 127) / 128) * L;
+        const int64_t target = 2 * (int64_t)ex->cus;
+        if (tiles < target) {
+            ks = (target + tiles - 1) / tiles;
+            if (ks > (K + 31) / 32) ks = (K + 31) / 32;
+            if (ks > 64) ks = 64;
+            kchunk = ((K + ks - 1) / ks + 31) / 32 * 32;
+            ks = (K + kchunk - 1) / kchunk;
+            part = sizeof(float) * (size_t)(ks * tiles * 128 * 128);   /* whole tiles */
+            if (ks < 2 || part > ((size_t)64 << 20)) { ks = 1; kchunk = 0; part = 0; }
+        }
+    }
+    w->kern = kern;
+    w->mode = mode;
+    w->ks = ks;
+    w->kchunk = kchunk;
+    w->part = part;
+    work_offsets(w);
+    return 0;
+}
+
+int tk_call_bytes(struct tapp_plan* p, const struct tapp_exec* ex, const tk_problem* P, const void* A,
+                  const void* B, const void* C, void* D, size_t* work, size_t* tables, bool* devops)
+{
+    work_layout w;
+    int rc = plan_work(p, ex, P, A, B, C, D, false, false, &w);
+    if (rc) return rc;
+    *work = w.total;
+    *devops = w.on_dev[0] && w.on_dev[1] && w.on_dev[2] && w.on_dev[3];
+    *tables = tk_plan_needs_tables(p, ex) ? table_layout(P, NULL) * sizeof(int64_t) : 0;
+    return 0;
+}
+
+bool tk_plan_needs_tables(struct tapp_plan* p, const struct tapp_exec* ex)
+{
+    tk_problem P;
+    if (tk_plan_problem(p, ex, NULL, NULL, NULL, &P)) return false;
+    pthread_mutex_lock(&p->mu);
+    const struct tapp_plan_cache* c = p->cache;
+    const bool have = c && c->key == problem_key(&P) && c->device == ex->device && c->dev;
+    pthread_mutex_unlock(&p->mu);
+    return !have;
+}
+
+int tk_exec_reserve_ws(struct tapp_exec* ex, size_t bytes)
+{
+    if (bytes <= ex->ws_size) return 0;
+    if (ex->ws) { hipStreamSynchronize(ex->stream); hipFree(ex->ws); ex->ws = NULL; ex->ws_size = 0; }
+    hipError_t e = hipMalloc(&ex->ws, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        ex->ws = NULL;
+        device_error("hipMalloc(workspace)", (int)e, __LINE__);
+        return TAPP_AMD_ERR_NOMEM;
+    }
+    ex->ws_size = bytes;
+    return 0;
+}
+
+void tk_exec_release_ws(struct tapp_exec* ex)
+{
+    if (!ex->ws) return;
+    hipStreamSynchronize(ex->stream);
+    hipFree(ex->ws);
+    (void)hipGetLastError();
+    ex->ws = NULL;
+    ex->ws_size = 0;
+}
+
 static int execute_on(struct tapp_plan* p, struct tapp_exec* ex, const void* alpha, const void* A, const void* B,
                       const void* beta, const void* C, void* D, bool sync, const region* scr)
 {
@@ -575,102 +750,34 @@ static int execute_on(struct tapp_plan* p, struct tapp_exec* ex, const void* alp
     const int ct = P.ctype;
     const size_t cs = ctype_size(ct);
 
-    /* what has to be staged or converted */
-    /* device operands are used in place, host ranges resident on the plan's
-     * handle are read from their copies (not D: it is written through the
-     * caller's pointer), other host operands are staged over their span */
-    const void* src[4] = {A, B, P.beta_zero ? NULL : C, D};
-    bool on_dev[4];
-    size_t span_bytes[4] = {0, 0, 0, 0};
-    for (int x = 0; x < 4; x++) {
-        on_dev[x] = true;
-        if (!src[x]) continue;
-        const int64_t es = (int64_t)tk_type_size(P.type[x]);
-        const void* r = x < 3 ? tk_resolve_operand(p->handle, src[x], P.lo[x] * es, (P.hi[x] + 1) * es)
-                              : (tk_is_device_ptr(src[x]) ? src[x] : NULL);
-        if (r) { src[x] = r; continue; }
-        on_dev[x] = false;
-        if (P.hi[x] >= P.lo[x]) span_bytes[x] = (size_t)(P.hi[x] - P.lo[x] + 1) * (size_t)es;
-        if (scr) {
-            snprintf(last_error, sizeof last_error,
-                     "allocation-free execute: operand %c is host memory (device or resident operands only)", "ABCD"[x]);
-            return TAPP_AMD_ERR_INVALID;
-        }
-    }
-    const bool needA = P.nUA > 1 || !compatible(P.type[0], ct) || (P.op_A && P.nUA > 1);
-    const bool needB = P.nUB > 1 || !compatible(P.type[1], ct);
-    const bool needC = !P.beta_zero && !compatible(P.type[2], ct);
-    const bool needD = !compatible(P.type[3], ct);
-    const size_t szA = needA ? (size_t)(M * K * L) * cs : 0;
-    const size_t szB = needB ? (size_t)(N * K * L) * cs : 0;
-    const size_t szC = needC ? (size_t)(M * N * L) * cs : 0;
-    const size_t szD = needD ? (size_t)(M * N * L) * cs : 0;
-
-    /* kernel choice (symmetric in M and N, so it holds after the orientation
-     * swap below) */
+    work_layout w;
+    if ((rc = plan_work(p, ex, &P, A, B, C, D, sync, scr != NULL, &w))) return rc;
+    const void* const* src = w.src;
+    const bool* on_dev = w.on_dev;
+    const size_t* span_bytes = w.span;
+    const bool needA = w.need[0], needB = w.need[1], needC = w.need[2], needD = w.need[3];
     const int64_t outs = M * N * L;
-    int kern = tk_kernel_for(&P, ex->force_kernel);
-
-    /* split-K for the FP32-data MFMA kernel when its 128 x 128 tiles cannot
-     * fill the GPU: workgroups take K slices (multiples of its 32-deep K
-     * tile) and store FP32 partial sums of whole tiles, which a second
-     * kernel adds in a fixed order */
-    const bool may_tune = ex->autotune && ex->force_kernel == TK_KERNEL_AUTO && K > 0 && P.mode != TK_MODE_SPLIT_F16;
-
-    /* split-F16 runs on the real FP32 MFMA GETT only: the other kernels,
-     * and complex products, compute in FP32, which is as accurate.  Its
-     * range check may ask for a second launch, which needs a synchronous
-     * call whose C is still intact after the first (tune_safe); otherwise
-     * the three-piece split-BF16 gives the same accuracy in one launch. */
-    int mode = P.mode;
-    if (mode == TK_MODE_SPLIT_F16) {
-        if (kern != TK_KERNEL_MFMA || ct != TK_F32) mode = TK_MODE_PLAIN;
-        else if (!sync || !tune_safe(&P, src, needC)) mode = TK_MODE_SPLIT6_BF16;
+    int kern = w.kern;
+    const int mode = w.mode;
+    const bool may_tune = w.may_tune;
+    int64_t ks = w.ks, kchunk = w.kchunk;
+    /* scratch: split-K gives way first (its partials are the only optional
+     * buffer), then the call is refused */
+    if (scr && tab_bytes + w.total > scr->bytes && w.part) {
+        w.part = 0;
+        ks = 1;
+        kchunk = 0;
+        work_offsets(&w);
     }
-    /* few outputs and a moderate K (configs[0]: 4 tiles of 128 x 128): 16 x
-     * 16 tiles with K split over each workgroup's waves, in one launch,
-     * instead of split-K's partial sums and second kernel */
-    if (kern == TK_KERNEL_MFMA && ct == TK_F32 && mode == TK_MODE_PLAIN && ex->force_kernel == TK_KERNEL_AUTO) {
-        const int64_t t128 = ((M + 127) / 128) * ((N + 127) / 128) * L;
-        const int64_t t16 = ((M + 15) / 16) * ((N + 15) / 16) * L;
-        if (t128 < 2 * (int64_t)ex->cus && K <= 1024 && t16 <= 4 * (int64_t)ex->cus) kern = TK_KERNEL_SMALL;
-    }
-    if (kern == TK_KERNEL_SMALL && (ct != TK_F32 || mode != TK_MODE_PLAIN)) kern = TK_KERNEL_MFMA;
-    int64_t ks = 1, kchunk = 0;
-    size_t part_bytes = 0;
-    if ((kern == TK_KERNEL_MFMA || may_tune) && ct == TK_F32 && K >= 64) {
-        const int64_t tiles = ((M + 127) / 128) * ((N + 127) / 128) * L;
-        const int64_t target = 2 * (int64_t)ex->cus;
-        if (tiles < target) {
-            ks = (target + tiles - 1) / tiles;
-            if (ks > (K + 31) / 32) ks = (K + 31) / 32;
-            if (ks > 64) ks = 64;
-            kchunk = ((K + ks - 1) / ks + 31) / 32 * 32;
-            ks = (K + kchunk - 1) / kchunk;
-            part_bytes = sizeof(float) * (size_t)(ks * tiles * 128 * 128);   /* whole tiles */
-            if (ks < 2 || part_bytes > ((size_t)64 << 20)) { ks = 1; part_bytes = 0; }
-        }
-    }
-
-    size_t off_stage[4], off_tmp[4], off_part, off_amax, total;
-    const size_t tmp_sz[4] = {szA, szB, szC, szD};
-    for (int pass = 0;; pass++) {
-        total = 0;
-        for (int x = 0; x < 4; x++) { off_stage[x] = total; total = ALIGN256(total + span_bytes[x]); }
-        for (int x = 0; x < 4; x++) { off_tmp[x] = total; total = ALIGN256(total + tmp_sz[x]); }
-        off_part = total;
-        total = ALIGN256(total + part_bytes);
-        off_amax = total;
-        if (mode == TK_MODE_SPLIT_F16) total = ALIGN256(total + 2 * sizeof(uint32_t));
-        /* scratch: split-K gives way first (its partials are the only
-         * optional buffer), then the call is refused */
-        if (!scr || tab_bytes + total <= scr->bytes) break;
-        if (pass == 0 && part_bytes) { ks = 1; kchunk = 0; part_bytes = 0; continue; }
+    if (scr && tab_bytes + w.total > scr->bytes) {
         snprintf(last_error, sizeof last_error,
                  "allocation-free execute: needs %zu bytes of scratch, the device scratch holds %zu",
-                 tab_bytes + total, scr->bytes);
+                 tab_bytes + w.total, scr->bytes);
         return TAPP_AMD_ERR_SCRATCH;
     }
+    const size_t* off_stage = w.off_stage;
+    const size_t* off_tmp = w.off_tmp;
+    const size_t off_part = w.off_part, off_amax = w.off_amax, total = w.total;
     hipStream_t st = ex->stream;
     char* ws;
     if (scr) {

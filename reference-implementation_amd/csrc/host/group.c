@@ -74,18 +74,24 @@ static int by_cost(const void* x, const void* y)
     return a->cost > b->cost ? -1 : a->cost < b->cost;
 }
 
-/* Descriptor of one eligible product; returns false when the product needs
- * the general path. */
-bool tk_group_describe(struct tapp_plan* plan, struct tapp_exec* ex, const tk_problem* P, const void* A,
-                       const void* B, const void* C, void* D, int force_cls, tk_prob* out, int* rc)
+bool tk_group_may_run(const tk_problem* P)
 {
-    *rc = 0;
     if (P->ctype != TK_F32 || P->mode == TK_MODE_SPLIT_BF16 || P->mode == TK_MODE_SPLIT6_BF16 ||
         P->mode == TK_MODE_SPLIT_F16)
         return false;
     if (P->nUA != 1 || P->nUB != 1) return false;
     for (int x = 0; x < 4; x++)
         if (P->type[x] != 0 && !(x == 2 && P->beta_zero)) return false;
+    return true;
+}
+
+/* Descriptor of one eligible product; returns false when the product needs
+ * the general path. */
+bool tk_group_describe(struct tapp_plan* plan, struct tapp_exec* ex, const tk_problem* P, const void* A,
+                       const void* B, const void* C, void* D, int force_cls, tk_prob* out, int* rc)
+{
+    *rc = 0;
+    if (!tk_group_may_run(P)) return false;
     /* device operands, or host ranges resident on the plan's handle (A, B, C) */
     A = tk_resolve_operand(plan->handle, A, P->lo[0] * 4, (P->hi[0] + 1) * 4);
     B = tk_resolve_operand(plan->handle, B, P->lo[1] * 4, (P->hi[1] + 1) * 4);

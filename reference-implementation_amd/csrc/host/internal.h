@@ -175,6 +175,21 @@ void tk_plan_cache_free(struct tapp_plan_cache* c);
  * before the first build). */
 const void* tk_plan_tables_id(const struct tapp_plan* plan, uint64_t* key);
 void tk_batch_cache_free(struct tapp_batch_cache* b);
+/* Device memory a call of `p` would allocate through `ex` (asynchronous, as
+ * the serving loop runs it): *work = its workspace, *tables = the plan's
+ * offset tables when not yet on ex's device; *devops = every operand is on
+ * the device or resident. */
+int tk_call_bytes(struct tapp_plan* p, const struct tapp_exec* ex, const tk_problem* P, const void* A,
+                  const void* B, const void* C, void* D, size_t* work, size_t* tables, bool* devops);
+/* The plan's tables for the current layout are not yet on ex's device. */
+bool tk_plan_needs_tables(struct tapp_plan* p, const struct tapp_exec* ex);
+/* Grows the executor's workspace to exactly `bytes` (no headroom) when it is
+ * smaller; frees it. */
+int tk_exec_reserve_ws(struct tapp_exec* ex, size_t bytes);
+void tk_exec_release_ws(struct tapp_exec* ex);
+/* The problem-level conditions of the persistent launch (compute type,
+ * precision form, unary labels, storage types). */
+bool tk_group_may_run(const tk_problem* P);
 int tk_problem_tables(struct tapp_plan* plan, const tk_problem* P, int device, const int64_t** mA,
                       const int64_t** kA, const int64_t** lA, const int64_t** nB, const int64_t** kB,
                       const int64_t** lB, const int64_t** mD, const int64_t** nD, const int64_t** lD,

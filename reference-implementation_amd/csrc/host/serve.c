@@ -6,7 +6,7 @@
  *
  * The dispatcher owns `slots` device contexts (stream + workspace each).
  * Whenever a slot is idle it admits requests from the head of the queue while
- * their staging footprint fits the HBM budget, launches them as one batch on
+ * the device memory they need fits the HBM budget, launches them as one batch on
  * that slot's stream and goes on: requests arriving while batches run are
  * admitted as soon as a slot and budget free up, and batches on different
  * slots overlap on the device.  Inside a batch, FP32 products on device
@@ -20,16 +20,26 @@
  * a second event marks the end of the batch.
  *
  * Heavy requests -- above the persistent launch's flop limit, or with a
- * footprint (host operands to stage plus pre-pass buffers) above
- * heavy_bytes -- hold a stream for a long time.  While
+ * workspace above heavy_bytes -- hold a stream for a long time.  While
  * every other slot holds one, admission passes over them (they keep their
  * place in the queue) and takes the small requests behind them, so small
  * requests never wait for a stream behind heavy ones.
  *
- * Footprint of a request = bytes of its host operands (staged into HBM) plus
- * the pre-pass buffers it needs; device operands and resident tensors cost
- * nothing.  A request larger than the whole budget is rejected at submit
- * (TAPP_AMD_ERR_BUDGET); a request that does not fit now waits (a stall).
+ * Device memory is accounted as the server holds it:
+ *   held = the slots' workspaces (each sized to the largest general-path
+ *          request of its batch: staged host operands, pre-pass and
+ *          conversion buffers, split-K partial sums; kept between batches)
+ *        + offset tables that the batches in flight build for their plans
+ *          (they stay with the plan afterwards, outside the budget).
+ * The persistent launch's descriptor buffers (232 bytes per request of a
+ * batch, per slot) are not counted.
+ * A batch is admitted while held + its growth fits the budget; when the
+ * first request does not fit, the workspaces of idle slots are freed and
+ * the check repeated, and if it still does not fit the request waits (a
+ * stall).  Requests that run in the persistent launch need no workspace;
+ * device operands and resident tensors cost nothing.  A request whose own
+ * workspace and tables exceed the whole budget is rejected at submit
+ * (TAPP_AMD_ERR_BUDGET).
  *
  * A request's completion is a TAPP_status object (status.h:9-17: "status
  * objects are created by execution functions"), waited on with
@@ -56,10 +66,12 @@ typedef struct srv_request {
     unsigned char alpha[16], beta[16];
     const void *A, *B, *C;
     void* D;
-    size_t footprint;
+    size_t work;                /* workspace of a general-path run (0 for the persistent launch) */
+    size_t tables;              /* offset tables the plan still has to build on the device */
     double t_submit, t_start, t_done;
     int in_group;               /* in the batch: 1 persistent launch, 0 general path, -1 nothing to launch */
-    bool heavy;                 /* above the persistent launch's flop limit, or footprint above heavy_bytes */
+    bool heavy;                 /* above the persistent launch's flop limit, or work above heavy_bytes */
+    bool stalled;               /* waited at the head of the queue for memory at least once */
     struct srv_request* next;   /* queue / batch list */
 } srv_request;
 
@@ -67,7 +79,8 @@ typedef struct {
     struct tapp_exec ex;        /* private context: stream, workspace, events */
     bool busy;
     srv_request* batch;         /* requests of the batch in flight */
-    size_t batch_bytes;
+    size_t batch_work;          /* workspace the batch was admitted with */
+    size_t batch_tables;        /* tables its requests build (held while it runs) */
     hipEvent_t done;            /* end of the batch */
     hipEvent_t done_group;      /* end of the batch's persistent launch */
     bool group_pending;         /* its requests not yet completed */
@@ -84,9 +97,10 @@ struct tapp_server {
     uint32_t magic;
     int device;
     int nslots, max_batch;
-    size_t budget, in_use;
+    size_t budget;
+    size_t tables_held;         /* tables of the batches in flight */
     int64_t large_flop;         /* products above this run on the general GETT path */
-    size_t heavy_bytes;         /* a footprint above this makes a request heavy */
+    size_t heavy_bytes;         /* a workspace above this makes a request heavy */
     int cus;
     srv_slot slot[SRV_MAX_SLOTS];
     pthread_t thread;
@@ -96,7 +110,8 @@ struct tapp_server {
     srv_request *head, *tail;
     int queued;
     /* statistics */
-    int64_t completed, stalls, batches, group_products, max_inflight, inflight;
+    int64_t completed, stalls, batches, group_products, max_inflight, inflight, reclaims;
+    size_t max_held, held_now;
     double* lat;                /* completed latencies, microseconds */
     size_t nlat, caplat;
     double flop_done, t_first, t_last;
@@ -152,6 +167,12 @@ static void launch_batch(struct tapp_server* sv, srv_slot* s)
     size_t count = 0;
     s->group_pending = false;
     for (srv_request* r = s->batch; r; r = r->next) count++;
+    /* the workspace the batch was admitted with, in one allocation */
+    if (s->batch_work && tk_exec_reserve_ws(ex, s->batch_work)) {
+        for (srv_request* r = s->batch; r; r = r->next) { r->rc = TAPP_AMD_ERR_NOMEM; r->in_group = -1; }
+        hipEventRecord(s->done, ex->stream);
+        return;
+    }
     if (ensure_desc(s, count)) {
         for (srv_request* r = s->batch; r; r = r->next) { r->rc = TAPP_AMD_ERR_NOMEM; r->in_group = -1; }
         hipEventRecord(s->done, ex->stream);
@@ -205,6 +226,34 @@ static void launch_batch(struct tapp_server* sv, srv_slot* s)
 
 /* --- dispatcher ----------------------------------------------------------- */
 
+/* Device memory the server holds against its budget now (dispatcher
+ * thread: the only one that changes the workspaces). */
+static size_t held_bytes(const struct tapp_server* sv)
+{
+    size_t h = sv->tables_held;
+    for (int i = 0; i < sv->nslots; i++) h += sv->slot[i].ex.ws_size;
+    return h;
+}
+
+/* Frees the workspaces of idle slots (all of them, `keep` included when its
+ * workspace is larger than `keep_need`). */
+static void reclaim(struct tapp_server* sv, int keep, size_t keep_need)
+{
+    for (int i = 0; i < sv->nslots; i++) {
+        srv_slot* o = &sv->slot[i];
+        if (o->busy || !o->ex.ws || (i == keep && o->ex.ws_size <= keep_need)) continue;
+        tk_exec_release_ws(&o->ex);
+        sv->reclaims++;
+    }
+}
+
+static bool plan_in(const srv_request* list, const srv_request* stop, const struct tapp_plan* p)
+{
+    for (const srv_request* q = list; q && q != stop; q = q->next)
+        if (q->plan == p) return true;
+    return false;
+}
+
 /* Completes the batch's requests that are done: with group_only, those of
  * the persistent launch; otherwise all that remain, and the slot frees. */
 static void finish_requests(struct tapp_server* sv, srv_slot* s, bool group_only)
@@ -230,9 +279,10 @@ static void finish_requests(struct tapp_server* sv, srv_slot* s, bool group_only
         release(r);
     }
     if (group_only) return;
-    sv->in_use -= s->batch_bytes;
+    sv->tables_held -= s->batch_tables;
     s->batch = NULL;
-    s->batch_bytes = 0;
+    s->batch_tables = 0;
+    s->batch_work = 0;
     s->busy = false;
     s->group_pending = false;
 }
@@ -271,8 +321,9 @@ static void* dispatcher(void* arg)
                 srv_request *first = NULL, *last = NULL, *prev = NULL;
                 srv_request** link = &sv->head;
                 int taken = 0;
-                size_t bytes = 0;
+                size_t need = 0, tabs = 0;
                 bool heavy = false;
+                size_t held = held_bytes(sv);
                 while (*link && taken < sv->max_batch) {
                     srv_request* r = *link;
                     if (r->state == 3) {          /* cancelled while queued */
@@ -288,8 +339,18 @@ static void* dispatcher(void* arg)
                         link = &r->next;
                         continue;
                     }
-                    if (sv->in_use + bytes + r->footprint > sv->budget) {
-                        if (taken == 0) sv->stalls++;
+                    /* tables count once per plan and only while not yet built */
+                    const size_t t = (r->tables && !plan_in(first, NULL, r->plan) && tk_plan_needs_tables(r->plan, &s->ex))
+                                         ? r->tables : 0;
+                    const size_t n2 = r->work > need ? r->work : need;
+                    size_t grow = (n2 > s->ex.ws_size ? n2 - s->ex.ws_size : 0) + tabs + t;
+                    if (held + grow > sv->budget && taken == 0) {
+                        reclaim(sv, i, n2);
+                        held = held_bytes(sv);
+                        grow = (n2 > s->ex.ws_size ? n2 - s->ex.ws_size : 0) + t;
+                    }
+                    if (held + grow > sv->budget) {
+                        if (taken == 0 && !r->stalled) { r->stalled = true; sv->stalls++; }
                         break;
                     }
                     *link = r->next;
@@ -299,15 +360,17 @@ static void* dispatcher(void* arg)
                     r->state = 1;
                     if (last) last->next = r; else first = r;
                     last = r;
-                    bytes += r->footprint;
+                    need = n2;
+                    tabs += t;
                     heavy |= r->heavy;
                     taken++;
                 }
                 if (!taken) break;
                 s->batch = first;
                 s->heavy = heavy;
-                s->batch_bytes = bytes;
-                sv->in_use += bytes;
+                s->batch_work = need;
+                s->batch_tables = tabs;
+                sv->tables_held += tabs;
                 sv->inflight += taken;
                 if (sv->inflight > sv->max_inflight) sv->max_inflight = sv->inflight;
                 sv->batches++;
@@ -318,6 +381,8 @@ static void* dispatcher(void* arg)
                 progressed = true;
             }
         }
+        sv->held_now = held_bytes(sv);
+        if (sv->held_now > sv->max_held) sv->max_held = sv->held_now;
         bool any_busy = false;
         for (int i = 0; i < sv->nslots; i++) any_busy |= sv->slot[i].busy;
         if (sv->stop && !sv->head && !any_busy) break;
@@ -395,33 +460,23 @@ TAPP_error TAPP_AMD_create_server(TAPP_AMD_server* server, TAPP_executor exec, s
     return 0;
 }
 
-/* staging footprint of a request: host operands over their address spans,
- * pre-pass buffers for unary labels or storage conversion */
-static int footprint(struct tapp_plan* p, const void* alpha, const void* A, const void* B, const void* beta,
-                     const void* C, void* D, size_t* out, int64_t* flop)
+/* Device memory a request needs: the workspace of a general-path run
+ * (staged host operands, pre-pass and conversion buffers, split-K
+ * partials) -- none when it will run in the persistent launch -- and the
+ * offset tables its plan has yet to build on the device. */
+static int footprint(struct tapp_server* sv, struct tapp_plan* p, const void* alpha, const void* A, const void* B,
+                     const void* beta, const void* C, void* D, size_t* work, size_t* tables, int64_t* flop)
 {
-    struct tapp_exec probe;
-    memset(&probe, 0, sizeof probe);
-    probe.kind = 1;
+    const struct tapp_exec* ex = &sv->slot[0].ex;   /* the slots' attributes */
     tk_problem P;
-    int rc = tk_plan_problem(p, &probe, alpha, beta, C, &P);
+    int rc = tk_plan_problem(p, ex, alpha, beta, C, &P);
     if (rc) return rc;
-    size_t f = 0;
-    const void* ops[4] = {A, B, P.beta_zero ? NULL : C, D};
-    for (int x = 0; x < 4; x++) {
-        if (!ops[x] || P.hi[x] < P.lo[x]) continue;
-        const int64_t es = (int64_t)tk_type_size(P.type[x]);
-        const bool dev = x < 3 ? tk_resolve_operand(p->handle, ops[x], P.lo[x] * es, (P.hi[x] + 1) * es) != NULL
-                               : tk_is_device_ptr(ops[x]);
-        if (!dev) f += (size_t)(P.hi[x] - P.lo[x] + 1) * (size_t)es;
-    }
-    size_t cs = (P.ctype == TK_F32) ? 4 : (P.ctype == TK_C64) ? 16 : 8;
-    if (P.nUA > 1 || P.type[0] != (P.ctype == TK_F32 ? 0 : P.ctype == TK_F64 ? 1 : P.ctype == TK_C32 ? 2 : 3))
-        f += (size_t)(P.nM * P.nK * P.nL) * cs;
-    if (P.nUB > 1 || P.type[1] != (P.ctype == TK_F32 ? 0 : P.ctype == TK_F64 ? 1 : P.ctype == TK_C32 ? 2 : 3))
-        f += (size_t)(P.nN * P.nK * P.nL) * cs;
-    *out = f;
+    *work = *tables = 0;
     *flop = P.empty ? 0 : product_flop(&P);
+    if (P.empty) return 0;
+    bool devops = false;
+    if ((rc = tk_call_bytes(p, ex, &P, A, B, C, D, work, tables, &devops))) return rc;
+    if (devops && *flop <= sv->large_flop && tk_group_may_run(&P)) *work = 0;
     return 0;
 }
 
@@ -432,11 +487,11 @@ TAPP_error TAPP_AMD_server_submit(TAPP_AMD_server server, TAPP_tensor_product pl
     struct tapp_server* sv = (struct tapp_server*)server;
     if (!sv || sv->magic != TAPP_AMD_EXEC_MAGIC || !plan || !status) return TAPP_AMD_ERR_INVALID;
     struct tapp_plan* p = (struct tapp_plan*)plan;
-    size_t f = 0;
+    size_t work = 0, tables = 0;
     int64_t fl = 0;
-    int rc = footprint(p, alpha, A, B, beta, C, D, &f, &fl);
+    int rc = footprint(sv, p, alpha, A, B, beta, C, D, &work, &tables, &fl);
     if (rc) return rc;
-    if (f > sv->budget) return TAPP_AMD_ERR_BUDGET;
+    if (work + tables > sv->budget) return TAPP_AMD_ERR_BUDGET;
     srv_request* r = calloc(1, sizeof *r);
     if (!r) return TAPP_AMD_ERR_NOMEM;
     r->magic = STATUS_MAGIC;
@@ -447,8 +502,9 @@ TAPP_error TAPP_AMD_server_submit(TAPP_AMD_server server, TAPP_tensor_product pl
     if (alpha && ss) memcpy(r->alpha, alpha, ss);
     if (beta && ss) memcpy(r->beta, beta, ss);
     r->A = A; r->B = B; r->C = C; r->D = D;
-    r->footprint = f;
-    r->heavy = fl > sv->large_flop || f > sv->heavy_bytes;
+    r->work = work;
+    r->tables = tables;
+    r->heavy = fl > sv->large_flop || work > sv->heavy_bytes;
     r->t_submit = now_us();
     pthread_mutex_lock(&sv->mu);
     if (!sv->t_first) sv->t_first = r->t_submit;
@@ -541,13 +597,14 @@ static int cmp_d(const void* a, const void* b)
 /* stats: [0] completed, [1] p50 us, [2] p99 us, [3] mean us, [4] stalls,
  * [5] max in flight, [6] batches, [7] products in persistent launches,
  * [8] GFLOP/s over first submit .. last completion, [9] queued now,
- * [10] budget bytes in use now.  reset != 0 clears the counters. */
-TAPP_error TAPP_AMD_server_stats(TAPP_AMD_server server, double stats[11], int reset)
+ * [4] counts requests that waited for memory at least once, [10] device
+ * bytes held now, [11] most device bytes held, [12] workspace reclaims.  reset != 0 clears the counters (not the bytes held now). */
+TAPP_error TAPP_AMD_server_stats(TAPP_AMD_server server, double stats[13], int reset)
 {
     struct tapp_server* sv = (struct tapp_server*)server;
     if (!sv || sv->magic != TAPP_AMD_EXEC_MAGIC || !stats) return TAPP_AMD_ERR_INVALID;
     pthread_mutex_lock(&sv->mu);
-    memset(stats, 0, sizeof(double) * 11);
+    memset(stats, 0, sizeof(double) * 13);
     stats[0] = (double)sv->completed;
     if (sv->nlat) {
         double* c = malloc(sizeof(double) * sv->nlat);
@@ -568,9 +625,12 @@ TAPP_error TAPP_AMD_server_stats(TAPP_AMD_server server, double stats[11], int r
     stats[7] = (double)sv->group_products;
     if (sv->t_last > sv->t_first) stats[8] = sv->flop_done / ((sv->t_last - sv->t_first) * 1e-6) / 1e9;
     stats[9] = (double)sv->queued;
-    stats[10] = (double)sv->in_use;
+    stats[10] = (double)sv->held_now;
+    stats[11] = (double)sv->max_held;
+    stats[12] = (double)sv->reclaims;
     if (reset) {
-        sv->completed = sv->stalls = sv->batches = sv->group_products = sv->max_inflight = 0;
+        sv->completed = sv->stalls = sv->batches = sv->group_products = sv->max_inflight = sv->reclaims = 0;
+        sv->max_held = sv->held_now;
         sv->nlat = 0;
         sv->flop_done = 0;
         sv->t_first = sv->t_last = 0;

@@ -3,7 +3,9 @@
  * tests, not part of the TAPP API).  Request i is submitted at
  * t0 + arrival_us[i] -- the submitting thread spins until then, so arrivals
  * follow the given (e.g. Poisson) schedule however the server keeps up --
- * and every status is waited on and released at the end.
+ * and every status is waited on and released at the end.  The same trace
+ * can be run one TAPP_execute_product call at a time instead, back to back
+ * on one executor, as the baseline the serving loop is compared with.
  */
 #define _POSIX_C_SOURCE 200809L
 #include <stdlib.h>
@@ -52,5 +54,26 @@ __attribute__((visibility("default"))) int tapp_trace_replay(TAPP_AMD_server ser
     }
     free(st);
     if (late_us) *late_us = late;
+    return failed;
+}
+
+/* Request i as one TAPP_execute_product call on `exec`, one after another
+ * (each returns when its D is written); arrivals are not waited for, so
+ * this is the most a one-call-at-a-time client gets through.  Returns the
+ * number of failed calls; *elapsed_us receives the wall time. */
+__attribute__((visibility("default"))) int tapp_trace_sequential(TAPP_executor exec, int n,
+                                                                 const TAPP_tensor_product* plans,
+                                                                 const void* const* alpha, const void* const* A,
+                                                                 const void* const* B, const void* const* beta,
+                                                                 const void* const* C, void* const* D, int* rc_out,
+                                                                 double* elapsed_us)
+{
+    int failed = 0;
+    const double t0 = now_us();
+    for (int i = 0; i < n; i++) {
+        rc_out[i] = TAPP_execute_product(plans[i], exec, NULL, alpha[i], A[i], B[i], beta[i], C ? C[i] : NULL, D[i]);
+        if (rc_out[i]) failed++;
+    }
+    if (elapsed_us) *elapsed_us = now_us() - t0;
     return failed;
 }

@@ -271,7 +271,7 @@ class TappLib:
     # --- serving -------------------------------------------------------------
     SERVER_PAUSED = 1
     STAT_NAMES = ("completed", "p50_us", "p99_us", "mean_us", "stalls", "max_inflight", "batches",
-                  "persistent_products", "gflops", "queued", "budget_in_use")
+                  "persistent_products", "gflops", "queued", "held_bytes", "max_held_bytes", "reclaims")
 
     def create_server(self, exec_, budget_bytes, slots=4, max_batch=64, flags=0):
         h = _H()
@@ -302,7 +302,7 @@ class TappLib:
         return self.lib.TAPP_AMD_server_drain(srv)
 
     def server_stats(self, srv, reset=False):
-        v = (ctypes.c_double * 11)()
+        v = (ctypes.c_double * len(self.STAT_NAMES))()
         rc = self.lib.TAPP_AMD_server_stats(srv, v, int(reset))
         return rc, dict(zip(self.STAT_NAMES, list(v)))
 
@@ -347,3 +347,20 @@ def trace_replay(lib, srv, plans, alphas, a_ptrs, b_ptrs, betas, c_ptrs, d_ptrs,
     failed = tr.tapp_trace_replay(srv, n, (_H * max(1, n))(*plans), VP(*alphas), VP(*a_ptrs), VP(*b_ptrs),
                                   VP(*betas), VP(*c_ptrs), VP(*d_ptrs), arr, rcs, ctypes.byref(late))
     return failed, list(rcs)[:n], late.value
+
+
+def trace_sequential(lib, exec_, plans, alphas, a_ptrs, b_ptrs, betas, c_ptrs, d_ptrs):
+    """The same requests as one TAPP_execute_product call each, back to back
+    on one executor (from C).  Returns (failed calls, per-request codes,
+    elapsed seconds)."""
+    _single_hip_runtime()
+    tr = ctypes.CDLL(TRACE_PATH, mode=os.RTLD_LOCAL)
+    n = len(plans)
+    VP = ctypes.c_void_p * max(1, n)
+    tr.tapp_trace_sequential.argtypes = [_H, ctypes.c_int, ctypes.POINTER(_H)] + [ctypes.POINTER(ctypes.c_void_p)] * 6 + [
+        ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double)]
+    rcs = (ctypes.c_int * max(1, n))()
+    el = ctypes.c_double(0)
+    failed = tr.tapp_trace_sequential(exec_, n, (_H * max(1, n))(*plans), VP(*alphas), VP(*a_ptrs), VP(*b_ptrs),
+                                      VP(*betas), VP(*c_ptrs), VP(*d_ptrs), rcs, ctypes.byref(el))
+    return failed, list(rcs)[:n], el.value * 1e-6

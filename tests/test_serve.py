@@ -252,3 +252,43 @@ def test_concurrent_client_threads(lib, gpu_exec, oracle):
         for d, a in zip(dcs, row):
             assert_parity(d.d(), a, d.case)
             d.close()
+
+
+@pytest.mark.gpu
+def test_budget_counts_workspace_and_split_k_partials(lib, gpu_exec, oracle):
+    """Requests on the general path with a staged host A (1 MiB) and split-K
+    partial sums (64 slices of one 128 x 128 tile: 4 MiB): 5 MiB of workspace
+    each.  With an 8 MiB budget only one runs at a time although three
+    streams are free: the others stall, idle workspaces are reclaimed, the
+    server never holds more than the budget, and every D is right."""
+    from test_gpu_parity import _gemm_case, assert_parity
+    cases = [_gemm_case(tc.F32, 128, 128, 2048, 40 + i) for i in range(6)]
+    plan, infos = _plan(lib, cases[0])
+    budget = 8 << 20
+    rc, srv = lib.create_server(gpu_exec, budget, 3, 4, lib.SERVER_PAUSED)
+    assert rc == 0
+    import torch
+    keep, sts = [], []
+    for c in cases:
+        dev = {n: torch.from_numpy(o.buf.copy()).cuda() for n, o in zip("BCD", (c.B, c.C, c.D))}
+        torch.cuda.synchronize()
+        al = T.scalar_bytes(c.alpha, c.D.type)
+        be = T.scalar_bytes(c.beta, c.D.type)
+        rc, st = lib.submit(srv, plan, al.ctypes.data, c.A.ptr(), dev["B"].data_ptr(), be.ctypes.data,
+                            dev["C"].data_ptr(), dev["D"].data_ptr())
+        assert rc == 0, lib.explain(rc)
+        keep.append((c, dev, al, be))
+        sts.append(st)
+    assert lib.set_paused(srv, 0) == 0
+    for st in sts:
+        assert lib.wait(srv, st) == (0, 0)
+        lib.destroy_status(st)
+    rc, stats = lib.server_stats(srv)
+    assert stats["completed"] == 6 and stats["stalls"] > 0, stats
+    assert 5 << 20 <= stats["max_held_bytes"] <= budget, stats
+    assert stats["persistent_products"] == 0, stats
+    for c, dev, _, _ in keep:
+        _, want = oracle.run(c, gpu=True)
+        assert_parity(dev["D"].cpu().numpy(), want, c)
+    lib.destroy_server(srv)
+    _free(lib, plan, infos)
