@@ -48,6 +48,7 @@
 #define _POSIX_C_SOURCE 200809L
 #include <math.h>
 #include <pthread.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 #include <time.h>
@@ -84,6 +85,7 @@ typedef struct {
     hipEvent_t done;            /* end of the batch */
     hipEvent_t done_group;      /* end of the batch's persistent launch */
     bool group_pending;         /* its requests not yet completed */
+    bool failed;                /* the device reported an error while the batch ran */
     bool heavy;                 /* the batch holds a heavy request */
     /* descriptor staging for the persistent launch */
     tk_prob* hprobs;
@@ -226,6 +228,27 @@ static void launch_batch(struct tapp_server* sv, srv_slot* s)
 
 /* --- dispatcher ----------------------------------------------------------- */
 
+/* Whether a batch event has completed.  hipEventQuery answers success or
+ * not-ready; any other answer is an error that says nothing about the
+ * batch's progress, so instead of polling on (the batch would never
+ * complete) the slot's stream is waited for, and the batch fails if that
+ * reports an error too. */
+static bool event_done(srv_slot* s, hipEvent_t e)
+{
+    const hipError_t q = hipEventQuery(e);
+    if (q == hipSuccess) return true;
+    if (q == hipErrorNotReady) return false;
+    (void)hipGetLastError();
+    const hipError_t w = hipStreamSynchronize(s->ex.stream);
+    if (w != hipSuccess) {
+        (void)hipGetLastError();
+        s->failed = true;
+    }
+    fprintf(stderr, "tapp_amd serving loop: hipEventQuery returned %d (%s); stream wait returned %d\n", (int)q,
+            hipGetErrorString(q), (int)w);
+    return true;
+}
+
 /* Device memory the server holds against its budget now (dispatcher
  * thread: the only one that changes the workspaces). */
 static size_t held_bytes(const struct tapp_server* sv)
@@ -265,6 +288,7 @@ static void finish_requests(struct tapp_server* sv, srv_slot* s, bool group_only
         if (group_only && r->in_group != 1) { link = &r->next; continue; }
         *link = r->next;
         r->t_done = t;
+        if (s->failed && r->rc == 0) r->rc = TAPP_AMD_ERR_DEVICE;
         r->state = 2;
         if (sv->nlat == sv->caplat) {
             size_t cap = sv->caplat ? sv->caplat * 2 : 4096;
@@ -285,6 +309,7 @@ static void finish_requests(struct tapp_server* sv, srv_slot* s, bool group_only
     s->batch_work = 0;
     s->busy = false;
     s->group_pending = false;
+    s->failed = false;
 }
 
 static void* dispatcher(void* arg)
@@ -297,12 +322,12 @@ static void* dispatcher(void* arg)
         /* completions */
         for (int i = 0; i < sv->nslots; i++) {
             srv_slot* s = &sv->slot[i];
-            if (s->busy && s->group_pending && hipEventQuery(s->done_group) == hipSuccess) {
+            if (s->busy && s->group_pending && event_done(s, s->done_group)) {
                 finish_requests(sv, s, true);
                 s->group_pending = false;
                 progressed = true;
             }
-            if (s->busy && hipEventQuery(s->done) == hipSuccess) {
+            if (s->busy && event_done(s, s->done)) {
                 finish_requests(sv, s, false);
                 progressed = true;
             }
@@ -350,6 +375,21 @@ static void* dispatcher(void* arg)
                         grow = (n2 > s->ex.ws_size ? n2 - s->ex.ws_size : 0) + t;
                     }
                     if (held + grow > sv->budget) {
+                        bool any = false;
+                        for (int j = 0; j < sv->nslots; j++) any |= sv->slot[j].busy;
+                        if (taken == 0 && !any) {
+                            /* nothing runs and nothing more can be freed: it will
+                             * never fit (its plan's layout grew since submit) */
+                            *link = r->next;
+                            if (sv->tail == r) sv->tail = prev;
+                            sv->queued--;
+                            r->next = NULL;
+                            r->rc = TAPP_AMD_ERR_BUDGET;
+                            r->state = 2;
+                            release(r);
+                            pthread_cond_broadcast(&sv->cv_done);
+                            continue;
+                        }
                         if (taken == 0 && !r->stalled) { r->stalled = true; sv->stalls++; }
                         break;
                     }

@@ -155,9 +155,6 @@ def test_resident_operands_in_group(lib, gpu_exec, oracle):
 
 
 @pytest.mark.gpu
-@pytest.mark.skip(reason="status_wait did not return once on the GPU when this ran after the serving tests; "
-                         "the cause is not shown yet (the same call sequence completes on a CPU stand-in of the "
-                         "HIP runtime), so the case stays off the GPU")
 def test_resident_operands_in_group_and_server(lib, gpu_exec, oracle):
     """Product groups run resident host operands in the persistent launch;
     the serving loop does not charge them to the HBM budget."""
