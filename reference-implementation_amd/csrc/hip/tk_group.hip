@@ -51,6 +51,7 @@ struct GView {
     __device__ __forceinline__ int64_t S() const { return d->S; }
     __device__ __forceinline__ int64_t G() const { return d->G; }
     __device__ __forceinline__ int64_t K() const { return d->K; }
+    __device__ __forceinline__ int64_t L() const { return d->L; }
     __device__ __forceinline__ gf X() const { return (gf)d->X; }
     __device__ __forceinline__ gf Y() const { return (gf)d->Y; }
     __device__ __forceinline__ gf C() const { return (gf)d->C; }
@@ -82,21 +83,21 @@ __device__ __forceinline__ float rnd_in(float x, int mode)
     return x;
 }
 
-// C value of output (l, s, g) as the epilogue scales it.  Tiles read all the
-// C values of a lane before any of its D stores: C and D may alias, so a
+// C value at element offset `off` as the epilogue scales it.  Tiles read all
+// the C values of a lane before any of its D stores: C and D may alias, so a
 // load issued after a store has to wait for it.
-__device__ __forceinline__ float group_c(const GView& p, int64_t l, int64_t s, int64_t g)
+__device__ __forceinline__ float group_c(const GView& p, int64_t off)
 {
     if (p.beta_zero()) return 0.0f;
-    float c = p.C()[p.lC()[l] + p.sC()[s] + p.gC()[g]];
+    const float c = p.C()[off];
     return p.round_C() ? rnd_in(c, p.round()) : c;
 }
 
-__device__ __forceinline__ void group_store(const GView& p, int64_t l, int64_t s, int64_t g, float acc, float c)
+__device__ __forceinline__ void group_store(const GView& p, int64_t off, float acc, float c)
 {
     float out = p.alpha() * acc;
     if (!p.beta_zero()) out += p.beta() * c;
-    p.D()[p.lD()[l] + p.sD()[s] + p.gD()[g]] = out;
+    p.D()[off] = out;
 }
 
 template <int RM>
@@ -112,9 +113,14 @@ __device__ __forceinline__ float rnd_m(float x)
 // one contiguous block) is prefetched: a chunk's loads, unconditional at
 // valid addresses and masked when stored, are issued into registers right
 // after the previous chunk reached LDS, so they are in flight during its
-// arithmetic.  Other Y layouts load each chunk when it is needed.
+// arithmetic.  Other Y layouts load each chunk when it is needed.  The
+// offsets of C and D are read with the column offsets at the start (the
+// lane's column in registers, the rows in LDS `soff`), so the epilogue waits
+// on no table; so are X's first KO K offsets (into `kxs`), so a chunk's X
+// load depends on no table load of its own.
 template <int SB, int RM>
-__device__ __forceinline__ void thin_tile(const GView& p, int64_t l, int64_t g0, float* lds, int64_t* goff)
+__device__ __forceinline__ void thin_tile(const GView& p, int64_t l, int64_t g0, float* lds, int64_t* goff,
+                                          int64_t* soff, int64_t* kxs)
 {
     constexpr int NY = KC * GT / 256;           // Y elements per lane and chunk
     float* Yl = lds;               // [KC][YP]
@@ -125,15 +131,34 @@ __device__ __forceinline__ void thin_tile(const GView& p, int64_t l, int64_t g0,
     for (int s = 0; s < SB; ++s) acc[s] = 0.0f;
     const int64_t lx = p.lX()[l], ly = p.lY()[l];
     const int gn = (int)(p.G() - g0 < GT ? p.G() - g0 : GT);
-    // column offsets of this tile, once (the K loop reuses them); column 0's
-    // offset stands in for columns past the end
-    __syncthreads();
-    goff[tid] = p.gY()[g0 + (tid < gn ? tid : 0)];
-    // this lane's X row (fixed for the tile)
+    const int gl = tid < gn ? tid : 0;
+    const int nks = (int)(p.K() < KO ? p.K() : KO);
     const int xs = tid / KC, xk = tid % KC;
     const bool x_row = tid < SB * KC && xs < p.S();
-    const int64_t xrow = lx + p.sX()[x_row ? xs : 0];
+    const int si = tid < SB && tid < p.S() ? tid : 0;
+    // every table read of the tile, issued together at valid indices (C's
+    // tables exist whether or not beta is 0) before any of them is waited
+    // for: the column offsets of Y, D and C, the rows of D and C, X's K
+    // offsets and this lane's X row
+    const int64_t gy = p.gY()[g0 + gl], gd = p.gD()[g0 + gl], gc = p.gC()[g0 + gl];
+    const int64_t ldl = p.lD()[l], lcl = p.lC()[l];
+    const int64_t sd = p.sD()[si], sc = p.sC()[si];
+    const int64_t kx0 = p.kX()[tid < nks ? tid : 0];
+    const int64_t sx = p.sX()[x_row ? xs : 0];
     __syncthreads();
+    goff[tid] = gy;
+    if (tid < SB) {
+        soff[tid] = sd;
+        soff[SMAX + tid] = p.beta_zero() ? 0 : sc;
+    }
+    if (tid < nks) kxs[tid] = kx0;
+    for (int k = tid + 256; k < nks; k += 256) kxs[k] = p.kX()[k];
+    const int64_t dcol = ldl + gd;
+    const int64_t ccol = p.beta_zero() ? 0 : lcl + gc;
+    const int64_t xrow = lx + sx;
+    __syncthreads();
+    // X's K offset of this lane's chunk element (staged for k < KO)
+    auto kx_of = [&](int64_t k) -> int64_t { return k < KO ? kxs[k] : p.kX()[k]; };
     const bool dense = p.y_kfast() == 2;
     const int64_t g0off = goff[0];
     float yv[NY], xv = 0.0f;
@@ -142,7 +167,7 @@ __device__ __forceinline__ void thin_tile(const GView& p, int64_t l, int64_t g0,
     // K < 2^21)
     auto fetch_dense = [&](int64_t k0) {
         const int kc = (int)(p.K() - k0 < KC ? p.K() - k0 : KC);
-        xv = p.X()[xrow + p.kX()[k0 + (xk < kc ? xk : 0)]];
+        xv = p.X()[xrow + kx_of(k0 + (xk < kc ? xk : 0))];
         gf yb = p.Y() + ly + g0off + k0;
         const uint32_t kk = (uint32_t)(xk < kc ? xk : 0);
 #pragma unroll
@@ -155,7 +180,7 @@ __device__ __forceinline__ void thin_tile(const GView& p, int64_t l, int64_t g0,
     for (int64_t k0 = 0; k0 < p.K(); k0 += KC) {
         const int kc = (int)(p.K() - k0 < KC ? p.K() - k0 : KC);
         __syncthreads();                          // the previous chunk's LDS reads are done
-        if (!dense) xv = p.X()[xrow + p.kX()[k0 + (xk < kc ? xk : 0)]];
+        if (!dense) xv = p.X()[xrow + kx_of(k0 + (xk < kc ? xk : 0))];
         if (tid < SB * KC) Xl[tid] = (x_row && xk < kc) ? rnd_m<RM>(xv) : 0.0f;
         if (dense) {
 #pragma unroll
@@ -191,14 +216,13 @@ __device__ __forceinline__ void thin_tile(const GView& p, int64_t l, int64_t g0,
             for (int s = 0; s < SB; ++s) acc[s] += Xl[s * KC + k] * y;
         }
     }
-    const int64_t g = g0 + tid;
-    if (g < p.G()) {
+    if (tid < gn) {
         float cv[SB];
 #pragma unroll
-        for (int s = 0; s < SB; ++s) cv[s] = s < p.S() ? group_c(p, l, s, g) : 0.0f;
+        for (int s = 0; s < SB; ++s) cv[s] = s < p.S() ? group_c(p, ccol + soff[SMAX + s]) : 0.0f;
 #pragma unroll
         for (int s = 0; s < SB; ++s)
-            if (s < p.S()) group_store(p, l, s, g, acc[s], cv[s]);
+            if (s < p.S()) group_store(p, dcol + soff[s], acc[s], cv[s]);
     }
 }
 
@@ -320,7 +344,7 @@ __device__ __forceinline__ void mfma_chunk(gf Xb, gf Yb, const int32_t (&arow)[2
 
 template <int RM>
 __device__ __forceinline__ void mfma_tile(const GView& p, int64_t l, int64_t m0, int64_t n0, int64_t* kox,
-                                          int64_t* koy)
+                                          int64_t* koy, int64_t* eoff)
 {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int il = lane & 31, kl = lane >> 5;
@@ -342,6 +366,21 @@ __device__ __forceinline__ void mfma_tile(const GView& p, int64_t l, int64_t m0,
         arow[i] = (int32_t)(lx + p.sX()[a_ok[i] ? m : 0] - xlo);
         bcol[i] = (int32_t)(ly + p.gY()[b_ok[i] ? n : 0] - ylo);
     }
+    // the tile's row and column offsets of C and D, read now into LDS (the
+    // epilogue then waits on no table): eoff = C rows | D rows | C columns |
+    // D columns.  Every lane loads (lanes MT.. repeat the first MT, at valid
+    // indices) and the first MT store, so no load sits in a branch.
+    {
+        const int et = tid & (MT - 1);
+        const int64_t m = m0 + et < p.S() ? m0 + et : 0, n = n0 + et < p.G() ? n0 + et : 0;
+        const int64_t esc = p.sC()[m], esd = p.sD()[m], egc = p.gC()[n], egd = p.gD()[n];
+        if (tid < MT) {
+            eoff[tid] = p.beta_zero() ? 0 : esc;
+            eoff[MT + tid] = esd;
+            eoff[2 * MT + tid] = p.beta_zero() ? 0 : egc;
+            eoff[3 * MT + tid] = egd;
+        }
+    }
     f32x16 acc[2][2];
 #pragma unroll
     for (int ib = 0; ib < 2; ++ib)
@@ -359,18 +398,7 @@ __device__ __forceinline__ void mfma_tile(const GView& p, int64_t l, int64_t m0,
         __syncthreads();
         if (active) mfma_chunk<RM>(Xb, Yb, arow, bcol, a_ok, b_ok, kx, ky, kc, kl, acc);
     }
-    // the tile's row and column offsets of C and D, staged once in LDS (the
-    // K offsets are no longer read): kox = C rows | D rows, koy = C columns
-    // | D columns; an output then costs one C load and one D store
-    __syncthreads();
-    if (tid < MT) {
-        const int64_t m = m0 + tid < p.S() ? m0 + tid : 0, n = n0 + tid < p.G() ? n0 + tid : 0;
-        kox[tid] = p.beta_zero() ? 0 : p.sC()[m];
-        kox[MT + tid] = p.sD()[m];
-        koy[tid] = p.beta_zero() ? 0 : p.gC()[n];
-        koy[MT + tid] = p.gD()[n];
-    }
-    __syncthreads();
+    __syncthreads();                             // eoff is visible (also when K is 0)
     if (!active) return;
     const int64_t lc = p.beta_zero() ? 0 : p.lC()[l], ld = p.lD()[l];
     // acc[ib][ia][r] of lane holds out[wn0 + 32 ib + 8 (r / 4) + 4 kl + r % 4][wm0 + 32 ia + il];
@@ -379,7 +407,7 @@ __device__ __forceinline__ void mfma_tile(const GView& p, int64_t l, int64_t m0,
     for (int ia = 0; ia < 2; ++ia) {
         if (!a_ok[ia]) continue;
         const int mt = (int)(wm0 - m0) + 32 * ia + il;
-        const int64_t rc = lc + kox[mt], rd = ld + kox[MT + mt];
+        const int64_t rc = lc + eoff[mt], rd = ld + eoff[MT + mt];
 #pragma unroll
         for (int ib = 0; ib < 2; ++ib) {
             float cv[16];
@@ -388,7 +416,7 @@ __device__ __forceinline__ void mfma_tile(const GView& p, int64_t l, int64_t m0,
                 const int nt = (int)(wn0 - n0) + 32 * ib + 8 * (r >> 2) + 4 * kl + (r & 3);
                 float c = 0.0f;
                 if (!p.beta_zero() && n0 + nt < p.G()) {
-                    c = p.C()[rc + koy[nt]];
+                    c = p.C()[rc + eoff[2 * MT + nt]];
                     if (p.round_C()) c = rnd_in(c, p.round());
                 }
                 cv[r] = c;
@@ -399,7 +427,7 @@ __device__ __forceinline__ void mfma_tile(const GView& p, int64_t l, int64_t m0,
                 if (n0 + nt < p.G()) {
                     float out = p.alpha() * acc[ib][ia][r];
                     if (!p.beta_zero()) out += p.beta() * cv[r];
-                    p.D()[rd + koy[MT + nt]] = out;
+                    p.D()[rd + eoff[3 * MT + nt]] = out;
                 }
             }
         }
@@ -407,11 +435,12 @@ __device__ __forceinline__ void mfma_tile(const GView& p, int64_t l, int64_t m0,
 }
 
 template <int RM>
-__device__ __forceinline__ void thin_dispatch(const GView& p, int64_t l, int64_t g0, float* lds, int64_t* goff)
+__device__ __forceinline__ void thin_dispatch(const GView& p, int64_t l, int64_t g0, float* lds, int64_t* goff,
+                                              int64_t* soff, int64_t* kxs)
 {
-    if (p.S() <= 4) thin_tile<4, RM>(p, l, g0, lds, goff);
-    else if (p.S() <= 8) thin_tile<8, RM>(p, l, g0, lds, goff);
-    else thin_tile<16, RM>(p, l, g0, lds, goff);
+    if (p.S() <= 4) thin_tile<4, RM>(p, l, g0, lds, goff, soff, kxs);
+    else if (p.S() <= 8) thin_tile<8, RM>(p, l, g0, lds, goff, soff, kxs);
+    else thin_tile<16, RM>(p, l, g0, lds, goff, soff, kxs);
 }
 
 __device__ __forceinline__ int64_t uniform64(int64_t v)
@@ -428,27 +457,36 @@ __device__ __forceinline__ int64_t uniform64(int64_t v)
 // XCDs round-robin, so a product's tiles share that XCD's L2 -- and then
 // takes what is left in the others, in order, so every workgroup ends after
 // every queue is empty.
-__global__ __launch_bounds__(256, TK_GROUP_WG_PER_CU) void k_group(const tk_prob* __restrict__ probs, const int64_t* __restrict__ tile_end,
-                                               const int* __restrict__ tile_prob, int nprobs, int64_t total,
-                                               const int64_t* __restrict__ qbeg, int nq,
-                                               unsigned long long* counter)
+// KIND (TK_GROUP_KIND_*) restricts the tiles a kernel can run, so the thin-
+// only kernel carries neither the MFMA tile's registers nor its LDS.
+template <int KIND>
+__global__ __launch_bounds__(256, KIND == TK_GROUP_KIND_THIN ? TK_GROUP_WG_PER_CU_THIN : TK_GROUP_WG_PER_CU)
+void k_group(const tk_prob* __restrict__ probs, const int64_t* __restrict__ tile_end,
+             const int* __restrict__ tile_prob, int nprobs, int64_t total, const int64_t* __restrict__ qbeg, int nq,
+             unsigned long long* counter)
 {
     __shared__ __attribute__((aligned(16))) float lds[KC * YP + SMAX * KC];
     __shared__ int64_t goff[GT];
+    __shared__ int64_t soff[2 * SMAX];
     __shared__ int64_t kox[KO], koy[KO];
+    __shared__ int64_t eoff[4 * MT];
     __shared__ int64_t next;
     const int home = (int)(blockIdx.x % (unsigned)nq);
     for (int qi = 0; qi < nq; ++qi) {
         const int q = (home + qi) % nq;
         const int64_t qb = qbeg ? qbeg[q] : 0, qe = qbeg ? qbeg[q + 1] : total;
-        for (;;) {
-            if (threadIdx.x == 0) next = qb + (int64_t)atomicAdd(counter + 32 * q, 1ull);
-            __syncthreads();
-            // uniform over the workgroup: said so, so the descriptor below is
-            // read with scalar loads into scalar registers
-            const int64_t t = uniform64(next);
-            __syncthreads();
-            if (t >= qe) break;
+        if (threadIdx.x == 0) next = qb + (int64_t)atomicAdd(counter + 32 * q, 1ull);
+        __syncthreads();
+        // uniform over the workgroup: said so, so the descriptor below is
+        // read with scalar loads into scalar registers
+        int64_t t = uniform64(next);
+        __syncthreads();
+        while (t < qe) {
+            // the next tile's index is taken now and kept in thread 0's
+            // registers until this tile is done: the atomic's round trip
+            // overlaps the tile's own loads instead of preceding them
+            unsigned long long ahead = 0;
+            if (threadIdx.x == 0) ahead = atomicAdd(counter + 32 * q, 1ull);
             int lo;
             if (tile_prob) {
                 lo = tile_prob[t];                       // direct map, one load
@@ -472,15 +510,19 @@ __global__ __launch_bounds__(256, TK_GROUP_WG_PER_CU) void k_group(const tk_prob
             const int64_t per_l = tiles_s * tiles_g;
             const int64_t l = local / per_l, rem = local % per_l;
             const int64_t ts = rem / tiles_g, tg = rem % tiles_g;
-            if (cls == TK_GROUP_THIN) {
-                if (p.round() == TK_MODE_BF16) thin_dispatch<TK_MODE_BF16>(p, l, tg * GT, lds, goff);
-                else if (p.round() == TK_MODE_F16) thin_dispatch<TK_MODE_F16>(p, l, tg * GT, lds, goff);
-                else thin_dispatch<0>(p, l, tg * GT, lds, goff);
+            if (KIND == TK_GROUP_KIND_THIN || (KIND == TK_GROUP_KIND_ALL && cls == TK_GROUP_THIN)) {
+                if (p.round() == TK_MODE_BF16) thin_dispatch<TK_MODE_BF16>(p, l, tg * GT, lds, goff, soff, kox);
+                else if (p.round() == TK_MODE_F16) thin_dispatch<TK_MODE_F16>(p, l, tg * GT, lds, goff, soff, kox);
+                else thin_dispatch<0>(p, l, tg * GT, lds, goff, soff, kox);
             } else {
-                if (p.round() == TK_MODE_BF16) mfma_tile<TK_MODE_BF16>(p, l, ts * MT, tg * MT, kox, koy);
-                else if (p.round() == TK_MODE_F16) mfma_tile<TK_MODE_F16>(p, l, ts * MT, tg * MT, kox, koy);
-                else mfma_tile<0>(p, l, ts * MT, tg * MT, kox, koy);
+                if (p.round() == TK_MODE_BF16) mfma_tile<TK_MODE_BF16>(p, l, ts * MT, tg * MT, kox, koy, eoff);
+                else if (p.round() == TK_MODE_F16) mfma_tile<TK_MODE_F16>(p, l, ts * MT, tg * MT, kox, koy, eoff);
+                else mfma_tile<0>(p, l, ts * MT, tg * MT, kox, koy, eoff);
             }
+            if (threadIdx.x == 0) next = qb + (int64_t)ahead;
+            __syncthreads();
+            t = uniform64(next);
+            __syncthreads();
         }
     }
 }
@@ -495,17 +537,38 @@ __global__ void k_zero_counters(unsigned long long* counter, int nq)
 
 using namespace tk;
 
-extern "C" int tk_launch_group_q(const tk_prob* probs, const int64_t* tile_end, const int* tile_prob, int nprobs,
-                                 int64_t total, const int64_t* qbeg, int nq, unsigned long long* counter, int nblocks,
-                                 void* stream)
+extern "C" int tk_launch_group_kind(int kind, const tk_prob* probs, const int64_t* tile_end, const int* tile_prob,
+                                    int nprobs, int64_t total, const int64_t* qbeg, int nq, unsigned long long* counter,
+                                    int nblocks, void* stream)
 {
     hipStream_t st = (hipStream_t)stream;
     if (nprobs <= 0 || total <= 0) return 0;
     if (nq < 1 || nq > 64 || (nq > 1 && !qbeg)) return (int)hipErrorInvalidValue;
+    const int64_t* q = nq > 1 ? qbeg : (const int64_t*)nullptr;
     hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, st, counter, nq);
-    hipLaunchKernelGGL(k_group, dim3((unsigned)nblocks), dim3(256), 0, st, probs, tile_end, tile_prob, nprobs, total,
-                       nq > 1 ? qbeg : (const int64_t*)nullptr, nq, counter);
+    switch (kind) {
+    case TK_GROUP_KIND_THIN:
+        hipLaunchKernelGGL(k_group<TK_GROUP_KIND_THIN>, dim3((unsigned)nblocks), dim3(256), 0, st, probs, tile_end,
+                           tile_prob, nprobs, total, q, nq, counter);
+        break;
+    case TK_GROUP_KIND_MFMA:
+        hipLaunchKernelGGL(k_group<TK_GROUP_KIND_MFMA>, dim3((unsigned)nblocks), dim3(256), 0, st, probs, tile_end,
+                           tile_prob, nprobs, total, q, nq, counter);
+        break;
+    default:
+        hipLaunchKernelGGL(k_group<TK_GROUP_KIND_ALL>, dim3((unsigned)nblocks), dim3(256), 0, st, probs, tile_end,
+                           tile_prob, nprobs, total, q, nq, counter);
+        break;
+    }
     return (int)hipGetLastError();
+}
+
+extern "C" int tk_launch_group_q(const tk_prob* probs, const int64_t* tile_end, const int* tile_prob, int nprobs,
+                                 int64_t total, const int64_t* qbeg, int nq, unsigned long long* counter, int nblocks,
+                                 void* stream)
+{
+    return tk_launch_group_kind(TK_GROUP_KIND_ALL, probs, tile_end, tile_prob, nprobs, total, qbeg, nq, counter,
+                                nblocks, stream);
 }
 
 extern "C" int tk_launch_group(const tk_prob* probs, const int64_t* tile_end, const int* tile_prob, int nprobs,

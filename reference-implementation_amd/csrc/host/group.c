@@ -22,18 +22,29 @@
 /* tile queues of a product group: one per XCD of an MI355X */
 #define GROUP_QUEUES 8
 
+/* One persistent launch of a group: the products of one tile class (their
+ * descriptors a contiguous run of the group's), with its own tile queues. */
+typedef struct {
+    int kind;                  /* TK_GROUP_KIND_THIN or TK_GROUP_KIND_MFMA */
+    int n;                     /* products */
+    int64_t total;             /* tiles */
+    int nblocks;
+    tk_prob* dprobs;
+    int64_t* dtile_end;        /* inclusive tile prefix over the launch's products */
+    int* dtile_prob;           /* tile -> product of the launch (or NULL) */
+    int64_t* dqbeg;            /* first tile of each queue, GROUP_QUEUES + 1 entries */
+    unsigned long long* dcounter;   /* one counter per queue, 256 bytes apart */
+} group_launch;
+
 struct tapp_group {
     uint32_t magic;
     struct tapp_exec* ex;
-    int n;                     /* products in the persistent launch */
-    int64_t total;             /* tiles */
-    int nblocks;
-    void* dmem;                /* tk_prob[n] | int64 tile_end[n] | counter */
-    tk_prob* dprobs;
-    int64_t* dtile_end;
-    int* dtile_prob;
-    int64_t* dqbeg;            /* first tile of each queue, GROUP_QUEUES + 1 entries */
-    unsigned long long* dcounter;   /* one counter per queue, 256 bytes apart */
+    int n;                     /* products in the persistent launches */
+    int64_t total;             /* their tiles */
+    void* dmem;                /* descriptors | per launch: tile ends, tile map, queue starts, counters */
+    tk_prob* dprobs;           /* all n descriptors: the thin launch's, then the MFMA launch's */
+    group_launch launch[2];
+    int nlaunch;
     hipGraph_t graph;
     hipGraphExec_t gexec;
     bool have_graph;
@@ -358,6 +369,129 @@ static int create_group(TAPP_AMD_product_group* group, TAPP_executor exec, int c
     return rc;
 }
 
+static int launch_all(struct tapp_group* g, hipStream_t st)
+{
+    for (int i = 0; i < g->nlaunch; i++) {
+        const group_launch* L = &g->launch[i];
+        int rc = tk_launch_group_kind(L->kind, L->dprobs, L->dtile_end, L->dtile_prob, L->n, L->total, L->dqbeg,
+                                      GROUP_QUEUES, L->dcounter, L->nblocks, st);
+        if (rc) return rc;
+    }
+    return 0;
+}
+
+/* Lays the ranked products (cost order) out on the device as up to two
+ * persistent launches -- thin tiles, then MFMA tiles, each kernel at its
+ * own occupancy -- and captures them in a graph.  Within a launch every
+ * product goes whole to the XCD queue with the least work so far, so its
+ * tiles share one L2; the queues are concatenated, each in cost order. */
+static int build_launches(struct tapp_group* g, struct tapp_exec* ex, const ranked* list, int n, bool capture,
+                          bool keep_host)
+{
+    const size_t pb = sizeof(tk_prob) * (size_t)n, tb = sizeof(int64_t) * (size_t)n;
+    tk_prob* hp = malloc(pb);
+    int64_t* he = malloc(tb);
+    int* bin = malloc(sizeof(int) * (size_t)n);
+    int* src = malloc(sizeof(int) * (size_t)n);
+    int* ord = malloc(sizeof(int) * (size_t)n);
+    int* hm = NULL;
+    int rc = 0;
+    if (!hp || !he || !bin || !src || !ord) { rc = TAPP_AMD_ERR_NOMEM; goto out; }
+    /* thin products first, then MFMA products, each run in cost order */
+    int nthin = 0;
+    for (int i = 0; i < n; i++) if (list[i].p.cls == TK_GROUP_THIN) ord[nthin++] = i;
+    for (int i = 0, k = nthin; i < n; i++) if (list[i].p.cls != TK_GROUP_THIN) ord[k++] = i;
+    const int first[3] = {0, nthin, n};
+    int64_t qbeg[2][GROUP_QUEUES + 1];
+    int64_t tot[2] = {0, 0};
+    for (int c = 0; c < 2; c++) {
+        double work[GROUP_QUEUES] = {0};
+        for (int i = first[c]; i < first[c + 1]; i++) {
+            const tk_prob* d = &list[ord[i]].p;
+            int q = 0;
+            for (int j = 1; j < GROUP_QUEUES; j++) if (work[j] < work[q]) q = j;
+            bin[i] = q;
+            work[q] += list[ord[i]].cost * (double)(d->L * d->tiles_s * d->tiles_g);
+        }
+        int64_t total = 0;
+        for (int q = 0, k = first[c]; q < GROUP_QUEUES; q++) {
+            qbeg[c][q] = total;
+            for (int i = first[c]; i < first[c + 1]; i++) {
+                if (bin[i] != q) continue;
+                hp[k] = list[ord[i]].p;
+                src[k] = list[ord[i]].idx;
+                total += hp[k].L * hp[k].tiles_s * hp[k].tiles_g;
+                he[k++] = total;
+            }
+        }
+        qbeg[c][GROUP_QUEUES] = total;
+        tot[c] = total;
+    }
+    g->total = tot[0] + tot[1];
+    /* tile -> product maps (int32), so a workgroup finds its product in one load */
+    const bool map = g->total <= INT32_MAX;
+    const size_t mb = map ? sizeof(int) * (size_t)g->total : 0;
+    if (map) {
+        if (!(hm = malloc(mb > 0 ? mb : 1))) { rc = TAPP_AMD_ERR_NOMEM; goto out; }
+        int64_t t = 0;
+        for (int c = 0; c < 2; c++)
+            for (int i = first[c], t0 = 0; i < first[c + 1]; i++)
+                for (; t0 < he[i]; t0++) hm[t++] = i - first[c];
+    }
+    const size_t o_end = (pb + 255) & ~(size_t)255, o_map = (o_end + tb + 255) & ~(size_t)255;
+    const size_t o_q = (o_map + mb + 255) & ~(size_t)255;
+    const size_t o_cnt = (o_q + sizeof qbeg + 255) & ~(size_t)255;
+    hipError_t e = hipMalloc(&g->dmem, o_cnt + 2 * 256 * GROUP_QUEUES);
+    if (e != hipSuccess) { (void)hipGetLastError(); g->dmem = NULL; rc = TAPP_AMD_ERR_NOMEM; goto out; }
+    char* base = (char*)g->dmem;
+    g->dprobs = (tk_prob*)base;
+    e = hipMemcpy(base, hp, pb, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(base + o_end, he, tb, hipMemcpyHostToDevice);
+    if (e == hipSuccess && map) e = hipMemcpy(base + o_map, hm, mb, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(base + o_q, qbeg, sizeof qbeg, hipMemcpyHostToDevice);
+    if (e != hipSuccess) { (void)hipGetLastError(); rc = TAPP_AMD_ERR_DEVICE; goto out; }
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ex->device) != hipSuccess || cus <= 0)
+        cus = 256;
+    g->nlaunch = 0;
+    for (int c = 0; c < 2; c++) {
+        if (first[c + 1] == first[c]) continue;
+        group_launch* L = &g->launch[g->nlaunch++];
+        L->kind = c == 0 ? TK_GROUP_KIND_THIN : TK_GROUP_KIND_MFMA;
+        L->n = first[c + 1] - first[c];
+        L->total = tot[c];
+        const int64_t nb = (int64_t)cus * (c == 0 ? TK_GROUP_WG_PER_CU_THIN : TK_GROUP_WG_PER_CU);
+        L->nblocks = (int)(tot[c] < nb ? tot[c] : nb);
+        L->dprobs = g->dprobs + first[c];
+        L->dtile_end = (int64_t*)(base + o_end) + first[c];
+        L->dtile_prob = map ? (int*)(base + o_map) + (c == 0 ? 0 : tot[0]) : NULL;
+        L->dqbeg = (int64_t*)(base + o_q) + c * (GROUP_QUEUES + 1);
+        L->dcounter = (unsigned long long*)(base + o_cnt + (size_t)c * 256 * GROUP_QUEUES);
+    }
+    if (keep_host) {
+        if (hipHostMalloc((void**)&g->hprobs, pb, hipHostMallocDefault) == hipSuccess) {
+            memcpy(g->hprobs, hp, pb);
+            g->hsrc = src;
+            src = NULL;
+        } else {
+            (void)hipGetLastError();
+            g->hprobs = NULL;
+        }
+    }
+    /* capture zero-counters + persistent kernels once; replays reuse them */
+    if (capture && hipStreamBeginCapture(ex->stream, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+        const int lrc = launch_all(g, ex->stream);
+        hipError_t ce = hipStreamEndCapture(ex->stream, &g->graph);
+        if (lrc == 0 && ce == hipSuccess && hipGraphInstantiate(&g->gexec, g->graph, NULL, NULL, 0) == hipSuccess)
+            g->have_graph = true;
+        else if (ce == hipSuccess && g->graph) hipGraphDestroy(g->graph);
+        (void)hipGetLastError();
+    }
+out:
+    free(hp); free(he); free(bin); free(src); free(ord); free(hm);
+    return rc;
+}
+
 static int create_group_on(TAPP_AMD_product_group* group, struct tapp_exec* ex, int count,
                            const TAPP_tensor_product* plans, const void* const* alpha, const void* const* A,
                            const void* const* B, const void* const* beta, const void* const* C, void* const* D,
@@ -412,86 +546,8 @@ static int create_group_on(TAPP_AMD_product_group* group, struct tapp_exec* ex, 
     qsort(list, (size_t)n, sizeof(ranked), by_cost);
     g->n = n;
     if (n > 0) {
-        size_t pb = sizeof(tk_prob) * (size_t)n, tb = sizeof(int64_t) * (size_t)n;
-        tk_prob* hp = malloc(pb);
-        int64_t* he = malloc(tb);
-        int* bin = malloc(sizeof(int) * (size_t)n);
-        int* src = malloc(sizeof(int) * (size_t)n);
-        if (!hp || !he || !bin || !src) {
-            free(hp); free(he); free(bin); free(src); free(list); group_free(g);
-            return TAPP_AMD_ERR_NOMEM;
-        }
-        /* one tile queue per XCD: each product goes whole to the queue with
-         * the least work so far (products in cost order), so its tiles share
-         * one L2; queues are concatenated, each in cost order */
-        double work[GROUP_QUEUES] = {0};
-        for (int i = 0; i < n; i++) {
-            int q = 0;
-            for (int j = 1; j < GROUP_QUEUES; j++) if (work[j] < work[q]) q = j;
-            bin[i] = q;
-            work[q] += list[i].cost * (double)(list[i].p.L * list[i].p.tiles_s * list[i].p.tiles_g);
-        }
-        int64_t total = 0, qbeg[GROUP_QUEUES + 1];
-        for (int q = 0, k = 0; q < GROUP_QUEUES; q++) {
-            qbeg[q] = total;
-            for (int i = 0; i < n; i++) {
-                if (bin[i] != q) continue;
-                hp[k] = list[i].p;
-                src[k] = list[i].idx;
-                total += hp[k].L * hp[k].tiles_s * hp[k].tiles_g;
-                he[k++] = total;
-            }
-        }
-        qbeg[GROUP_QUEUES] = total;
-        free(bin);
-        g->total = total;
-        /* tile -> product map (int32), so a workgroup finds its product in one load */
-        size_t mb = total <= INT32_MAX ? sizeof(int) * (size_t)total : 0;
-        int* hm = mb ? malloc(mb) : NULL;
-        if (hm)
-            for (int i = 0, t = 0; i < n; i++)
-                for (; t < he[i]; t++) hm[t] = i;
-        size_t o_end = (pb + 255) & ~(size_t)255, o_map = (o_end + tb + 255) & ~(size_t)255;
-        size_t o_q = (o_map + (hm ? mb : 0) + 255) & ~(size_t)255;
-        size_t o_cnt = (o_q + sizeof qbeg + 255) & ~(size_t)255;
-        hipError_t e = hipMalloc(&g->dmem, o_cnt + 256 * GROUP_QUEUES);
-        if (e != hipSuccess) { (void)hipGetLastError(); free(hp); free(he); free(hm); free(src); free(list); group_free(g); return TAPP_AMD_ERR_NOMEM; }
-        g->dprobs = (tk_prob*)g->dmem;
-        g->dtile_end = (int64_t*)((char*)g->dmem + o_end);
-        g->dtile_prob = hm ? (int*)((char*)g->dmem + o_map) : NULL;
-        g->dqbeg = (int64_t*)((char*)g->dmem + o_q);
-        g->dcounter = (unsigned long long*)((char*)g->dmem + o_cnt);
-        e = hipMemcpy(g->dprobs, hp, pb, hipMemcpyHostToDevice);
-        if (e == hipSuccess) e = hipMemcpy(g->dtile_end, he, tb, hipMemcpyHostToDevice);
-        if (e == hipSuccess && hm) e = hipMemcpy(g->dtile_prob, hm, mb, hipMemcpyHostToDevice);
-        if (e == hipSuccess) e = hipMemcpy(g->dqbeg, qbeg, sizeof qbeg, hipMemcpyHostToDevice);
-        if (e == hipSuccess && keep_host) {
-            if (hipHostMalloc((void**)&g->hprobs, pb, hipHostMallocDefault) == hipSuccess) {
-                memcpy(g->hprobs, hp, pb);
-                g->hsrc = src;
-                src = NULL;
-            } else {
-                (void)hipGetLastError();
-                g->hprobs = NULL;
-            }
-        }
-        free(hp); free(he); free(hm); free(src);
-        if (e != hipSuccess) { free(list); group_free(g); return TAPP_AMD_ERR_DEVICE; }
-        int cus = 0;
-        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ex->device) != hipSuccess || cus <= 0)
-            cus = 256;
-        int64_t nb = (int64_t)cus * TK_GROUP_WG_PER_CU;
-        g->nblocks = (int)(total < nb ? total : nb);
-        /* capture zero-counter + persistent kernel once; replays reuse it */
-        if (capture && hipStreamBeginCapture(ex->stream, hipStreamCaptureModeThreadLocal) == hipSuccess) {
-            int lrc = tk_launch_group_q(g->dprobs, g->dtile_end, g->dtile_prob, n, total, g->dqbeg, GROUP_QUEUES,
-                                        g->dcounter, g->nblocks, ex->stream);
-            hipError_t ce = hipStreamEndCapture(ex->stream, &g->graph);
-            if (lrc == 0 && ce == hipSuccess && hipGraphInstantiate(&g->gexec, g->graph, NULL, NULL, 0) == hipSuccess)
-                g->have_graph = true;
-            else if (ce == hipSuccess && g->graph) hipGraphDestroy(g->graph);
-            (void)hipGetLastError();
-        }
+        int rc = build_launches(g, ex, list, n, capture, keep_host);
+        if (rc) { free(list); group_free(g); return rc; }
     }
     free(list);
     *group = (TAPP_AMD_product_group)g;
@@ -736,8 +792,7 @@ static int execute_group(struct tapp_group* g)
     if (g->n > 0) {
         if (g->have_graph) {
             if (hipGraphLaunch(g->gexec, ex->stream) != hipSuccess) return TAPP_AMD_ERR_DEVICE;
-        } else if (tk_launch_group_q(g->dprobs, g->dtile_end, g->dtile_prob, g->n, g->total, g->dqbeg, GROUP_QUEUES,
-                                     g->dcounter, g->nblocks, ex->stream)) {
+        } else if (launch_all(g, ex->stream)) {
             return TAPP_AMD_ERR_DEVICE;
         }
     }

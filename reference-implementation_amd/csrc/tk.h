@@ -121,12 +121,18 @@ typedef struct tk_scatter {
  * group's problem order. */
 enum { TK_GROUP_THIN = 0, TK_GROUP_MFMA = 1 };
 
-/* Persistent group workgroups per CU: the group kernel's occupancy bound
- * (launch bounds) and the grid the host launches.  Three 256-lane
- * workgroups (24 KB of LDS, 168 VGPRs each): the MFMA tile's four 64 x 64
+/* Which tiles a persistent launch runs: both classes (the serving loop's
+ * batches), only thin tiles or only MFMA tiles (product groups launch the
+ * two classes as two kernels, each at its own occupancy). */
+enum { TK_GROUP_KIND_ALL = 0, TK_GROUP_KIND_THIN = 1, TK_GROUP_KIND_MFMA = 2 };
+/* Persistent group workgroups per CU: the kernel's occupancy bound (launch
+ * bounds) and the grid the host launches.  A kernel with MFMA tiles runs
+ * three 256-lane workgroups (168 VGPRs each): the MFMA tile's four 64 x 64
  * wave blocks hold 64 accumulators per lane beside two batches of loads in
- * flight, which four per CU (128 VGPRs) would spill. */
+ * flight, which four per CU (128 VGPRs) would spill.  The thin-only kernel
+ * fits 128 VGPRs and runs four, with more of the streamed Y in flight. */
 #define TK_GROUP_WG_PER_CU 3
+#define TK_GROUP_WG_PER_CU_THIN 4
 /* Edge of a TK_GROUP_MFMA output tile. */
 #define TK_GROUP_MT 128
 
@@ -171,6 +177,11 @@ int tk_launch_group(const tk_prob* probs, const int64_t* tile_end, const int* ti
 int tk_launch_group_q(const tk_prob* probs, const int64_t* tile_end, const int* tile_prob, int nprobs,
                       int64_t total, const int64_t* qbeg, int nq, unsigned long long* counter, int nblocks,
                       void* stream);
+/* Same, running only the tiles of `kind` (TK_GROUP_KIND_*): every product
+ * given must be of that class. */
+int tk_launch_group_kind(int kind, const tk_prob* probs, const int64_t* tile_end, const int* tile_prob, int nprobs,
+                         int64_t total, const int64_t* qbeg, int nq, unsigned long long* counter, int nblocks,
+                         void* stream);
 
 /* Element-wise comparison of two results of one product (self-checking
  * executor): x[i] and y[i] (storage type `type`, TAPP_datatype) for i < n.

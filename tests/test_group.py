@@ -328,3 +328,24 @@ def test_batched_product_reuse_follows_layout_change(lib, oracle):
     bs.check(oracle, want)
     bs.close()
     lib.destroy_executor(ex)
+
+
+def test_group_dense_thin_tiles_across_k_passes(lib, gpu_exec, oracle):
+    """Tucker products whose Y is dense run the one-sweep thin tile, K in
+    passes of 32: K = 1, 31, 32, 33 (a last pass of one row), 64, 65, 97,
+    with column counts across tile edges and 1 to 16 rows.  Each D equals
+    the oracle's and the single call's."""
+    shapes = [(1, 5, 7, 3), (31, 17, 16, 4), (32, 16, 16, 16), (33, 9, 30, 5), (64, 20, 13, 8), (65, 16, 17, 1),
+              (97, 7, 40, 12), (33, 1, 1, 2), (40, 300, 1, 16)]
+    cases = [T.tucker_case(500 + n, i, j, k, r) for n, (i, j, k, r) in enumerate(shapes)]
+    dcs = [DevCase(lib, c) for c in cases]
+    rc, g = make_group(lib, gpu_exec, dcs)
+    assert rc == 0
+    assert lib.group_info(g)[1][0] == len(dcs)
+    assert lib.execute_group(g) == 0
+    for d in dcs:
+        _, want = oracle.run(d.case, gpu=True)
+        assert_parity(d.d(), want, d.case)
+    assert lib.destroy_group(g) == 0
+    for d in dcs:
+        d.close()
