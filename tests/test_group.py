@@ -331,10 +331,10 @@ def test_batched_product_reuse_follows_layout_change(lib, oracle):
 
 
 def test_group_dense_thin_tiles_across_k_passes(lib, gpu_exec, oracle):
-    """Tucker products whose Y is dense run the one-sweep thin tile, K in
-    passes of 32: K = 1, 31, 32, 33 (a last pass of one row), 64, 65, 97,
-    with column counts across tile edges and 1 to 16 rows.  Each D equals
-    the oracle's and the single call's."""
+    """Tucker products on thin tiles across K chunk and pass edges: K = 1,
+    31, 32, 33 (one row past two 16-deep chunks), 64, 65, 97, with column
+    counts across the 256-column tile edge and 1 to 16 rows.  Each D equals
+    the oracle's."""
     shapes = [(1, 5, 7, 3), (31, 17, 16, 4), (32, 16, 16, 16), (33, 9, 30, 5), (64, 20, 13, 8), (65, 16, 17, 1),
               (97, 7, 40, 12), (33, 1, 1, 2), (40, 300, 1, 16)]
     cases = [T.tucker_case(500 + n, i, j, k, r) for n, (i, j, k, r) in enumerate(shapes)]
