@@ -86,10 +86,11 @@ class Product:
             self.lib.destroy_info(h)
 
 
-def time_form(lib, torch, prec, split, A, B, D, steps, warmup):
+def time_form(lib, torch, prec, split, A, B, D, steps, warmup, tiles=0):
     rc, ex = lib.create_executor()
     assert rc == 0
     lib.attr_set_int(ex, tc.ATTR_SPLIT_BF16, int(split))
+    lib.attr_set_int(ex, tc.ATTR_TILES, int(tiles))
     lib.attr_set_int(ex, tc.ATTR_TIMING, 1)
     p = Product(lib, prec)
     for _ in range(warmup):
@@ -779,6 +780,8 @@ def main():
                     help="only run a 1 GiB device copy (calibrates the FETCH_SIZE / WRITE_SIZE counters)")
     ap.add_argument("--only-form", choices=["fp32", "bf16", "f16", "split_bf16x3", "split_bf16x6", "split_f16x3"],
                     help="only time one form of the ladder (for profiler runs)")
+    ap.add_argument("--tiles", type=int, choices=[0, 1, 2], default=0,
+                    help="with --only-form: MFMA tile size (TAPP_AMD_ATTR_TILES: 0 by shape, 1 large, 2 small)")
     args = ap.parse_args()
     # autotuned measurements start from an empty cache on every run
     os.environ.setdefault("TAPP_AMD_TUNE_CACHE", os.path.join(tempfile.mkdtemp(prefix="tapp_bench_tune_"), "tune.txt"))
@@ -812,7 +815,7 @@ def main():
         prec, split = {"fp32": (tc.DEFAULT_PREC, 0), "bf16": (tc.BF16BF16_ACCUM_F32, 0),
                        "f16": (tc.F16F16_ACCUM_F32, 0), "split_bf16x3": (tc.DEFAULT_PREC, 1),
                        "split_bf16x6": (tc.DEFAULT_PREC, 2), "split_f16x3": (tc.DEFAULT_PREC, 3)}[args.only_form]
-        w, du, k = time_form(lib, torch, prec, split, A, B, D, args.steps, args.warmup)
+        w, du, k = time_form(lib, torch, prec, split, A, B, D, args.steps, args.warmup, args.tiles)
         print(json.dumps({"form": args.only_form, "kernel_us": du, "gflops": flops / w / 1e9, "kernel_family": k[0],
                           "kernel": k[1]}))
         return

@@ -103,7 +103,13 @@ enum {
     TAPP_AMD_ATTR_LAST_REUSED = 0x1009,
     /* Read-only: name of the kernel template the last execute launched
      * (const char*, owned by the executor, valid until its next call). */
-    TAPP_AMD_ATTR_LAST_KERNEL_NAME = 0x100A
+    TAPP_AMD_ATTR_LAST_KERNEL_NAME = 0x100A,
+    /* int: MFMA tile size of single products.  0 = by shape (default);
+     * 1 = large: FP32 split forms run 256 x 128 tiles (one workgroup per CU)
+     * where both operands load along a contiguous direction; F64 and complex
+     * run their large tiles; 2 = small: 128 x 128 split-form tiles, the
+     * small F64 / complex tiles.  Both sizes give the same bits. */
+    TAPP_AMD_ATTR_TILES = 0x100B
 };
 
 /* Kernel family identifiers reported by TAPP_AMD_ATTR_LAST_KERNEL. */

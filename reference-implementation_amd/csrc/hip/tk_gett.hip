@@ -1224,6 +1224,11 @@ extern "C" int tk_launch_gett(const tk_gett* gp, void* stream)
             hipLaunchKernelGGL((k_small_f32<16>), dim3((unsigned)tiles, (unsigned)ly), dim3(256), 0, st, g);
             return (int)hipGetLastError();
         }
+        if (g.tile_variant != 2 && tk_gett_wide_fits(&g)) {
+            snprintf(last_name, sizeof last_name, "k_gett_wide<%s, %s, %s>", load_name(g.a_load),
+                     load_name(g.b_load), mode_name(g.mode));
+            return tk_launch_gett_wide(&g, st);
+        }
         return launch_gett_f32(g, st);
     case TK_F64:
         if (kern == TK_KERNEL_SIMPLE) return launch_simple<double, false>(g, st);

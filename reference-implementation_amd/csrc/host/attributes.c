@@ -28,6 +28,12 @@ TAPP_error TAPP_attr_set(TAPP_attr attr, TAPP_key key, void* value)
         return 0;
     }
     case TAPP_AMD_ATTR_TIMING: e->timing = *(int*)value != 0; return 0;
+    case TAPP_AMD_ATTR_TILES: {
+        int t = *(int*)value;
+        if (t < 0 || t > 2) return TAPP_AMD_ERR_INVALID;
+        e->tiles = t;
+        return 0;
+    }
     case TAPP_AMD_ATTR_AUTOTUNE: {
         int a = *(int*)value;
         if (a < 0 || a > 2) return TAPP_AMD_ERR_INVALID;
@@ -60,6 +66,7 @@ TAPP_error TAPP_attr_get(TAPP_attr attr, TAPP_key key, void** value)
     case TAPP_AMD_ATTR_SPLIT_BF16: *(int*)value = e->split_bf16; return 0;
     case TAPP_AMD_ATTR_KERNEL: *(int*)value = e->force_kernel; return 0;
     case TAPP_AMD_ATTR_TIMING: *(int*)value = e->timing; return 0;
+    case TAPP_AMD_ATTR_TILES: *(int*)value = e->tiles; return 0;
     case TAPP_AMD_ATTR_AUTOTUNE: *(int*)value = e->autotune; return 0;
     case TAPP_AMD_ATTR_LAST_KERNEL: *(int*)value = e->last_kernel; return 0;
     case TAPP_AMD_ATTR_LAST_DEVICE_US: *(double*)value = e->last_device_us; return 0;
@@ -79,6 +86,7 @@ TAPP_error TAPP_attr_clear(TAPP_attr attr, TAPP_key key)
     case TAPP_AMD_ATTR_SPLIT_BF16: e->split_bf16 = 0; return 0;
     case TAPP_AMD_ATTR_KERNEL: e->force_kernel = 0; return 0;
     case TAPP_AMD_ATTR_TIMING: e->timing = 0; return 0;
+    case TAPP_AMD_ATTR_TILES: e->tiles = 0; return 0;
     case TAPP_AMD_ATTR_AUTOTUNE: e->autotune = 0; return 0;
     case TAPP_AMD_ATTR_SELF_CHECK: e->kind = 1; return 0;
     default: return TAPP_AMD_ERR_KEY;

@@ -350,12 +350,13 @@ static int launch_gett_ctx(void* c)
 static int tune_gett(struct tapp_exec* ex, tk_gett* g, int64_t ks, int64_t kchunk, float* partial, int64_t outs,
                      int* best)
 {
-    int cand[7], nc = 0;
+    int cand[8], nc = 0;
     const double macs = (double)outs * (double)g->K;
     if (macs <= (double)(1 << 30)) cand[nc++] = TK_KERNEL_SIMPLE;
     if (g->ctype == TK_F32) {
         cand[nc++] = TK_KERNEL_MFMA;
         if (ks > 1) cand[nc++] = TK_KERNEL_MFMA | (1 << 8);
+        if (tk_gett_wide_fits(g)) cand[nc++] = TK_KERNEL_MFMA | (2 << 4);
         const int64_t t16 = ((g->M + 15) / 16) * ((g->N + 15) / 16) * g->L;
         if (g->mode == TK_MODE_PLAIN && g->K <= 8192 && t16 <= 16 * (int64_t)ex->cus) cand[nc++] = TK_KERNEL_SMALL;
     } else {
@@ -887,6 +888,7 @@ static int execute_on(struct tapp_plan* p, struct tapp_exec* ex, const void* alp
 
     g.kernel = kern;
     g.cus = ex->cus;
+    g.tile_variant = ex->tiles;
 
     /* tile access patterns for the MFMA kernel */
     if (kern == TK_KERNEL_MFMA || may_tune) {

@@ -68,7 +68,8 @@ typedef struct tk_gett {
     int c_contig;
     int kernel;              /* chosen kernel variant, TK_KERNEL_* */
     int cus;                 /* compute units of the device (tile-size choice) */
-    int tile_variant;        /* F64/C32/C64 MFMA tiles: 0 by workgroup count, 1 large, 2 small */
+    int tile_variant;        /* MFMA tiles: 0 by shape, 1 large, 2 small (F64/C32/C64 tile size;
+                                FP32 split forms: 256 x 128 unless 2) */
     int ksplit;              /* > 1: K split over gridDim.z; partial sums go to `partial` */
     int64_t kchunk;          /* K length per split (multiple of the kernel's K tile) */
     float* partial;          /* [ksplit][L][N][M] FP32 partial sums (real compute only) */
@@ -162,6 +163,12 @@ typedef struct tk_prob {
 /* All launchers are asynchronous on `stream` (a hipStream_t) and return 0
  * or a hipError_t value. */
 int tk_launch_gett(const tk_gett* g, void* stream);
+/* FP32 data on 16-bit MFMA with 256 x 128 tiles (split forms, both
+ * operands loaded along a contiguous direction, no split-K, at least one
+ * tile per CU): whether g qualifies, and its launch.  tk_launch_gett takes
+ * it by itself unless g->tile_variant is 2. */
+int tk_gett_wide_fits(const tk_gett* g);
+int tk_launch_gett_wide(const tk_gett* g, void* stream);
 int tk_launch_prep(const tk_prep* p, void* stream);
 /* A's and B's pre-passes (same compute type) as one launch */
 int tk_launch_prep2(const tk_prep* a, const tk_prep* b, void* stream);

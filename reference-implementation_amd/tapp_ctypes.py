@@ -31,6 +31,7 @@ ATTR_AUTOTUNE = 0x1007
 ATTR_SELF_CHECK = 0x1008
 ATTR_LAST_REUSED = 0x1009
 ATTR_LAST_KERNEL_NAME = 0x100A
+ATTR_TILES = 0x100B
 # this library's error codes (tapp_amd.h)
 AMD_ERR_SCRATCH = 108
 AMD_ERR_UNSUPPORTED, AMD_ERR_DEVICE, AMD_ERR_NOMEM, AMD_ERR_INVALID, AMD_ERR_KEY, AMD_ERR_BUDGET, AMD_ERR_BUSY = \
