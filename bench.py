@@ -259,9 +259,11 @@ def cpu_reference_batch(sample=16):
             "serving_ladder64_sample": "D-slice i = j = 0 (1/1024 of D) at full K = 4096, one core"}
 
 
-def batching_bench(lib, torch, steps, prec=tc.DEFAULT_PREC):
+def batching_bench(lib, torch, steps, prec=tc.DEFAULT_PREC, full=True):
     """one persistent group launch vs one TAPP_execute_product per
-    product, same device operands, same GPU."""
+    product, same device operands, same GPU.  full=False (profiler runs):
+    only the group's launches, no one-call-per-product loop and no autotuned
+    group."""
     probs = batching_problems()
     L = lambda s: [ord(c) for c in s]  # noqa: E731
     specs, sizes = [], []
@@ -310,6 +312,25 @@ def batching_bench(lib, torch, steps, prec=tc.DEFAULT_PREC):
         scal.append((np.array([alpha], dtype=np.float32), np.array([beta], dtype=np.float32)))
     rc, ex = lib.create_executor()
     lib.attr_set_int(ex, tc.ATTR_TIMING, 1)
+    if not full:
+        rc, grp = lib.create_group(ex, plans, [sc[0].ctypes.data for sc in scal], [p[0] for p in ptrs],
+                                   [p[1] for p in ptrs], [sc[1].ctypes.data for sc in scal], [p[2] for p in ptrs],
+                                   [p[3] for p in ptrs])
+        assert rc == 0, lib.explain(rc)
+        assert lib.execute_group(grp) == 0
+        dev_us = []
+        for _ in range(steps):
+            assert lib.execute_group(grp) == 0
+            dev_us.append(lib.attr_get_double(ex, tc.ATTR_LAST_DEVICE_US)[1])
+        torch.cuda.synchronize()
+        lib.destroy_group(grp)
+        lib.destroy_executor(ex)
+        for plan, hs in zip(plans, infos):
+            lib.destroy_plan(plan)
+            for h in hs:
+                lib.destroy_info(h)
+        kernel_s = float(np.mean(dev_us)) * 1e-6
+        return {"group_kernel_ms": kernel_s * 1e3, "roofline_frac": ideal / kernel_s, "ideal_ms": ideal * 1e3}
     # one call per product
     for i in range(len(plans)):
         rc = lib.execute(plans[i], ex, scal[i][0].ctypes.data, ptrs[i][0], ptrs[i][1], scal[i][1].ctypes.data,
@@ -780,6 +801,8 @@ def main():
                     help="only run a 1 GiB device copy (calibrates the FETCH_SIZE / WRITE_SIZE counters)")
     ap.add_argument("--only-form", choices=["fp32", "bf16", "f16", "split_bf16x3", "split_bf16x6", "split_f16x3"],
                     help="only time one form of the ladder (for profiler runs)")
+    ap.add_argument("--only-batch", choices=["fp32", "bf16"],
+                    help="only time the uneven batch's group launches (for profiler runs)")
     ap.add_argument("--tiles", type=int, choices=[0, 1, 2], default=0,
                     help="with --only-form: MFMA tile size (TAPP_AMD_ATTR_TILES: 0 by shape, 1 large, 2 small)")
     args = ap.parse_args()
@@ -807,6 +830,10 @@ def main():
         print(json.dumps({"copy_bytes": src.numel() * 4, "dispatches": 3}))
         return
     lib = tc.TappLib()
+    if args.only_batch:
+        prec = tc.DEFAULT_PREC if args.only_batch == "fp32" else tc.BF16BF16_ACCUM_F32
+        print(json.dumps({"batch": args.only_batch, **batching_bench(lib, torch, args.steps, prec, full=False)}))
+        return
     flops = 2.0 * (V * V) * (O * O) * (V * V)
     log("bench: generating ladder operands")
     A, B, D = ladder_operands(torch)

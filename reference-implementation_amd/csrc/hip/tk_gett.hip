@@ -1225,8 +1225,8 @@ extern "C" int tk_launch_gett(const tk_gett* gp, void* stream)
             return (int)hipGetLastError();
         }
         if (g.tile_variant != 2 && tk_gett_wide_fits(&g)) {
-            snprintf(last_name, sizeof last_name, "k_gett_wide<%s, %s, %s>", load_name(g.a_load),
-                     load_name(g.b_load), mode_name(g.mode));
+            snprintf(last_name, sizeof last_name, "k_gett_wide<%s, %s, 128>", load_name(g.a_load),
+                     load_name(g.b_load));
             return tk_launch_gett_wide(&g, st);
         }
         return launch_gett_f32(g, st);

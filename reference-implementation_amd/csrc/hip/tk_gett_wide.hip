@@ -38,7 +38,7 @@ typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef _Float16 f16x4 __attribute__((ext_vector_type(4)));
 
-constexpr int WM = 256, WN = 128, WK = 32;
+constexpr int WM = 256, WK = 32;
 
 __device__ __forceinline__ f32x16 wmfma(u32x4 b, u32x4 a, f32x16 c)
 {
@@ -242,27 +242,27 @@ __device__ __forceinline__ void for_each_slot(F&& f, std::integer_sequence<int, 
     (f(IC<I>{}), ...);
 }
 
-// D tile of 256 x 128 = 2 x 2 waves of 128 x 64 (4 x 2 MFMA blocks of 32 x 32).
-// Tile kt's K step runs the MFMAs of LDS buffer kt % 2 while tile kt + 1's
-// raw data (one register stage per operand, loaded during the previous
-// step) is converted into LDS buffer (kt + 1) % 2 and tile kt + 2 is loaded
-// into the freed registers.  The step is 48 MFMA slots (three products of
-// the split pieces for each of two 16-k steps, eight blocks each) fenced by
-// sched_barrier: a wave cannot issue while its next MFMA waits for the
-// matrix core, so each slot pairs one MFMA with a piece of work sized to
-// run beside it:
-//   slots  0-7   B's conversion, a pair of values per slot, then B's loads
-//   slots  8-23  A's conversion, a pair of values per slot; in 16-23 also
-//                the second k step's operand reads
-//   slots 24-31  A's loads, one per slot; slot 32 the next K-table entries
-// so a load has about two thirds of a step to arrive.  Interior tiles run
-// steps in pairs without branches; edge tiles and the last steps run with
-// bounds checks.
-template <int AL, int BL>
+// D tile of 256 x WN = 2 x 2 waves of 128 x WN / 2 (4 x FN MFMA blocks of
+// 32 x 32, FN = WN / 64; launched with WN = 128: at WN = 256 the 256
+// accumulators per lane fill the accumulation registers and spill).  Tile kt's K step runs the MFMAs of LDS
+// buffer kt % 2 while tile kt + 1's raw data (one register stage per
+// operand, loaded during the previous step) is converted into LDS buffer
+// (kt + 1) % 2 and tile kt + 2 is loaded into the freed registers.  The step
+// is 6 NB MFMA slots (three products of the split pieces for each of two
+// 16-k steps, NB = 4 FN blocks each) fenced by sched_barrier: a wave cannot
+// issue while its next MFMA waits for the matrix core, so each slot pairs
+// one MFMA with a piece of work sized to run beside it: B's conversion a
+// pair of values per slot, then B's loads one per slot beside A's
+// conversion, the second k step's operand reads once the first's lo pieces
+// are used up, A's loads after its conversion, then the next K-table
+// entries.  A load has about two thirds of a step to arrive.  Interior tiles
+// run steps in pairs without branches; edge tiles and the last steps run
+// with bounds checks.
+template <int AL, int BL, int WN>
 __global__ __launch_bounds__(256, 1) void k_gett_wide(tk_gett g)
 {
     constexpr int P = 2;                               // 16-bit planes per operand
-    constexpr int FM = 4, FN = 2, S = 32;
+    constexpr int FM = 4, FN = WN / 64, S = 32;
     constexpr int AU = 4 * WM, BU = 4 * WN;            // 16-byte slots of an A / B plane
     // [buffer][A plane 0, A plane 1, B plane 0, B plane 1]
     __shared__ u32x4 lds[2][P * (AU + BU)];
@@ -280,8 +280,13 @@ __global__ __launch_bounds__(256, 1) void k_gett_wide(tk_gett g)
     using TA = WOp<AL, WM, 1>;
     using TB = WOp<BL, WN, 1>;
     constexpr int NUA = TA::NU, NUB = TB::NU;         // conversion units per tile
-    constexpr int PA = NUA * TA::NP, PB = NUB * TB::NP;   // pairs per tile: 16 and 8
-    static_assert(PA == 16 && PB == 8, "slot plan");
+    constexpr int PA = NUA * TA::NP, PB = NUB * TB::NP;   // conversion pairs per tile
+    // the slot plan: B pairs [0, PB), B loads [PB, PB + NV_B), A pairs
+    // [PB, PB + PA), A loads [LA, LA + NV_A), the K-table after them, the
+    // second k step's operand reads two per slot from RD (the first k step's
+    // lo pieces are used up by then)
+    constexpr int NB = FN * FM, NS = 6 * NB, LA = PB + PA, RD = 2 * NB, NRD = 2 * (FN + FM);
+    static_assert(LA + TA::NV < NS && PB + TB::NV <= LA + TA::NV && RD + NRD / 2 <= NS, "slot plan");
     TA ta;
     TB tb;
     ta.rows(g.mA, m0, g.M);
@@ -303,34 +308,34 @@ __global__ __launch_bounds__(256, 1) void k_gett_wide(tk_gett g)
 #pragma unroll
             for (int fm = 0; fm < FM; ++fm) acc[fn][fm] = f32x16{};
 
-        auto ktab = [&](auto CHK, int64_t t) {
+        auto ktab = [&](auto CHK, int64_t t) __attribute__((always_inline)) {
             ta.template ktab<decltype(CHK)::value>(g.kA, t * WK, g.K);
             tb.template ktab<decltype(CHK)::value>(g.kB, t * WK, g.K);
         };
         // pair i of A's (B's) conversion into LDS buffer buf; the unit is
         // stored after its last pair
-        auto a_pair = [&](int buf, int i) {
+        auto a_pair = [&](int buf, int i) __attribute__((always_inline)) {
             const int u = i / TA::NP, p = i % TA::NP;
             ta.pair(0, u, p);
             if (p == TA::NP - 1) ta.put(&lds[buf][0], AU, sa[u]);
         };
-        auto b_pair = [&](int buf, int i) {
+        auto b_pair = [&](int buf, int i) __attribute__((always_inline)) {
             const int u = i / TB::NP, p = i % TB::NP;
             tb.pair(0, u, p);
             if (p == TB::NP - 1) tb.put(&lds[buf][P * AU], BU, sb[u]);
         };
         // operands of k step kk (0 or 16) of LDS buffer `cur`: lanes 0-31
         // give k kk .. kk + 7, lanes 32-63 k kk + 8 .. kk + 15
-        auto operand_b = [&](int cur, int kk, int q, int fn) {
+        auto operand_b = [&](int cur, int kk, int q, int fn) __attribute__((always_inline)) {
             return lds[cur][P * AU + q * BU + rb[fn & 1] + (kk >> 3) * WN + fn * S];
         };
-        auto operand_a = [&](int cur, int kk, int q, int fm) {
+        auto operand_a = [&](int cur, int kk, int q, int fm) __attribute__((always_inline)) {
             return lds[cur][q * AU + ra[fm & 1] + (kk >> 3) * WM + fm * S];
         };
 
         // one K step; PAR = kt % 2; CHECK: bounds-checked loads and guarded
         // conversion
-        auto step = [&](auto PAR_, auto CHK, int64_t kt) {
+        auto step = [&](auto PAR_, auto CHK, int64_t kt) __attribute__((always_inline)) {
             constexpr int PAR = decltype(PAR_)::value;
             constexpr bool CHECK = decltype(CHK)::value;
             // [k step][plane][block]; the split pieces' products, smallest
@@ -346,42 +351,38 @@ __global__ __launch_bounds__(256, 1) void k_gett_wide(tk_gett g)
 #pragma unroll
                 for (int fm = 0; fm < FM; ++fm) a[0][q][fm] = operand_a(PAR, 0, q, fm);
             }
+            // read r of the second k step, in the order its products use
+            // them: B lo, A hi, B hi, A lo
+            auto read2 = [&](auto R) __attribute__((always_inline)) {
+                constexpr int r = decltype(R)::value;
+                if constexpr (r < FN) b[1][1][r] = operand_b(PAR, 16, 1, r);
+                else if constexpr (r < FN + FM) a[1][0][r - FN] = operand_a(PAR, 16, 0, r - FN);
+                else if constexpr (r < 2 * FN + FM) b[1][0][r - FN - FM] = operand_b(PAR, 16, 0, r - FN - FM);
+                else a[1][1][r - 2 * FN - FM] = operand_a(PAR, 16, 1, r - 2 * FN - FM);
+            };
             __builtin_amdgcn_sched_barrier(0);
-            for_each_slot([&](auto SLOT) {
+            for_each_slot([&](auto SLOT) __attribute__((always_inline)) {
                 constexpr int slot = decltype(SLOT)::value;
-                constexpr int h = slot / 24, pr = (slot % 24) / 8, fn = (slot % 8) / FM, fm = slot % FM;
+                constexpr int h = slot / (3 * NB), pr = (slot % (3 * NB)) / NB;
+                constexpr int fn = (slot % NB) / FM, fm = slot % FM;
                 acc[fn][fm] = wmfma(b[h][PBQ[pr]][fn], a[h][PAQ[pr]][fm], acc[fn][fm]);
-                if (slot < PB) {
+                if constexpr (slot < PB) {
                     if (next) b_pair(PAR ^ 1, slot);
-                } else if (slot < PB + PA) {
+                } else if constexpr (slot < PB + PA) {
                     if (next) a_pair(PAR ^ 1, slot - PB);
                 }
-                if (slot >= PB && slot < PB + TB::NV && next2)
-                    tb.template load<CHECK>(0, slot - PB, Bb, n0, (kt + 2) * WK, g.N, g.K);
-                if (slot >= 16 && slot < 22) {
-                    // the second k step's operands, two reads per slot in
-                    // the order its products use them (the first k step's
-                    // lo pieces are dead after slot 15)
-                    const int i = slot - 16;
-                    if (i == 0) {
-                        b[1][1][0] = operand_b(PAR, 16, 1, 0);
-                        b[1][1][1] = operand_b(PAR, 16, 1, 1);
-                    } else if (i < 3) {
-                        a[1][0][2 * i - 2] = operand_a(PAR, 16, 0, 2 * i - 2);
-                        a[1][0][2 * i - 1] = operand_a(PAR, 16, 0, 2 * i - 1);
-                    } else if (i == 3) {
-                        b[1][0][0] = operand_b(PAR, 16, 0, 0);
-                        b[1][0][1] = operand_b(PAR, 16, 0, 1);
-                    } else {
-                        a[1][1][2 * i - 8] = operand_a(PAR, 16, 1, 2 * i - 8);
-                        a[1][1][2 * i - 7] = operand_a(PAR, 16, 1, 2 * i - 7);
-                    }
+                if constexpr (slot >= PB && slot < PB + TB::NV)
+                    if (next2) tb.template load<CHECK>(0, slot - PB, Bb, n0, (kt + 2) * WK, g.N, g.K);
+                if constexpr (slot >= RD && slot < RD + NRD / 2) {
+                    read2(IC<2 * (slot - RD)>{});
+                    read2(IC<2 * (slot - RD) + 1>{});
                 }
-                if (slot >= 24 && slot < 24 + TA::NV && next2)
-                    ta.template load<CHECK>(0, slot - 24, Ab, m0, (kt + 2) * WK, g.M, g.K);
-                if (slot == 24 + TA::NV && (!CHECK || kt + 3 < nk)) ktab(CHK, kt + 3);
+                if constexpr (slot >= LA && slot < LA + TA::NV)
+                    if (next2) ta.template load<CHECK>(0, slot - LA, Ab, m0, (kt + 2) * WK, g.M, g.K);
+                if constexpr (slot == LA + TA::NV)
+                    if (!CHECK || kt + 3 < nk) ktab(CHK, kt + 3);
                 __builtin_amdgcn_sched_barrier(0);
-            }, std::make_integer_sequence<int, 48>{});
+            }, std::make_integer_sequence<int, NS>{});
             __syncthreads();
         };
 
@@ -456,12 +457,16 @@ __global__ __launch_bounds__(256, 1) void k_gett_wide(tk_gett g)
     }
 }
 
+__host__ __device__ inline int64_t wide_tiles(const tk_gett& g, int wn)
+{
+    return ((g.M + WM - 1) / WM) * ((g.N + wn - 1) / wn) * g.L;
+}
+
 template <int AL, int BL>
 hipError_t launch(const tk_gett& g, hipStream_t st)
 {
-    const int64_t tiles = ((g.M + WM - 1) / WM) * ((g.N + WN - 1) / WN);
-    const dim3 grid((unsigned)tiles, (unsigned)(g.L < 65535 ? g.L : 65535)), block(256);
-    hipLaunchKernelGGL((k_gett_wide<AL, BL>), grid, block, 0, st, g);
+    const dim3 grid((unsigned)(wide_tiles(g, 128) / g.L), (unsigned)(g.L < 65535 ? g.L : 65535)), block(256);
+    hipLaunchKernelGGL((k_gett_wide<AL, BL, 128>), grid, block, 0, st, g);
     return hipGetLastError();
 }
 
@@ -477,8 +482,7 @@ extern "C" int tk_gett_wide_fits(const tk_gett* g)
 {
     if (g->ctype != TK_F32 || g->mode != TK_MODE_SPLIT_BF16 || !g->a_vec || !g->b_vec || g->ksplit > 1) return 0;
     if (g->a_load == TK_GATHER || g->b_load == TK_GATHER) return 0;
-    const int64_t tiles = ((g->M + WM - 1) / WM) * ((g->N + WN - 1) / WN) * g->L;
-    return tiles >= (int64_t)(g->cus > 0 ? g->cus : 256);
+    return wide_tiles(*g, 128) >= (int64_t)(g->cus > 0 ? g->cus : 256);
 }
 
 extern "C" int tk_launch_gett_wide(const tk_gett* gp, void* stream)

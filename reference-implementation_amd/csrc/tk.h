@@ -163,10 +163,10 @@ typedef struct tk_prob {
 /* All launchers are asynchronous on `stream` (a hipStream_t) and return 0
  * or a hipError_t value. */
 int tk_launch_gett(const tk_gett* g, void* stream);
-/* FP32 data on 16-bit MFMA with 256 x 128 tiles (split forms, both
- * operands loaded along a contiguous direction, no split-K, at least one
- * tile per CU): whether g qualifies, and its launch.  tk_launch_gett takes
- * it by itself unless g->tile_variant is 2. */
+/* Split-BF16 with 256 x 128 tiles (both operands loaded along a contiguous
+ * direction, no split-K, at least one tile per CU): whether g qualifies,
+ * and its launch.  tk_launch_gett takes it by itself unless
+ * g->tile_variant is 2. */
 int tk_gett_wide_fits(const tk_gett* g);
 int tk_launch_gett_wide(const tk_gett* g, void* stream);
 int tk_launch_prep(const tk_prep* p, void* stream);
