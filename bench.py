@@ -517,11 +517,13 @@ def batched_product_bench(lib, torch, steps, count=8192, shape=(32, 32, 32, 8)):
     return out
 
 
-def serving_bench(lib, torch, rates, nreq, split=False, seed=3, budget=64 << 30, sequential=False):
+def serving_bench(lib, torch, rates, nreq, split=False, seed=3, budget=64 << 30, sequential=False, small_only=False):
     """open-loop Poisson arrivals into the serving loop.
     Requests: 5 % large ladder-form products (virtual extent 64:
     D[a,b,i,j] = sum A[a,b,c,d] B[c,i,d,j], a..d = 64, i,j = 32), 20 %
     configs[0] products (extents 16), 75 % drawn from 256 configs[3]-mix shapes.
+    With `small_only` the large class is left out (configs[0] and the mix
+    shapes keep their 20 : 75 proportion).
     70 % of requests pass a host A that is resident on the plans' handle
     (TAPP_AMD_make_resident: read from its HBM copy, no transfer); 30 % pass
     another pinned host A that the server stages.  B, C and D are device
@@ -590,6 +592,8 @@ def serving_bench(lib, torch, rates, nreq, split=False, seed=3, budget=64 << 30,
 
     def draw(n):
         u = rng.random(n)
+        if small_only:
+            u = 0.05 + 0.95 * u
         idx = np.where(u < 0.05, 0, np.where(u < 0.25, 1, 2 + rng.integers(0, 256, n)))
         host_a = rng.random(n) >= 0.7
         return idx, host_a
@@ -640,10 +644,22 @@ def serving_bench(lib, torch, rates, nreq, split=False, seed=3, budget=64 << 30,
             seq_ex = lib.create_executor()[1]
             f, r, el = tc.trace_sequential(lib, seq_ex, plans, als, As, Bs, bes, Cs, Ds)
             assert f == 0, [x for x in r if x][:5]
-            lib.destroy_executor(seq_ex)
             row["one_call_at_a_time"] = {"wall_s": el, "req_s": nreq / el, "gflops": flop / el / 1e9}
             row["served_vs_one_call_at_a_time_req_s"] = (nreq / wall) / (nreq / el)
             row["served_vs_one_call_at_a_time_gflops"] = st["gflops"] / (flop / el / 1e9)
+            large = [i for i, c in enumerate(idx) if c == 0]
+            if large:
+                # the large requests' share of the one-call-at-a-time time:
+                # served throughput beats one call at a time by about
+                # 1 / share at most, unless serving also runs the large
+                # requests faster (two of them overlapping on the device)
+                pick = lambda v: [v[i] for i in large]  # noqa: E731
+                f, r, el_large = tc.trace_sequential(lib, seq_ex, pick(plans), pick(als), pick(As), pick(Bs),
+                                                     pick(bes), pick(Cs), pick(Ds))
+                assert f == 0, [x for x in r if x][:5]
+                row["one_call_at_a_time"]["large_requests_wall_s"] = el_large
+                row["one_call_at_a_time"]["large_requests_share"] = el_large / el
+            lib.destroy_executor(seq_ex)
         results.append(row)
         log(f"bench: serving {'split-BF16' if split else 'FP32'} {rate} req/s: p50 {st['p50_us']:.0f} us "
             f"p99 {st['p99_us']:.0f} us, {st['gflops']:.0f} GFLOP/s, stalls {st['stalls']:.0f}")
@@ -803,6 +819,8 @@ def main():
                     help="only time one form of the ladder (for profiler runs)")
     ap.add_argument("--only-batch", choices=["fp32", "bf16"],
                     help="only time the uneven batch's group launches (for profiler runs)")
+    ap.add_argument("--only-serve", action="store_true",
+                    help="only run the serving sweeps (full mix with one call at a time, small requests)")
     ap.add_argument("--tiles", type=int, choices=[0, 1, 2], default=0,
                     help="with --only-form: MFMA tile size (TAPP_AMD_ATTR_TILES: 0 by shape, 1 large, 2 small)")
     args = ap.parse_args()
@@ -833,6 +851,12 @@ def main():
     if args.only_batch:
         prec = tc.DEFAULT_PREC if args.only_batch == "fp32" else tc.BF16BF16_ACCUM_F32
         print(json.dumps({"batch": args.only_batch, **batching_bench(lib, torch, args.steps, prec, full=False)}))
+        return
+    if args.only_serve:
+        out = {"fp32": serving_bench(lib, torch, [50000], args.serve_requests, sequential=True),
+               "fp32_small_requests": serving_bench(lib, torch, [500000], args.serve_requests, sequential=True,
+                                                    small_only=True)}
+        print(json.dumps({"serving": out}))
         return
     flops = 2.0 * (V * V) * (O * O) * (V * V)
     log("bench: generating ladder operands")
@@ -921,6 +945,9 @@ def main():
                                     "the workload's full trace is 200000 requests)")
         serving["fp32"] = serving_bench(lib, torch, rates, args.serve_requests, sequential=True)
         serving["split_bf16"] = serving_bench(lib, torch, [5000], args.serve_requests, split=True)
+        log("bench: serving loop, the mix without its large ladder-form class")
+        serving["fp32_small_requests"] = serving_bench(lib, torch, [500000], args.serve_requests, sequential=True,
+                                                       small_only=True)
         log("bench: serving loop with a 128 MiB device-memory budget")
         serving["fp32_budget_128MiB"] = serving_bench(lib, torch, [20000], args.serve_requests, budget=128 << 20)
 

@@ -434,10 +434,84 @@ __device__ __forceinline__ void mfma_tile(const GView& p, int64_t l, int64_t m0,
     }
 }
 
+// Thin tile of a product whose tables are all index * stride, over a dense
+// Y (element (k, g) at g K + k): thin_tile's data flow with every offset
+// computed from the descriptor's strides, so the tile's first loads are its
+// data.  Y's chunk loads are issued one chunk ahead, as in thin_tile.
+template <int SB, int RM>
+__device__ __forceinline__ void thin_affine(const GView& p, int64_t l, int64_t g0, float* lds)
+{
+    constexpr int NY = KC * GT / 256;           // Y elements per lane and chunk
+    float* Yl = lds;               // [KC][YP]
+    float* Xl = lds + KC * YP;     // [SMAX][KC]
+    const int tid = threadIdx.x;
+    const tk_prob TK_CONST* d = p.d;
+    const int64_t K = p.K(), S = p.S();
+    float acc[SB];
+#pragma unroll
+    for (int s = 0; s < SB; ++s) acc[s] = 0.0f;
+    const int gn = (int)(p.G() - g0 < GT ? p.G() - g0 : GT);
+    const int xs = tid / KC, xk = tid % KC;
+    const bool x_row = tid < SB * KC && xs < S;
+    const int64_t skX = d->skX;
+    gf xb = p.X() + l * d->slX + (int64_t)(x_row ? xs : 0) * d->ssX;
+    // 32-bit offsets within the tile's block: the host takes the dense path
+    // only when K < 2^21
+    gf yb0 = p.Y() + l * d->slY + g0 * K;
+    float yv[NY], xv = 0.0f;
+    auto fetch = [&](int64_t k0) {
+        const int kc = (int)(K - k0 < KC ? K - k0 : KC);
+        const uint32_t kk = (uint32_t)(xk < kc ? xk : 0);
+        xv = xb[(k0 + kk) * skX];
+        gf yb = yb0 + k0;
+#pragma unroll
+        for (int i = 0; i < NY; ++i) {
+            const int gg = tid / KC + (256 / KC) * i;
+            yv[i] = yb[(uint32_t)(gg < gn ? gg : 0) * (uint32_t)K + kk];
+        }
+    };
+    if (K > 0) fetch(0);
+    for (int64_t k0 = 0; k0 < K; k0 += KC) {
+        const int kc = (int)(K - k0 < KC ? K - k0 : KC);
+        __syncthreads();                          // the previous chunk's LDS reads are done
+        if (tid < SB * KC) Xl[tid] = (x_row && xk < kc) ? rnd_m<RM>(xv) : 0.0f;
+#pragma unroll
+        for (int i = 0; i < NY; ++i) {
+            const int gg = tid / KC + (256 / KC) * i;
+            Yl[xk * YP + gg] = (xk < kc && gg < gn) ? rnd_m<RM>(yv[i]) : 0.0f;
+        }
+        __syncthreads();
+        if (k0 + KC < K) fetch(k0 + KC);
+        for (int k = 0; k < kc; ++k) {
+            const float y = Yl[k * YP + tid];
+#pragma unroll
+            for (int s = 0; s < SB; ++s) acc[s] += Xl[s * KC + k] * y;
+        }
+    }
+    if (tid < gn) {
+        const int64_t g = g0 + tid;
+        const int64_t dcol = l * d->slD + g * d->sgD;
+        const int64_t ccol = p.beta_zero() ? 0 : l * d->slC + g * d->sgC;
+        const int64_t ssD = d->ssD, ssC = d->ssC;
+        float cv[SB];
+#pragma unroll
+        for (int s = 0; s < SB; ++s) cv[s] = s < S ? group_c(p, ccol + s * ssC) : 0.0f;
+#pragma unroll
+        for (int s = 0; s < SB; ++s)
+            if (s < S) group_store(p, dcol + s * ssD, acc[s], cv[s]);
+    }
+}
+
 template <int RM>
 __device__ __forceinline__ void thin_dispatch(const GView& p, int64_t l, int64_t g0, float* lds, int64_t* goff,
                                               int64_t* soff, int64_t* kxs)
 {
+    if (p.d->affine) {
+        if (p.S() <= 4) thin_affine<4, RM>(p, l, g0, lds);
+        else if (p.S() <= 8) thin_affine<8, RM>(p, l, g0, lds);
+        else thin_affine<16, RM>(p, l, g0, lds);
+        return;
+    }
     if (p.S() <= 4) thin_tile<4, RM>(p, l, g0, lds, goff, soff, kxs);
     else if (p.S() <= 8) thin_tile<8, RM>(p, l, g0, lds, goff, soff, kxs);
     else thin_tile<16, RM>(p, l, g0, lds, goff, soff, kxs);

@@ -77,6 +77,39 @@ static int y_access(const int64_t* kt, int64_t K, const int64_t* gt, int64_t G)
     return kt[0] == 0 ? 2 : 1;
 }
 
+/* Whether operand x's offset table over group g (first mode fastest) is
+ * index * stride: each mode's stride is the previous one times its extent.
+ * The stride goes to *st (0 for an empty group). */
+static bool affine_group(const tk_group* g, int x, int64_t* st)
+{
+    *st = g->n > 0 ? g->s[x][0] : 0;
+    for (int i = 1; i < g->n; i++)
+        if (g->s[x][i] != g->s[x][i - 1] * g->ext[i - 1]) return false;
+    return true;
+}
+
+/* Thin tiles over a dense Y: the strides that replace X's, D's and C's
+ * tables and the Hadamard tables when all of them are affine. */
+static void describe_affine(const tk_problem* P, bool x_is_b, tk_prob* out)
+{
+    out->affine = 0;
+    if (out->cls != TK_GROUP_THIN || out->y_kfast != 2) return;
+    const int xi = x_is_b ? 1 : 0, yi = 1 - xi;
+    const tk_group* gs = x_is_b ? &P->N : &P->M;     /* X's rows: D's s */
+    const tk_group* gg = x_is_b ? &P->M : &P->N;     /* Y's columns: D's g */
+    int64_t v[10] = {0};
+    bool ok = affine_group(gs, xi, &v[0]) && affine_group(&P->K, xi, &v[1]) && affine_group(&P->L, xi, &v[2]) &&
+              affine_group(&P->L, yi, &v[3]) && affine_group(gs, 3, &v[4]) && affine_group(gg, 3, &v[5]) &&
+              affine_group(&P->L, 3, &v[6]);
+    if (!P->beta_zero)
+        ok = ok && affine_group(gs, 2, &v[7]) && affine_group(gg, 2, &v[8]) && affine_group(&P->L, 2, &v[9]);
+    if (!ok) return;
+    out->ssX = v[0]; out->skX = v[1]; out->slX = v[2]; out->slY = v[3];
+    out->ssD = v[4]; out->sgD = v[5]; out->slD = v[6];
+    out->ssC = v[7]; out->sgC = v[8]; out->slC = v[9];
+    out->affine = 1;
+}
+
 typedef struct { tk_prob p; double cost; int idx; } ranked;
 
 static int by_cost(const void* x, const void* y)
@@ -164,6 +197,7 @@ bool tk_group_describe(struct tapp_plan* plan, struct tapp_exec* ex, const tk_pr
     out->round_C = P->round_C;
     if (out->cls == TK_GROUP_THIN) { out->tiles_s = 1; out->tiles_g = (out->G + 255) / 256; }
     else { out->tiles_s = (out->S + TK_GROUP_MT - 1) / TK_GROUP_MT; out->tiles_g = (out->G + TK_GROUP_MT - 1) / TK_GROUP_MT; }
+    describe_affine(P, x_is_b, out);
     return true;
 }
 

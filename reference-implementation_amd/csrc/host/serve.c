@@ -9,11 +9,13 @@
  * the device memory they need fits the HBM budget, launches them as one batch on
  * that slot's stream and goes on: requests arriving while batches run are
  * admitted as soon as a slot and budget free up, and batches on different
- * slots overlap on the device.  Inside a batch, FP32 products on device
- * operands that fit the persistent kernel (small and thin products) run as
+ * slots overlap on the device.  Inside a batch, FP32 products that fit the
+ * persistent kernel (small and thin products, D in device memory) run as
  * one persistent group launch -- a work queue over all their tiles, mixed
- * shape classes in one kernel -- and every other request (large GETTs,
- * complex, FP64, split-BF16, unary labels, host operands) runs through the
+ * shape classes in one kernel.  Their host operands that are not resident
+ * (up to heavy_bytes per request) are first copied into the batch's
+ * workspace on the slot's stream.  Every other request (large GETTs,
+ * complex, FP64, split-BF16, unary labels, host D) runs through the
  * TAPP_execute_product path on the same stream without waiting.  The
  * persistent launch goes first and has its own HIP event, so its (small)
  * requests complete when it ends, not behind the batch's large products;
@@ -26,9 +28,12 @@
  * requests never wait for a stream behind heavy ones.
  *
  * Device memory is accounted as the server holds it:
- *   held = the slots' workspaces (each sized to the largest general-path
- *          request of its batch: staged host operands, pre-pass and
- *          conversion buffers, split-K partial sums; kept between batches)
+ *   held = the slots' workspaces (each sized to the larger of its batch's
+ *          largest general-path request -- staged host operands, pre-pass
+ *          and conversion buffers, split-K partial sums -- and the sum of
+ *          the host operands its persistent launch stages: the general path
+ *          runs after that launch on the same stream, so both reuse one
+ *          workspace; kept between batches)
  *        + offset tables that the batches in flight build for their plans
  *          (they stay with the plan afterwards, outside the budget).
  * The persistent launch's descriptor buffers (232 bytes per request of a
@@ -36,8 +41,9 @@
  * A batch is admitted while held + its growth fits the budget; when the
  * first request does not fit, the workspaces of idle slots are freed and
  * the check repeated, and if it still does not fit the request waits (a
- * stall).  Requests that run in the persistent launch need no workspace;
- * device operands and resident tensors cost nothing.  A request whose own
+ * stall).  Requests that run in the persistent launch need workspace only
+ * for their staged host operands; device operands and resident tensors
+ * cost nothing.  A request whose own
  * workspace and tables exceed the whole budget is rejected at submit
  * (TAPP_AMD_ERR_BUDGET).
  *
@@ -68,6 +74,7 @@ typedef struct srv_request {
     const void *A, *B, *C;
     void* D;
     size_t work;                /* workspace of a general-path run (0 for the persistent launch) */
+    size_t stage;               /* host operands the persistent launch stages (256-aligned spans) */
     size_t tables;              /* offset tables the plan still has to build on the device */
     double t_submit, t_start, t_done;
     int in_group;               /* in the batch: 1 persistent launch, 0 general path, -1 nothing to launch */
@@ -161,6 +168,52 @@ static int ensure_desc(srv_slot* s, size_t n)
     return 0;
 }
 
+#define SRV_ALIGN256(x) (((x) + 255) & ~(size_t)255)
+
+/* Bytes the persistent launch stages for a product it can run: the spans
+ * of A, B and C (C only when beta is not 0) that are host memory and not
+ * resident on the plan's handle, each 256-aligned.  -1 when the product
+ * cannot join the launch (the kernel does not run it, or D is host memory:
+ * the launch writes D in place). */
+static int64_t group_stage_bytes(struct tapp_plan* p, const tk_problem* P, const void* A, const void* B,
+                                 const void* C, void* D)
+{
+    if (!tk_group_may_run(P) || !tk_is_device_ptr(D) || !A || !B) return -1;
+    const void* ops[3] = {A, B, P->beta_zero ? NULL : C};
+    int64_t total = 0;
+    for (int x = 0; x < 3; x++) {
+        if (!ops[x] || P->hi[x] < P->lo[x]) continue;
+        if (tk_resolve_operand(p->handle, ops[x], P->lo[x] * 4, (P->hi[x] + 1) * 4)) continue;
+        total += (int64_t)SRV_ALIGN256((size_t)(P->hi[x] - P->lo[x] + 1) * 4);
+    }
+    return total;
+}
+
+/* Copies the host operands of a request joining the persistent launch into
+ * the batch's workspace at *off (advanced past them) on the slot's stream
+ * and points ops[] (A, B, C) at the copies.  False when they no longer fit
+ * what the batch was admitted with (the plan's layouts changed since
+ * submit) or a copy fails: the request then takes the general path. */
+static bool stage_group_operands(struct tapp_exec* ex, struct tapp_plan* p, const tk_problem* P, size_t limit,
+                                 size_t* off, const void* ops[3])
+{
+    for (int x = 0; x < 3; x++) {
+        if (!ops[x] || (x == 2 && P->beta_zero) || P->hi[x] < P->lo[x]) continue;
+        if (tk_resolve_operand(p->handle, ops[x], P->lo[x] * 4, (P->hi[x] + 1) * 4)) continue;
+        const size_t bytes = (size_t)(P->hi[x] - P->lo[x] + 1) * 4;
+        if (*off + SRV_ALIGN256(bytes) > limit) return false;
+        char* buf = (char*)ex->ws + *off;
+        if (hipMemcpyAsync(buf, (const char*)ops[x] + P->lo[x] * 4, bytes, hipMemcpyHostToDevice, ex->stream) !=
+            hipSuccess) {
+            (void)hipGetLastError();
+            return false;
+        }
+        ops[x] = buf - P->lo[x] * 4;
+        *off += SRV_ALIGN256(bytes);
+    }
+    return true;
+}
+
 static void launch_batch(struct tapp_server* sv, srv_slot* s)
 {
     struct tapp_exec* ex = &s->ex;
@@ -180,7 +233,9 @@ static void launch_batch(struct tapp_server* sv, srv_slot* s)
         hipEventRecord(s->done, ex->stream);
         return;
     }
-    /* which requests join the persistent launch */
+    /* which requests join the persistent launch; their host operands are
+     * staged first, into the front of the workspace */
+    size_t staged = 0;
     for (srv_request* r = s->batch; r; r = r->next) {
         r->t_start = now_us();
         r->in_group = -1;
@@ -190,8 +245,10 @@ static void launch_batch(struct tapp_server* sv, srv_slot* s)
         if (P.empty) { r->rc = 0; continue; }
         sv->flop_done += (double)product_flop(&P);
         tk_prob d;
-        if (product_flop(&P) <= sv->large_flop &&
-            tk_group_describe(r->plan, ex, &P, r->A, r->B, r->C, r->D, -1, &d, &rc)) {
+        const void* ops[3] = {r->A, r->B, r->C};
+        const bool fits = product_flop(&P) <= sv->large_flop &&
+                          (!r->stage || stage_group_operands(ex, r->plan, &P, s->batch_work, &staged, ops));
+        if (fits && tk_group_describe(r->plan, ex, &P, ops[0], ops[1], ops[2], r->D, -1, &d, &rc)) {
             s->hprobs[n] = d;
             total += d.L * d.tiles_s * d.tiles_g;
             s->hend[n] = total;
@@ -346,7 +403,7 @@ static void* dispatcher(void* arg)
                 srv_request *first = NULL, *last = NULL, *prev = NULL;
                 srv_request** link = &sv->head;
                 int taken = 0;
-                size_t need = 0, tabs = 0;
+                size_t need = 0, stg = 0, tabs = 0;
                 bool heavy = false;
                 size_t held = held_bytes(sv);
                 while (*link && taken < sv->max_batch) {
@@ -367,12 +424,13 @@ static void* dispatcher(void* arg)
                     /* tables count once per plan and only while not yet built */
                     const size_t t = (r->tables && !plan_in(first, NULL, r->plan) && tk_plan_needs_tables(r->plan, &s->ex))
                                          ? r->tables : 0;
-                    const size_t n2 = r->work > need ? r->work : need;
-                    size_t grow = (n2 > s->ex.ws_size ? n2 - s->ex.ws_size : 0) + tabs + t;
+                    const size_t n2 = r->work > need ? r->work : need, s2 = stg + r->stage;
+                    const size_t w2 = n2 > s2 ? n2 : s2;
+                    size_t grow = (w2 > s->ex.ws_size ? w2 - s->ex.ws_size : 0) + tabs + t;
                     if (held + grow > sv->budget && taken == 0) {
-                        reclaim(sv, i, n2);
+                        reclaim(sv, i, w2);
                         held = held_bytes(sv);
-                        grow = (n2 > s->ex.ws_size ? n2 - s->ex.ws_size : 0) + t;
+                        grow = (w2 > s->ex.ws_size ? w2 - s->ex.ws_size : 0) + t;
                     }
                     if (held + grow > sv->budget) {
                         bool any = false;
@@ -401,6 +459,7 @@ static void* dispatcher(void* arg)
                     if (last) last->next = r; else first = r;
                     last = r;
                     need = n2;
+                    stg = s2;
                     tabs += t;
                     heavy |= r->heavy;
                     taken++;
@@ -408,7 +467,7 @@ static void* dispatcher(void* arg)
                 if (!taken) break;
                 s->batch = first;
                 s->heavy = heavy;
-                s->batch_work = need;
+                s->batch_work = need > stg ? need : stg;
                 s->batch_tables = tabs;
                 sv->tables_held += tabs;
                 sv->inflight += taken;
@@ -502,21 +561,29 @@ TAPP_error TAPP_AMD_create_server(TAPP_AMD_server* server, TAPP_executor exec, s
 
 /* Device memory a request needs: the workspace of a general-path run
  * (staged host operands, pre-pass and conversion buffers, split-K
- * partials) -- none when it will run in the persistent launch -- and the
- * offset tables its plan has yet to build on the device. */
+ * partials) -- none when it will run in the persistent launch, which
+ * needs only its staged host operands (*stage) -- and the offset tables
+ * its plan has yet to build on the device. */
 static int footprint(struct tapp_server* sv, struct tapp_plan* p, const void* alpha, const void* A, const void* B,
-                     const void* beta, const void* C, void* D, size_t* work, size_t* tables, int64_t* flop)
+                     const void* beta, const void* C, void* D, size_t* work, size_t* stage, size_t* tables,
+                     int64_t* flop)
 {
     const struct tapp_exec* ex = &sv->slot[0].ex;   /* the slots' attributes */
     tk_problem P;
     int rc = tk_plan_problem(p, ex, alpha, beta, C, &P);
     if (rc) return rc;
-    *work = *tables = 0;
+    *work = *stage = *tables = 0;
     *flop = P.empty ? 0 : product_flop(&P);
     if (P.empty) return 0;
     bool devops = false;
     if ((rc = tk_call_bytes(p, ex, &P, A, B, C, D, work, tables, &devops))) return rc;
-    if (devops && *flop <= sv->large_flop && tk_group_may_run(&P)) *work = 0;
+    if (*flop > sv->large_flop) return 0;
+    if (devops && tk_group_may_run(&P)) {
+        *work = 0;
+    } else {
+        const int64_t sb = group_stage_bytes(p, &P, A, B, C, D);
+        if (sb > 0 && (size_t)sb <= sv->heavy_bytes) { *work = 0; *stage = (size_t)sb; }
+    }
     return 0;
 }
 
@@ -527,11 +594,11 @@ TAPP_error TAPP_AMD_server_submit(TAPP_AMD_server server, TAPP_tensor_product pl
     struct tapp_server* sv = (struct tapp_server*)server;
     if (!sv || sv->magic != TAPP_AMD_EXEC_MAGIC || !plan || !status) return TAPP_AMD_ERR_INVALID;
     struct tapp_plan* p = (struct tapp_plan*)plan;
-    size_t work = 0, tables = 0;
+    size_t work = 0, stage = 0, tables = 0;
     int64_t fl = 0;
-    int rc = footprint(sv, p, alpha, A, B, beta, C, D, &work, &tables, &fl);
+    int rc = footprint(sv, p, alpha, A, B, beta, C, D, &work, &stage, &tables, &fl);
     if (rc) return rc;
-    if (work + tables > sv->budget) return TAPP_AMD_ERR_BUDGET;
+    if (work + stage + tables > sv->budget) return TAPP_AMD_ERR_BUDGET;
     srv_request* r = calloc(1, sizeof *r);
     if (!r) return TAPP_AMD_ERR_NOMEM;
     r->magic = STATUS_MAGIC;
@@ -543,6 +610,7 @@ TAPP_error TAPP_AMD_server_submit(TAPP_AMD_server server, TAPP_tensor_product pl
     if (beta && ss) memcpy(r->beta, beta, ss);
     r->A = A; r->B = B; r->C = C; r->D = D;
     r->work = work;
+    r->stage = stage;
     r->tables = tables;
     r->heavy = fl > sv->large_flop || work > sv->heavy_bytes;
     r->t_submit = now_us();

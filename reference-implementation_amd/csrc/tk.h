@@ -158,6 +158,12 @@ typedef struct tk_prob {
     /* MFMA tiles: X + xlo (Y + ylo) is the lowest element the tile can read,
      * and every element lies within 2^29 of it, so offsets are 32-bit */
     int64_t xlo, ylo;
+    /* Thin tiles over a dense Y (y_kfast 2) whose other tables are all
+     * index * stride: affine = 1 and the strides below, so a tile reads no
+     * table.  X (s, k, l) at s ssX + k skX + l slX, Y's l at l slY, D
+     * (s, g, l) at s ssD + g sgD + l slD, C likewise. */
+    int affine;
+    int64_t ssX, skX, slX, slY, ssD, sgD, slD, ssC, sgC, slC;
 } tk_prob;
 
 /* All launchers are asynchronous on `stream` (a hipStream_t) and return 0

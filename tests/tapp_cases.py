@@ -144,7 +144,7 @@ def make_operand(dtype, extents, idx, seed, order=None, pad=0, negative=(), op=0
     if strides is None:
         strides, off, n = layout(extents, order, pad, negative)
     else:
-        o = Operand(dtype, extents, strides, idx)
+        o = Operand(dtype, extents, strides, idx if not isinstance(idx, str) else labels(idx))
         lo, hi = o.span()
         off, n = -lo, max(1, hi - lo + 1)
     buf = fill(dtype, n, seed)

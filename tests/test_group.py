@@ -330,6 +330,53 @@ def test_batched_product_reuse_follows_layout_change(lib, oracle):
     lib.destroy_executor(ex)
 
 
+def test_group_thin_tiles_affine_and_table_layouts(lib, gpu_exec, oracle):
+    """Thin tiles over a dense Y take the strided path when every other
+    table of the product is index * stride (a mode's stride is the previous
+    one times its extent), and the table path otherwise.  Mode products
+    D[r,j,k] = alpha sum_i A[i,j,k] U[r,i] + beta C with: dense operands;
+    U stored transposed; D and C with a padded but affine column stride;
+    D and C stored backwards along r; D with j and k swapped in storage
+    (table path); A padded (not dense: table path); a Hadamard label k in
+    U and D.  Each D equals the oracle's."""
+    i, j, k, r = 37, 11, 29, 9
+    mk = T.make_operand
+    f32 = tc.F32
+
+    def case(n, A, U, C, D, beta=0.5):
+        return T.Case(A, U, C, D, tc.DEFAULT_PREC, 1.25, beta, name=f"thin_layout{n}")
+
+    def dense_A(seed, pad=0):
+        return mk(f32, [i, j, k], "ijk", seed, pad=pad)
+
+    cases = [
+        case(0, dense_A(1), mk(f32, [r, i], "ri", 2), mk(f32, [r, j, k], "rjk", 3), mk(f32, [r, j, k], "rjk", 4)),
+        case(1, dense_A(5), mk(f32, [r, i], "ri", 6, order=[1, 0]), mk(f32, [r, j, k], "rjk", 7),
+             mk(f32, [r, j, k], "rjk", 8), beta=0.0),
+        case(2, dense_A(9), mk(f32, [r, i], "ri", 10), mk(f32, [r, j, k], "rjk", 11, strides=[1, r + 3, (r + 3) * j]),
+             mk(f32, [r, j, k], "rjk", 12, strides=[1, r + 5, (r + 5) * j])),
+        case(3, dense_A(13), mk(f32, [r, i], "ri", 14), mk(f32, [r, j, k], "rjk", 15, negative=(0,)),
+             mk(f32, [r, j, k], "rjk", 16, negative=(0,))),
+        case(4, dense_A(17), mk(f32, [r, i], "ri", 18), mk(f32, [r, j, k], "rjk", 19),
+             mk(f32, [r, j, k], "rjk", 20, order=[0, 2, 1])),
+        case(5, dense_A(21, pad=2), mk(f32, [r, i], "ri", 22), mk(f32, [r, j, k], "rjk", 23),
+             mk(f32, [r, j, k], "rjk", 24)),
+        case(6, dense_A(25), mk(f32, [r, i, k], "rik", 26), mk(f32, [r, j, k], "rjk", 27),
+             mk(f32, [r, j, k], "rjk", 28)),
+    ]
+    dcs = [DevCase(lib, c) for c in cases]
+    rc, g = make_group(lib, gpu_exec, dcs)
+    assert rc == 0
+    assert lib.group_info(g)[1][0] == len(dcs)          # every product in the persistent launch
+    assert lib.execute_group(g) == 0
+    for d in dcs:
+        _, want = oracle.run(d.case, gpu=True)
+        assert_parity(d.d(), want, d.case)
+    assert lib.destroy_group(g) == 0
+    for d in dcs:
+        d.close()
+
+
 def test_group_dense_thin_tiles_across_k_passes(lib, gpu_exec, oracle):
     """Tucker products on thin tiles across K chunk and pass edges: K = 1,
     31, 32, 33 (one row past two 16-deep chunks), 64, 65, 97, with column

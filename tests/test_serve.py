@@ -256,13 +256,23 @@ def test_concurrent_client_threads(lib, gpu_exec, oracle):
 
 @pytest.mark.gpu
 def test_budget_counts_workspace_and_split_k_partials(lib, gpu_exec, oracle):
-    """Requests on the general path with a staged host A (1 MiB) and split-K
-    partial sums (64 slices of one 128 x 128 tile: 4 MiB): 5 MiB of workspace
-    each.  With an 8 MiB budget only one runs at a time although three
-    streams are free: the others stall, idle workspaces are reclaimed, the
-    server never holds more than the budget, and every D is right."""
-    from test_gpu_parity import _gemm_case, assert_parity
-    cases = [_gemm_case(tc.F32, 128, 128, 2048, 40 + i) for i in range(6)]
+    """Requests on the general path (a unary label v on B keeps them out of
+    the persistent kernel): D[m,n] = sum_{k,v} A[m,k] B[k,n,v] with a staged
+    host A (1 MiB), B's pre-reduction (1 MiB) and split-K partial sums (64
+    slices of one 128 x 128 tile: 4 MiB): 6 MiB of workspace each.  With an
+    8 MiB budget only one runs at a time although three streams are free:
+    the others stall, idle workspaces are reclaimed, the server never holds
+    more than the budget, and every D is right."""
+    from test_gpu_parity import assert_parity
+
+    def unary_gemm(seed, M=128, N=128, K=2048, V=2):
+        A = T.make_operand(tc.F32, [M, K], "mk", seed)
+        B = T.make_operand(tc.F32, [K, N, V], "knv", seed + 1)
+        C = T.make_operand(tc.F32, [M, N], "mn", seed + 2)
+        D = T.make_operand(tc.F32, [M, N], "mn", seed + 3)
+        return T.Case(A, B, C, D, tc.DEFAULT_PREC, 0.75, 0.5, name=f"unary_gemm{seed}")
+
+    cases = [unary_gemm(40 + 4 * i) for i in range(6)]
     plan, infos = _plan(lib, cases[0])
     budget = 8 << 20
     rc, srv = lib.create_server(gpu_exec, budget, 3, 4, lib.SERVER_PAUSED)
@@ -292,3 +302,53 @@ def test_budget_counts_workspace_and_split_k_partials(lib, gpu_exec, oracle):
         assert_parity(dev["D"].cpu().numpy(), want, c)
     lib.destroy_server(srv)
     _free(lib, plan, infos)
+
+
+@pytest.mark.gpu
+def test_host_operand_requests_join_the_persistent_launch(lib, gpu_exec, oracle):
+    """Small FP32 products whose A (and for some B, or C with beta != 0) is
+    host memory that is not resident -- pinned or pageable -- and whose D is
+    device memory: the batch copies those operands into its workspace and
+    runs every request in its persistent launch.  Each D equals the
+    oracle's; the server held workspace for the staged operands."""
+    import torch
+    from test_gpu_parity import assert_parity
+    cases = [T.tucker_case(700 + i, 20 + i, 9, 13, 3 + i) for i in range(6)]
+    cases += [T.four_index_case(720 + i, (6 + i, 5, 7, 4, 3 + i, 5)) for i in range(4)]
+    for c in cases[::3]:
+        c.beta = 0.5                                     # C read (and staged where host)
+    rc, srv = lib.create_server(gpu_exec, 1 << 30, 2, 16, lib.SERVER_PAUSED)
+    assert rc == 0
+    keep, sts = [], []
+    for n, c in enumerate(cases):
+        plan, infos = _plan(lib, c)
+        dev = {x: torch.from_numpy(o.buf.view(np.uint8).copy()).cuda() for x, o in zip("ABCD", c.operands())}
+        host = {}
+        if n % 2:                                        # pinned host A
+            host["A"] = torch.from_numpy(c.A.buf.view(np.uint8).copy()).pin_memory()
+        ptr = {x: dev[x].data_ptr() + o.offset * o.buf.itemsize for x, o in zip("ABCD", c.operands())}
+        ptr["A"] = (host["A"].data_ptr() + c.A.offset * c.A.buf.itemsize) if n % 2 else c.A.ptr()
+        if n % 3 == 1:
+            ptr["B"] = c.B.ptr()                          # pageable host B
+        if n % 3 == 0:
+            ptr["C"] = c.C.ptr()                          # pageable host C, read with beta 0.5
+        torch.cuda.synchronize()
+        al = T.scalar_bytes(c.alpha, c.D.type)
+        be = T.scalar_bytes(c.beta, c.D.type)
+        rc, st = lib.submit(srv, plan, al.ctypes.data, ptr["A"], ptr["B"], be.ctypes.data, ptr["C"], ptr["D"])
+        assert rc == 0, lib.explain(rc)
+        keep.append((c, plan, infos, dev, host, al, be))
+        sts.append(st)
+    assert lib.set_paused(srv, 0) == 0
+    for st in sts:
+        assert lib.wait(srv, st) == (0, 0)
+        lib.destroy_status(st)
+    rc, stats = lib.server_stats(srv)
+    assert stats["completed"] == len(cases), stats
+    assert stats["persistent_products"] == len(cases), stats
+    assert stats["max_held_bytes"] > 0, stats
+    for c, plan, infos, dev, host, al, be in keep:
+        _, want = oracle.run(c, gpu=True)
+        assert_parity(dev["D"].cpu().numpy().view(c.D.buf.dtype), want, c)
+        _free(lib, plan, infos)
+    lib.destroy_server(srv)
