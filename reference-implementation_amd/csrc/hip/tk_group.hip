@@ -19,6 +19,8 @@
 //             (K offsets staged in LDS per 256-deep chunk).
 #include "tk_device.hpp"
 
+#include <type_traits>
+
 namespace tk {
 
 typedef float f32x16 __attribute__((ext_vector_type(16)));
@@ -124,7 +126,7 @@ __device__ __forceinline__ void thin_tile(const GView& p, int64_t l, int64_t g0,
 {
     constexpr int NY = KC * GT / 256;           // Y elements per lane and chunk
     float* Yl = lds;               // [KC][YP]
-    float* Xl = lds + KC * YP;     // [SMAX][KC]
+    float* Xl = lds + KC * YP;     // [KC][SMAX]: a k's rows contiguous, read four at a time
     const int tid = threadIdx.x;
     float acc[SB];
 #pragma unroll
@@ -181,7 +183,7 @@ __device__ __forceinline__ void thin_tile(const GView& p, int64_t l, int64_t g0,
         const int kc = (int)(p.K() - k0 < KC ? p.K() - k0 : KC);
         __syncthreads();                          // the previous chunk's LDS reads are done
         if (!dense) xv = p.X()[xrow + kx_of(k0 + (xk < kc ? xk : 0))];
-        if (tid < SB * KC) Xl[tid] = (x_row && xk < kc) ? rnd_m<RM>(xv) : 0.0f;
+        if (tid < SB * KC) Xl[xk * SMAX + xs] = (x_row && xk < kc) ? rnd_m<RM>(xv) : 0.0f;
         if (dense) {
 #pragma unroll
             for (int i = 0; i < NY; ++i) {
@@ -213,7 +215,13 @@ __device__ __forceinline__ void thin_tile(const GView& p, int64_t l, int64_t g0,
         for (int k = 0; k < kc; ++k) {
             const float y = Yl[k * YP + tid];
 #pragma unroll
-            for (int s = 0; s < SB; ++s) acc[s] += Xl[s * KC + k] * y;
+            for (int s4 = 0; s4 < SB / 4; ++s4) {
+                const float4 x4 = reinterpret_cast<const float4*>(Xl + k * SMAX)[s4];
+                acc[4 * s4] += x4.x * y;
+                acc[4 * s4 + 1] += x4.y * y;
+                acc[4 * s4 + 2] += x4.z * y;
+                acc[4 * s4 + 3] += x4.w * y;
+            }
         }
     }
     if (tid < gn) {
@@ -255,7 +263,10 @@ __device__ __forceinline__ uint32_t pack16(float lo, float hi)
     return (uint32_t)__builtin_bit_cast(uint16_t, x) | ((uint32_t)__builtin_bit_cast(uint16_t, y) << 16);
 }
 
-template <int RM>
+// NA (NB): the wave's 32-row (column) blocks that hold an element of the
+// product (1 or 2): a block wholly past the edge is neither loaded nor
+// multiplied.
+template <int RM, int NA, int NB>
 __device__ __forceinline__ void mfma_chunk(gf Xb, gf Yb, const int32_t (&arow)[2], const int32_t (&bcol)[2],
                                            const bool (&a_ok)[2], const bool (&b_ok)[2], const int32_t* kox,
                                            const int32_t* koy, int kc, int kl, f32x16 (&acc)[2][2])
@@ -271,10 +282,9 @@ __device__ __forceinline__ void mfma_chunk(gf Xb, gf Yb, const int32_t (&arow)[2
                 const int k = j0 + 2 * j + kl;
                 const int32_t ka = kox[k < kc ? k : 0], kb = koy[k < kc ? k : 0];
 #pragma unroll
-                for (int i = 0; i < 2; ++i) {
-                    a[j][i] = Xb[(uint32_t)(arow[i] + ka)];
-                    b[j][i] = Yb[(uint32_t)(bcol[i] + kb)];
-                }
+                for (int i = 0; i < NA; ++i) a[j][i] = Xb[(uint32_t)(arow[i] + ka)];
+#pragma unroll
+                for (int i = 0; i < NB; ++i) b[j][i] = Yb[(uint32_t)(bcol[i] + kb)];
             }
         };
         auto compute = [&](int j0, const float (&a)[JB][2], const float (&b)[JB][2]) {
@@ -282,9 +292,9 @@ __device__ __forceinline__ void mfma_chunk(gf Xb, gf Yb, const int32_t (&arow)[2
             for (int j = 0; j < JB; ++j) {
                 const bool in = j0 + 2 * j + kl < kc;
 #pragma unroll
-                for (int ib = 0; ib < 2; ++ib)
+                for (int ib = 0; ib < NB; ++ib)
 #pragma unroll
-                    for (int ia = 0; ia < 2; ++ia)
+                    for (int ia = 0; ia < NA; ++ia)
                         acc[ib][ia] = __builtin_amdgcn_mfma_f32_32x32x2f32((in && b_ok[ib]) ? b[j][ib] : 0.0f,
                                                                            (in && a_ok[ia]) ? a[j][ia] : 0.0f,
                                                                            acc[ib][ia], 0, 0, 0);
@@ -304,18 +314,19 @@ __device__ __forceinline__ void mfma_chunk(gf Xb, gf Yb, const int32_t (&arow)[2
         // set for the next step's loads would spill beside 64 accumulators)
         for (int j0 = 0; j0 < kc; j0 += 16) {
             uint4 ua[2], ub[2];
-            auto load_pack = [&](gf base, const int32_t (&off)[2], const bool (&ok)[2], const int32_t* ko,
+            auto load_pack = [&](auto NI_, gf base, const int32_t (&off)[2], const bool (&ok)[2], const int32_t* ko,
                                  uint4 (&out)[2]) {
+                constexpr int NI = decltype(NI_)::value;
                 float v[2][8];
 #pragma unroll
                 for (int t = 0; t < 8; ++t) {
                     const int k = j0 + 8 * kl + t;
                     const int32_t kk = ko[k < kc ? k : 0];
 #pragma unroll
-                    for (int i = 0; i < 2; ++i) v[i][t] = base[(uint32_t)(off[i] + kk)];
+                    for (int i = 0; i < NI; ++i) v[i][t] = base[(uint32_t)(off[i] + kk)];
                 }
 #pragma unroll
-                for (int i = 0; i < 2; ++i) {
+                for (int i = 0; i < NI; ++i) {
                     uint32_t pk[4];
 #pragma unroll
                     for (int t = 0; t < 8; t += 2) {
@@ -325,12 +336,12 @@ __device__ __forceinline__ void mfma_chunk(gf Xb, gf Yb, const int32_t (&arow)[2
                     out[i] = make_uint4(pk[0], pk[1], pk[2], pk[3]);
                 }
             };
-            load_pack(Xb, arow, a_ok, kox, ua);
-            load_pack(Yb, bcol, b_ok, koy, ub);
+            load_pack(std::integral_constant<int, NA>{}, Xb, arow, a_ok, kox, ua);
+            load_pack(std::integral_constant<int, NB>{}, Yb, bcol, b_ok, koy, ub);
 #pragma unroll
-            for (int ib = 0; ib < 2; ++ib)
+            for (int ib = 0; ib < NB; ++ib)
 #pragma unroll
-                for (int ia = 0; ia < 2; ++ia) {
+                for (int ia = 0; ia < NA; ++ia) {
                     if (RM == TK_MODE_BF16)
                         acc[ib][ia] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
                             __builtin_bit_cast(bf16x8, ub[ib]), __builtin_bit_cast(bf16x8, ua[ia]), acc[ib][ia], 0, 0, 0);
@@ -350,8 +361,10 @@ __device__ __forceinline__ void mfma_tile(const GView& p, int64_t l, int64_t m0,
     const int il = lane & 31, kl = lane >> 5;
     const int64_t lx = p.lX()[l], ly = p.lY()[l];
     const int64_t wm0 = m0 + 64 * (wave & 1), wn0 = n0 + 64 * (wave >> 1);
-    // a wave whose block lies past the edge only helps stage the K offsets
+    // a wave whose block lies past the edge only helps stage the K offsets;
+    // a2 (b2): the wave's second 32-row (column) block holds an element
     const bool active = wm0 < p.S() && wn0 < p.G();
+    const bool a2 = wm0 + 32 < p.S(), b2 = wn0 + 32 < p.G();
     // operands addressed from their lowest readable element with 32-bit
     // offsets (the host checked every element lies within 2^29 of it)
     const int64_t xlo = p.d->xlo, ylo = p.d->ylo;
@@ -396,7 +409,11 @@ __device__ __forceinline__ void mfma_tile(const GView& p, int64_t l, int64_t m0,
             ky[k] = (int32_t)p.kY()[k0 + k];
         }
         __syncthreads();
-        if (active) mfma_chunk<RM>(Xb, Yb, arow, bcol, a_ok, b_ok, kx, ky, kc, kl, acc);
+        if (!active) continue;
+        if (a2 && b2) mfma_chunk<RM, 2, 2>(Xb, Yb, arow, bcol, a_ok, b_ok, kx, ky, kc, kl, acc);
+        else if (a2) mfma_chunk<RM, 2, 1>(Xb, Yb, arow, bcol, a_ok, b_ok, kx, ky, kc, kl, acc);
+        else if (b2) mfma_chunk<RM, 1, 2>(Xb, Yb, arow, bcol, a_ok, b_ok, kx, ky, kc, kl, acc);
+        else mfma_chunk<RM, 1, 1>(Xb, Yb, arow, bcol, a_ok, b_ok, kx, ky, kc, kl, acc);
     }
     __syncthreads();                             // eoff is visible (also when K is 0)
     if (!active) return;
@@ -443,7 +460,7 @@ __device__ __forceinline__ void thin_affine(const GView& p, int64_t l, int64_t g
 {
     constexpr int NY = KC * GT / 256;           // Y elements per lane and chunk
     float* Yl = lds;               // [KC][YP]
-    float* Xl = lds + KC * YP;     // [SMAX][KC]
+    float* Xl = lds + KC * YP;     // [KC][SMAX]: a k's rows contiguous, read four at a time
     const int tid = threadIdx.x;
     const tk_prob TK_CONST* d = p.d;
     const int64_t K = p.K(), S = p.S();
@@ -474,7 +491,7 @@ __device__ __forceinline__ void thin_affine(const GView& p, int64_t l, int64_t g
     for (int64_t k0 = 0; k0 < K; k0 += KC) {
         const int kc = (int)(K - k0 < KC ? K - k0 : KC);
         __syncthreads();                          // the previous chunk's LDS reads are done
-        if (tid < SB * KC) Xl[tid] = (x_row && xk < kc) ? rnd_m<RM>(xv) : 0.0f;
+        if (tid < SB * KC) Xl[xk * SMAX + xs] = (x_row && xk < kc) ? rnd_m<RM>(xv) : 0.0f;
 #pragma unroll
         for (int i = 0; i < NY; ++i) {
             const int gg = tid / KC + (256 / KC) * i;
@@ -485,7 +502,13 @@ __device__ __forceinline__ void thin_affine(const GView& p, int64_t l, int64_t g
         for (int k = 0; k < kc; ++k) {
             const float y = Yl[k * YP + tid];
 #pragma unroll
-            for (int s = 0; s < SB; ++s) acc[s] += Xl[s * KC + k] * y;
+            for (int s4 = 0; s4 < SB / 4; ++s4) {
+                const float4 x4 = reinterpret_cast<const float4*>(Xl + k * SMAX)[s4];
+                acc[4 * s4] += x4.x * y;
+                acc[4 * s4 + 1] += x4.y * y;
+                acc[4 * s4 + 2] += x4.z * y;
+                acc[4 * s4 + 3] += x4.w * y;
+            }
         }
     }
     if (tid < gn) {
@@ -524,19 +547,63 @@ __device__ __forceinline__ int64_t uniform64(int64_t v)
     return (int64_t)(((uint64_t)hi << 32) | lo);
 }
 
+// A tile of the launch: its product and its (l, ts, tg) position.
+struct TileRef {
+    int lo, l, ts, tg;
+};
+
+// The decoded tile record (one 16-byte load), or without records the first
+// product whose inclusive tile end is past t (binary search) and the
+// position from its tile counts.
+__device__ __forceinline__ TileRef tile_ref(const tk_prob* __restrict__ probs, const int64_t* __restrict__ tile_end,
+                                            const tk_tile* __restrict__ tiles, int nprobs, int64_t t)
+{
+    TileRef r;
+    if (tiles) {
+        typedef const int32_t TK_CONST* cint;
+        const cint v = (cint)(tiles + t);
+        r.lo = __builtin_amdgcn_readfirstlane(v[0]);
+        r.l = __builtin_amdgcn_readfirstlane(v[1]);
+        r.ts = __builtin_amdgcn_readfirstlane(v[2]);
+        r.tg = __builtin_amdgcn_readfirstlane(v[3]);
+        return r;
+    }
+    int a = 0, b = nprobs - 1;
+    while (a < b) {
+        const int mid = (a + b) >> 1;
+        if (tile_end[mid] > t) b = mid; else a = mid + 1;
+    }
+    r.lo = __builtin_amdgcn_readfirstlane(a);
+    const tk_prob TK_CONST* pd = (const tk_prob TK_CONST*)probs + r.lo;
+    const int64_t local = t - (r.lo ? tile_end[r.lo - 1] : 0);
+    const int64_t per_l = pd->tiles_s * pd->tiles_g;
+    const int64_t rem = local % per_l;
+    r.l = (int)(local / per_l);
+    r.ts = (int)(rem / pd->tiles_g);
+    r.tg = (int)(rem % pd->tiles_g);
+    return r;
+}
+
 // Tiles come from `nq` queues: queue q holds tiles [qbeg[q], qbeg[q + 1])
 // (one queue over [0, total) when qbeg is null) and has its own counter,
 // counter[32 q].  A workgroup starts with queue blockIdx.x % nq -- with one
 // queue per XCD that is its own XCD's, since workgroups are dispatched to the
 // XCDs round-robin, so a product's tiles share that XCD's L2 -- and then
 // takes what is left in the others, in order, so every workgroup ends after
-// every queue is empty.
+// every queue is empty.  A workgroup takes the index of its next tile from
+// the counter when it starts a tile.  The thin-tile kernel, given decoded
+// tile records, also holds the tile after the one it runs (its first atomic
+// takes two) and loads that tile's record while the current tile runs.
+// MFMA tiles are long enough to take their record at the start, and a
+// second held tile would lengthen the tail of their launch.  An index at or past the
+// queue's end is dropped; every index below it is run by the workgroup that
+// took it.
 // KIND (TK_GROUP_KIND_*) restricts the tiles a kernel can run, so the thin-
 // only kernel carries neither the MFMA tile's registers nor its LDS.
 template <int KIND>
 __global__ __launch_bounds__(256, KIND == TK_GROUP_KIND_THIN ? TK_GROUP_WG_PER_CU_THIN : TK_GROUP_WG_PER_CU)
 void k_group(const tk_prob* __restrict__ probs, const int64_t* __restrict__ tile_end,
-             const int* __restrict__ tile_prob, int nprobs, int64_t total, const int64_t* __restrict__ qbeg, int nq,
+             const tk_tile* __restrict__ tiles, int nprobs, int64_t total, const int64_t* __restrict__ qbeg, int nq,
              unsigned long long* counter)
 {
     __shared__ __attribute__((aligned(16))) float lds[KC * YP + SMAX * KC];
@@ -546,44 +613,35 @@ void k_group(const tk_prob* __restrict__ probs, const int64_t* __restrict__ tile
     __shared__ int64_t eoff[4 * MT];
     __shared__ int64_t next;
     const int home = (int)(blockIdx.x % (unsigned)nq);
+    const bool prefetch = KIND == TK_GROUP_KIND_THIN && tiles;
     for (int qi = 0; qi < nq; ++qi) {
         const int q = (home + qi) % nq;
         const int64_t qb = qbeg ? qbeg[q] : 0, qe = qbeg ? qbeg[q + 1] : total;
-        if (threadIdx.x == 0) next = qb + (int64_t)atomicAdd(counter + 32 * q, 1ull);
+        if (threadIdx.x == 0) next = qb + (int64_t)atomicAdd(counter + 32 * q, prefetch ? 2ull : 1ull);
         __syncthreads();
         // uniform over the workgroup: said so, so the descriptor below is
         // read with scalar loads into scalar registers
-        int64_t t = uniform64(next);
+        int64_t t = uniform64(next), t1 = t + 1;
         __syncthreads();
+        TileRef cur{};
+        if (t < qe) cur = tile_ref(probs, tile_end, tiles, nprobs, t);
         while (t < qe) {
-            // the next tile's index is taken now and kept in thread 0's
-            // registers until this tile is done: the atomic's round trip
-            // overlaps the tile's own loads instead of preceding them
+            // the next tile's record, in flight during this tile (records
+            // only: the binary search would wait on its loads here)
+            TileRef nxt{};
+            if (prefetch && t1 < qe) nxt = tile_ref(probs, tile_end, tiles, nprobs, t1);
+            // the index of a later tile (the next one, or with prefetching
+            // the one after it) is taken now and kept in thread 0's
+            // registers until this tile is done
             unsigned long long ahead = 0;
             if (threadIdx.x == 0) ahead = atomicAdd(counter + 32 * q, 1ull);
-            int lo;
-            if (tile_prob) {
-                lo = tile_prob[t];                       // direct map, one load
-            } else {
-                int a = 0, b = nprobs - 1;               // first problem with tile_end > t
-                while (a < b) {
-                    const int mid = (a + b) >> 1;
-                    if (tile_end[mid] > t) b = mid; else a = mid + 1;
-                }
-                lo = a;
-            }
-            lo = __builtin_amdgcn_readfirstlane(lo);
             // the descriptor through the constant address space: its scalar
             // loads are invariant, so they are re-issued where a field is
             // used instead of holding every field in registers for the tile
-            const tk_prob TK_CONST* pd = (const tk_prob TK_CONST*)probs + lo;
+            const tk_prob TK_CONST* pd = (const tk_prob TK_CONST*)probs + cur.lo;
             const GView p(pd);
-            const int64_t local = t - (lo ? tile_end[lo - 1] : 0);
-            const int64_t tiles_s = pd->tiles_s, tiles_g = pd->tiles_g;
             const int cls = pd->cls;
-            const int64_t per_l = tiles_s * tiles_g;
-            const int64_t l = local / per_l, rem = local % per_l;
-            const int64_t ts = rem / tiles_g, tg = rem % tiles_g;
+            const int64_t l = cur.l, ts = cur.ts, tg = cur.tg;   // tile records hold them as int32
             if (KIND == TK_GROUP_KIND_THIN || (KIND == TK_GROUP_KIND_ALL && cls == TK_GROUP_THIN)) {
                 if (p.round() == TK_MODE_BF16) thin_dispatch<TK_MODE_BF16>(p, l, tg * GT, lds, goff, soff, kox);
                 else if (p.round() == TK_MODE_F16) thin_dispatch<TK_MODE_F16>(p, l, tg * GT, lds, goff, soff, kox);
@@ -595,8 +653,14 @@ void k_group(const tk_prob* __restrict__ probs, const int64_t* __restrict__ tile
             }
             if (threadIdx.x == 0) next = qb + (int64_t)ahead;
             __syncthreads();
-            t = uniform64(next);
+            if (prefetch) {
+                t = t1;
+                t1 = uniform64(next);
+            } else {
+                t = uniform64(next);
+            }
             __syncthreads();
+            if (t < qe) cur = prefetch ? nxt : tile_ref(probs, tile_end, tiles, nprobs, t);
         }
     }
 }
@@ -611,7 +675,7 @@ __global__ void k_zero_counters(unsigned long long* counter, int nq)
 
 using namespace tk;
 
-extern "C" int tk_launch_group_kind(int kind, const tk_prob* probs, const int64_t* tile_end, const int* tile_prob,
+extern "C" int tk_launch_group_kind(int kind, const tk_prob* probs, const int64_t* tile_end, const tk_tile* tiles,
                                     int nprobs, int64_t total, const int64_t* qbeg, int nq, unsigned long long* counter,
                                     int nblocks, void* stream)
 {
@@ -623,30 +687,30 @@ extern "C" int tk_launch_group_kind(int kind, const tk_prob* probs, const int64_
     switch (kind) {
     case TK_GROUP_KIND_THIN:
         hipLaunchKernelGGL(k_group<TK_GROUP_KIND_THIN>, dim3((unsigned)nblocks), dim3(256), 0, st, probs, tile_end,
-                           tile_prob, nprobs, total, q, nq, counter);
+                           tiles, nprobs, total, q, nq, counter);
         break;
     case TK_GROUP_KIND_MFMA:
         hipLaunchKernelGGL(k_group<TK_GROUP_KIND_MFMA>, dim3((unsigned)nblocks), dim3(256), 0, st, probs, tile_end,
-                           tile_prob, nprobs, total, q, nq, counter);
+                           tiles, nprobs, total, q, nq, counter);
         break;
     default:
         hipLaunchKernelGGL(k_group<TK_GROUP_KIND_ALL>, dim3((unsigned)nblocks), dim3(256), 0, st, probs, tile_end,
-                           tile_prob, nprobs, total, q, nq, counter);
+                           tiles, nprobs, total, q, nq, counter);
         break;
     }
     return (int)hipGetLastError();
 }
 
-extern "C" int tk_launch_group_q(const tk_prob* probs, const int64_t* tile_end, const int* tile_prob, int nprobs,
+extern "C" int tk_launch_group_q(const tk_prob* probs, const int64_t* tile_end, const tk_tile* tiles, int nprobs,
                                  int64_t total, const int64_t* qbeg, int nq, unsigned long long* counter, int nblocks,
                                  void* stream)
 {
-    return tk_launch_group_kind(TK_GROUP_KIND_ALL, probs, tile_end, tile_prob, nprobs, total, qbeg, nq, counter,
+    return tk_launch_group_kind(TK_GROUP_KIND_ALL, probs, tile_end, tiles, nprobs, total, qbeg, nq, counter,
                                 nblocks, stream);
 }
 
-extern "C" int tk_launch_group(const tk_prob* probs, const int64_t* tile_end, const int* tile_prob, int nprobs,
+extern "C" int tk_launch_group(const tk_prob* probs, const int64_t* tile_end, const tk_tile* tiles, int nprobs,
                                int64_t total, unsigned long long* counter, int nblocks, void* stream)
 {
-    return tk_launch_group_q(probs, tile_end, tile_prob, nprobs, total, nullptr, 1, counter, nblocks, stream);
+    return tk_launch_group_q(probs, tile_end, tiles, nprobs, total, nullptr, 1, counter, nblocks, stream);
 }

@@ -31,7 +31,7 @@ typedef struct {
     int nblocks;
     tk_prob* dprobs;
     int64_t* dtile_end;        /* inclusive tile prefix over the launch's products */
-    int* dtile_prob;           /* tile -> product of the launch (or NULL) */
+    tk_tile* dtiles;           /* every tile decoded: product of the launch, l, ts, tg (or NULL) */
     int64_t* dqbeg;            /* first tile of each queue, GROUP_QUEUES + 1 entries */
     unsigned long long* dcounter;   /* one counter per queue, 256 bytes apart */
 } group_launch;
@@ -407,7 +407,7 @@ static int launch_all(struct tapp_group* g, hipStream_t st)
 {
     for (int i = 0; i < g->nlaunch; i++) {
         const group_launch* L = &g->launch[i];
-        int rc = tk_launch_group_kind(L->kind, L->dprobs, L->dtile_end, L->dtile_prob, L->n, L->total, L->dqbeg,
+        int rc = tk_launch_group_kind(L->kind, L->dprobs, L->dtile_end, L->dtiles, L->n, L->total, L->dqbeg,
                                       GROUP_QUEUES, L->dcounter, L->nblocks, st);
         if (rc) return rc;
     }
@@ -428,7 +428,7 @@ static int build_launches(struct tapp_group* g, struct tapp_exec* ex, const rank
     int* bin = malloc(sizeof(int) * (size_t)n);
     int* src = malloc(sizeof(int) * (size_t)n);
     int* ord = malloc(sizeof(int) * (size_t)n);
-    int* hm = NULL;
+    tk_tile* hm = NULL;
     int rc = 0;
     if (!hp || !he || !bin || !src || !ord) { rc = TAPP_AMD_ERR_NOMEM; goto out; }
     /* thin products first, then MFMA products, each run in cost order */
@@ -462,15 +462,19 @@ static int build_launches(struct tapp_group* g, struct tapp_exec* ex, const rank
         tot[c] = total;
     }
     g->total = tot[0] + tot[1];
-    /* tile -> product maps (int32), so a workgroup finds its product in one load */
+    /* every tile decoded (product, l, ts, tg), so a workgroup knows its next
+     * tile from one 16-byte load, issued while it runs the current one */
     const bool map = g->total <= INT32_MAX;
-    const size_t mb = map ? sizeof(int) * (size_t)g->total : 0;
+    const size_t mb = map ? sizeof(tk_tile) * (size_t)g->total : 0;
     if (map) {
         if (!(hm = malloc(mb > 0 ? mb : 1))) { rc = TAPP_AMD_ERR_NOMEM; goto out; }
         int64_t t = 0;
         for (int c = 0; c < 2; c++)
-            for (int i = first[c], t0 = 0; i < first[c + 1]; i++)
-                for (; t0 < he[i]; t0++) hm[t++] = i - first[c];
+            for (int i = first[c]; i < first[c + 1]; i++)
+                for (int64_t l = 0; l < hp[i].L; l++)
+                    for (int64_t ts = 0; ts < hp[i].tiles_s; ts++)
+                        for (int64_t tg = 0; tg < hp[i].tiles_g; tg++)
+                            hm[t++] = (tk_tile){i - first[c], (int32_t)l, (int32_t)ts, (int32_t)tg};
     }
     const size_t o_end = (pb + 255) & ~(size_t)255, o_map = (o_end + tb + 255) & ~(size_t)255;
     const size_t o_q = (o_map + mb + 255) & ~(size_t)255;
@@ -498,7 +502,7 @@ static int build_launches(struct tapp_group* g, struct tapp_exec* ex, const rank
         L->nblocks = (int)(tot[c] < nb ? tot[c] : nb);
         L->dprobs = g->dprobs + first[c];
         L->dtile_end = (int64_t*)(base + o_end) + first[c];
-        L->dtile_prob = map ? (int*)(base + o_map) + (c == 0 ? 0 : tot[0]) : NULL;
+        L->dtiles = map ? (tk_tile*)(base + o_map) + (c == 0 ? 0 : tot[0]) : NULL;
         L->dqbeg = (int64_t*)(base + o_q) + c * (GROUP_QUEUES + 1);
         L->dcounter = (unsigned long long*)(base + o_cnt + (size_t)c * 256 * GROUP_QUEUES);
     }

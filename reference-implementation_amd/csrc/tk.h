@@ -166,6 +166,12 @@ typedef struct tk_prob {
     int64_t ssX, skX, slX, slY, ssD, sgD, slD, ssC, sgC, slC;
 } tk_prob;
 
+/* A tile of a persistent launch, decoded on the host: its product (index
+ * into the launch's descriptors), Hadamard index and row / column tile. */
+typedef struct tk_tile {
+    int32_t prob, l, ts, tg;
+} tk_tile;
+
 /* All launchers are asynchronous on `stream` (a hipStream_t) and return 0
  * or a hipError_t value. */
 int tk_launch_gett(const tk_gett* g, void* stream);
@@ -181,18 +187,19 @@ int tk_launch_prep2(const tk_prep* a, const tk_prep* b, void* stream);
 int tk_launch_scatter(const tk_scatter* s, void* stream);
 /* Zeroes *counter, then runs the persistent group kernel: `nblocks`
  * workgroups take tiles from the atomic counter until `total` is reached.
- * tile_prob (optional, int[total]) maps a tile to its product. */
-int tk_launch_group(const tk_prob* probs, const int64_t* tile_end, const int* tile_prob, int nprobs,
+ * tiles (optional, tk_tile[total]) decodes every tile; without it a
+ * workgroup finds a tile's product by a binary search over tile_end. */
+int tk_launch_group(const tk_prob* probs, const int64_t* tile_end, const tk_tile* tiles, int nprobs,
                     int64_t total, unsigned long long* counter, int nblocks, void* stream);
 /* Same with `nq` tile queues: queue q holds tiles [qbeg[q], qbeg[q + 1])
  * (qbeg on the device, nq + 1 entries) and counts with counter[32 q]; a
  * workgroup drains queue blockIdx % nq first, then the others. */
-int tk_launch_group_q(const tk_prob* probs, const int64_t* tile_end, const int* tile_prob, int nprobs,
+int tk_launch_group_q(const tk_prob* probs, const int64_t* tile_end, const tk_tile* tiles, int nprobs,
                       int64_t total, const int64_t* qbeg, int nq, unsigned long long* counter, int nblocks,
                       void* stream);
 /* Same, running only the tiles of `kind` (TK_GROUP_KIND_*): every product
  * given must be of that class. */
-int tk_launch_group_kind(int kind, const tk_prob* probs, const int64_t* tile_end, const int* tile_prob, int nprobs,
+int tk_launch_group_kind(int kind, const tk_prob* probs, const int64_t* tile_end, const tk_tile* tiles, int nprobs,
                          int64_t total, const int64_t* qbeg, int nq, unsigned long long* counter, int nblocks,
                          void* stream);
 
