@@ -12,6 +12,11 @@ has the reference sources at /root/reference and oracle/_ref built).
 2. ref_configs0.npz: the configs[0] contraction (D[a,b,c,d] = 1.5 sum_{e,f}
    A[a,e,b,f] B[d,f,c,e] + 0.5 C, extents 16, FP32, seed 1) computed by the
    compiled reference, full D.
+3. ref_configs1_sample.npz: the configs[1] contraction (complex double,
+   TAPP_F64F64_ACCUM_F64: D[a,b,h,i] = alpha sum_{c,u,v} A[a,c,c,h,u]
+   conj(B[h,b,c,i,v]) + beta C, seed 1) computed by the compiled reference;
+   its D storage (26 MB) is sampled at 4096 positions drawn with seed 11,
+   which are stored with their values.
 
 The fixtures are data only (inputs and outputs).  Nothing from the reference
 sources is stored.
@@ -83,6 +88,16 @@ def configs0(tmp):
     print("configs0: D", out.shape, float(np.abs(out).max()))
 
 
+def configs1_sample():
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import tapp_cases as tc  # noqa: E402
+    case = tc.configs1_case(seed=1)
+    out = tc.run_reference(REFLIB, case)
+    idx = np.sort(np.random.default_rng(11).choice(out.size, 4096, replace=False)).astype(np.int64)
+    np.savez_compressed(os.path.join(HERE, "ref_configs1_sample.npz"), idx=idx, D=out[idx], seed=np.int64(1))
+    print("configs1: D sample", idx.size, "of", out.size, float(np.abs(out[idx]).max()))
+
+
 def main():
     with tempfile.TemporaryDirectory() as tmp:
         cases = capture_programs(tmp)
@@ -91,7 +106,11 @@ def main():
             json.dump({"source": "reference programs run against the compiled reference library",
                        "cases": cases}, f, indent=None, separators=(",", ":"))
         configs0(tmp)
+    configs1_sample()
 
 
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["configs1"]:
+        configs1_sample()
+    else:
+        main()

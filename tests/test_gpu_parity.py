@@ -131,6 +131,19 @@ def test_configs0_against_reference_golden(lib, gpu_exec):
     assert_parity(got, g["D"], c)
 
 
+def test_configs1_against_reference_golden_sample(lib, gpu_exec):
+    """configs[1] on the GPU against the compiled reference's D at 4096
+    sampled positions (FP64 tolerance, normwise over the sample)."""
+    import os
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "ref_configs1_sample.npz"))
+    c = T.configs1_case(seed=int(g["seed"]))
+    rc, got = T.run_lib(lib, c, gpu_exec)
+    assert rc == 0
+    want = g["D"]
+    err = np.abs(got[g["idx"]] - want).max() / np.abs(want).max()
+    assert err < 1e-12, err
+
+
 def test_small_tile_kernel_by_rule_and_batches(lib, gpu_exec, oracle):
     """Few outputs and a moderate K take the 16 x 16 small-tile kernel by
     rule (configs[0] itself; a Hadamard batch; ragged edges; K not a

@@ -39,6 +39,16 @@ def test_oracle_configs0_golden(oracle):
     assert same_bits(d, g["D"])
 
 
+def test_oracle_configs1_golden_sample(oracle):
+    """configs[1] (complex double, a diagonal, unary labels on both sides,
+    conj(B), negative and padded strides): the oracle's D equals the
+    compiled reference's, bit for bit, at 4096 sampled positions."""
+    g = np.load(os.path.join(GOLDEN, "ref_configs1_sample.npz"))
+    rc, d = oracle.run(T.configs1_case(seed=int(g["seed"])))
+    assert rc == 0
+    assert same_bits(d[g["idx"]], g["D"])
+
+
 def _case_from_kat(k):
     ops = []
     for name in ("A", "B", "C", "D_in"):
