@@ -156,15 +156,17 @@ TAPP_EXPORT TAPP_error TAPP_AMD_group_info(TAPP_AMD_product_group group, int64_t
  * A request loop with continuous batching on one GPU.  A dispatcher thread
  * owns `slots` HIP streams; whenever one is idle it admits queued requests
  * (FIFO) while the device memory the server holds fits `hbm_budget_bytes`,
- * launches them as one batch (small FP32 products in one persistent launch,
- * the rest on the general path) and keeps admitting while batches run on
- * other streams.  Held memory: each stream's workspace (sized to the largest
- * general-path request of its batch -- staged host operands, pre-pass and
- * conversion buffers, split-K partial sums -- and kept between batches),
- * and the offset tables that batches in flight build for their plans
- * (afterwards the plan keeps them, outside the budget); the persistent
- * launch's descriptor buffers (232 bytes per request of a batch) are not
- * counted.
+ * launches them as one batch (small FP32 products with D in device memory
+ * in one persistent launch, their host operands that are not resident --
+ * up to 4 MiB per request -- copied into the stream's workspace first; the
+ * rest on the general path) and keeps admitting while batches run on other
+ * streams.  Held memory: each stream's workspace (sized to the larger of
+ * the persistent launch's staged host operands and the largest general-path
+ * request of its batch -- staged host operands, pre-pass and conversion
+ * buffers, split-K partial sums -- and kept between batches), and the
+ * offset tables that batches in flight build for their plans (afterwards
+ * the plan keeps them, outside the budget); the persistent launch's
+ * descriptor buffers are not counted.
  * When a request does not fit, idle streams' workspaces are freed first;
  * if it still does not fit it waits (an admission stall).  Device operands
  * and resident tensors cost nothing.  A heavy request (above 2^31 flop, or
