@@ -174,7 +174,7 @@ def cpu_baseline(A, B, D_gpu, sample_j=2):
 def batching_problems(seed=0, count=8192):
     """The uneven batching workload, drawn per problem: 80 % Tucker mode
     products D[r,j,k] = sum_i A[i,j,k] U[r,i] with i,j,k ~ U{4..64},
-    r ~ U{2..16}; 20 % configs[0]-form products with each of the six extents
+    r ~ U{2..16}; 20 % four-index products (D[a,b,c,d] = sum A[a,e,b,f] B[d,f,c,e]) with each of the six extents
     ~ U{6..24} (drawn per index).  Returns (kind, extents) tuples."""
     rng = np.random.default_rng(seed)
     out = []
@@ -189,7 +189,7 @@ def batching_problems(seed=0, count=8192):
 
 def cpu_reference_batch(sample=16):
     """The compiled reference (oracle/_ref) on one host core for the batching
-    and serving shape classes: the first `sample` Tucker and configs[0]-form
+    and serving shape classes: the first `sample` Tucker and four-index
     products of the batching draw, timed one by one and extrapolated per
     class to the 8192-product batch; and one virtual-extent-64 ladder-form
     serving request on a D-slice (i = j = 0, 1/1024 of D) at its full K,
@@ -252,15 +252,15 @@ def cpu_reference_batch(sample=16):
                   1.0, 0.0)
     ref.destroy_executor(ex)
     return {"batch_8192_s_extrapolated": batch_s,
-            "batch_sample": f"first {sample} Tucker and {sample} configs[0]-form products of the seed-0 draw, "
+            "batch_sample": f"first {sample} Tucker and {sample} four-index products of the seed-0 draw, "
                             f"one core; extrapolated per class to {counts['tucker']} + {counts['c0']}",
-            "tucker_ms_mean": 1e3 * mean["tucker"], "configs0_form_ms_mean": 1e3 * mean["c0"],
+            "tucker_ms_mean": 1e3 * mean["tucker"], "four_index_form_ms_mean": 1e3 * mean["c0"],
             "serving_ladder64_request_s_extrapolated": slice_s * (o * o),
             "serving_ladder64_sample": "D-slice i = j = 0 (1/1024 of D) at full K = 4096, one core"}
 
 
 def batching_bench(lib, torch, steps, prec=tc.DEFAULT_PREC, full=True):
-    """one persistent group launch vs one TAPP_execute_product per
+    """The uneven 8192-product batch: one persistent group launch vs one TAPP_execute_product per
     product, same device operands, same GPU.  full=False (profiler runs):
     only the group's launches, no one-call-per-product loop and no autotuned
     group."""
@@ -412,7 +412,7 @@ def batching_bench(lib, torch, steps, prec=tc.DEFAULT_PREC, full=True):
             lib.destroy_info(h)
     kernel_s = float(np.mean(dev_us)) * 1e-6
     return {"workload": "8192 uneven products: 80% Tucker D[r,j,k]=sum_i A[i,j,k]U[r,i] (i,j,k~U{4..64}, "
-                        "r~U{2..16}), 20% configs[0] form (extents ~U{6..24} per index), seed 0",
+                        "r~U{2..16}), 20% four-index form D[a,b,c,d]=sum A[a,e,b,f]B[d,f,c,e] (extents ~U{6..24} per index), seed 0",
             "precision": "FP32" if prec == tc.DEFAULT_PREC else "BF16BF16_ACCUM_F32",
             "flop": flops, "bytes": bytes_, "group_gflops": flops / group_s / 1e9,
             "group_ms": group_s * 1e3, "group_kernel_ms": kernel_s * 1e3,
@@ -518,11 +518,11 @@ def batched_product_bench(lib, torch, steps, count=8192, shape=(32, 32, 32, 8)):
 
 
 def serving_bench(lib, torch, rates, nreq, split=False, seed=3, budget=64 << 30, sequential=False, small_only=False):
-    """open-loop Poisson arrivals into the serving loop.
+    """The serving workload: open-loop Poisson arrivals into the serving loop.
     Requests: 5 % large ladder-form products (virtual extent 64:
     D[a,b,i,j] = sum A[a,b,c,d] B[c,i,d,j], a..d = 64, i,j = 32), 20 %
-    configs[0] products (extents 16), 75 % drawn from 256 configs[3]-mix shapes.
-    With `small_only` the large class is left out (configs[0] and the mix
+    four-index products (extents 16), 75 % drawn from 256 shapes of the batch mix.
+    With `small_only` the large class is left out (the four-index and mix
     shapes keep their 20 : 75 proportion).
     70 % of requests pass a host A that is resident on the plans' handle
     (TAPP_AMD_make_resident: read from its HBM copy, no transfer); 30 % pass
@@ -674,8 +674,8 @@ def serving_bench(lib, torch, rates, nreq, split=False, seed=3, budget=64 << 30,
 
 
 def config_cases(lib, torch, steps):
-    """Per-call device time and rate of configs[0] (FP32, extents 16) and
-    configs[1] (C64 with a diagonal, unary labels, conj(B), a negative and
+    """Per-call device time and rate of the four-index product (FP32, extents 16) and
+    the complex-double feature product (C64 with a diagonal, unary labels, conj(B), a negative and
     padded strides, TAPP_F64F64_ACCUM_F64) on device operands."""
     L = lambda t: [ord(c) for c in t]  # noqa: E731
     out = {}
@@ -700,7 +700,7 @@ def config_cases(lib, torch, steps):
                 t = t.float()
             bufs.append(t)
             ptrs.append(t.data_ptr() + off * t.element_size())
-        ops = [0, 1 if name == "configs1" else 0, 0, 0]
+        ops = [0, 1 if name == "c64_feature" else 0, 0, 0]
         torch.cuda.synchronize()   # operands written on torch's stream before the library reads them
         _, plan = lib.create_plan(ops, infos, [L(x[2]) for x in specs], prec)
         npd = np.complex128 if ctype == tc.C64 else np.float32
@@ -733,7 +733,7 @@ def config_cases(lib, torch, steps):
 
     e = [16] * 4
     dense = [1, 16, 256, 4096]
-    run("configs0", [(e, dense, "aebf", tc.F32), (e, dense, "dfce", tc.F32), (e, dense, "abcd", tc.F32),
+    run("four_index16", [(e, dense, "aebf", tc.F32), (e, dense, "dfce", tc.F32), (e, dense, "abcd", tc.F32),
                      (e, dense, "abcd", tc.F32)], tc.DEFAULT_PREC, 1.5, 0.5, tc.F32,
         2.0 * 16 ** 6, 4.0 * 4 * 16 ** 4)
     a, b, c, h, i, u, v = 40, 40, 24, 32, 32, 8, 6
@@ -743,15 +743,15 @@ def config_cases(lib, torch, steps):
         sB.append(acc)
         acc = acc * x + 1
     sD = [1, a, a * b, a * b * h]
-    run("configs1", [([a, c, c, h, u], sA, "acchu", tc.C64), ([h, b, c, i, v], sB, "hbciv", tc.C64),
+    run("c64_feature", [([a, c, c, h, u], sA, "acchu", tc.C64), ([h, b, c, i, v], sB, "hbciv", tc.C64),
                      ([a, b, h, i], sD, "abhi", tc.C64), ([a, b, h, i], sD, "abhi", tc.C64)],
         tc.F64F64_ACCUM_F64, 0.7 - 0.3j, 1.1 + 0.2j, tc.C64,
         8.0 * h * a * (b * i) * c, 16.0 * (a * c * h * u + h * b * c * i * v + 2 * a * b * h * i))
     return out, (operands, specs_by_name)
 
 
-def cpu_reference_configs(cases):
-    """configs[0] and configs[1] run whole on one host core through the
+def cpu_reference_products(cases):
+    """The four-index and complex-double feature products run whole on one host core through the
     compiled reference (oracle/_ref), on the operands the GPU used; the GPU's
     D against the reference's."""
     path = os.path.join(ROOT, "oracle", "_ref", "libtapp-reference.so")
@@ -765,7 +765,7 @@ def cpu_reference_configs(cases):
         host = [t.numpy().copy() for t in operands[name]]
         gpu_d = host[3].copy()
         infos = [ref.create_info(dtype, ext, st)[1] for ext, st, lab, dtype in specs]
-        _, plan = ref.create_plan([0, 1 if name == "configs1" else 0, 0, 0], infos, [L(x[2]) for x in specs], prec)
+        _, plan = ref.create_plan([0, 1 if name == "c64_feature" else 0, 0, 0], infos, [L(x[2]) for x in specs], prec)
         npd = np.complex128 if ctype == tc.C64 else np.float32
         al, be = np.array([alpha], dtype=npd), np.array([beta], dtype=npd)
         offs = []
@@ -821,8 +821,8 @@ def main():
                     help="only time the uneven batch's group launches (for profiler runs)")
     ap.add_argument("--only-serve", action="store_true",
                     help="only run the serving sweeps (full mix with one call at a time, small requests)")
-    ap.add_argument("--tiles", type=int, choices=[0, 1, 2], default=0,
-                    help="with --only-form: MFMA tile size (TAPP_AMD_ATTR_TILES: 0 by shape, 1 large, 2 small)")
+    ap.add_argument("--tiles", type=int, choices=[0, 1, 2, 3], default=0,
+                    help="with --only-form: MFMA tile size (TAPP_AMD_ATTR_TILES: 0 by shape, 1 large, 2 small, 3 tall)")
     args = ap.parse_args()
     # autotuned measurements start from an empty cache on every run
     os.environ.setdefault("TAPP_AMD_TUNE_CACHE", os.path.join(tempfile.mkdtemp(prefix="tapp_bench_tune_"), "tune.txt"))
@@ -915,11 +915,11 @@ def main():
         log("bench: CPU reference baseline on a D-slice")
         cpu = cpu_baseline(A, B, D_fp32)
 
-    log("bench: configs[0] and configs[1] per call")
+    log("bench: four-index and complex-double feature products per call")
     others, config_operands = config_cases(lib, torch, max(5, args.steps))
     if cpu is not None:
-        log("bench: configs[0] and configs[1] on one host core through the reference")
-        cpu["configs"] = cpu_reference_configs(config_operands)
+        log("bench: four-index and complex-double feature products on one host core through the reference")
+        cpu["products"] = cpu_reference_products(config_operands)
         log("bench: batching and serving shape classes on one host core through the reference (sampled)")
         cpu["classes"] = cpu_reference_batch()
     del config_operands
@@ -974,7 +974,7 @@ def main():
                      "kernel": kern[1], "kernel_us": dev_us, "kernel_family": kern[0]},
         "cpu_baseline": cpu,
         "forms": forms,
-        "configs": others,
+        "products": others,
         "batched": batched,
         "serving": serving,
     }

@@ -108,7 +108,9 @@ enum {
      * 1 = large: FP32 split forms run 256 x 128 tiles (one workgroup per CU)
      * where both operands load along a contiguous direction; F64 and complex
      * run their large tiles; 2 = small: 128 x 128 split-form tiles, the
-     * small F64 / complex tiles.  Both sizes give the same bits. */
+     * small F64 / complex tiles; 3 = tall: F64 and complex run tiles of the
+     * small tiles' columns and twice their rows (FP32 as 0).  Every size
+     * gives the same bits. */
     TAPP_AMD_ATTR_TILES = 0x100B
 };
 

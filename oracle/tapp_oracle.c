@@ -4,8 +4,8 @@
  * A single-threaded C restatement of reference_implementation/src/product.c
  * of TAPPorg/reference-implementation.  Each function names the reference
  * lines it restates.  Loaded only by the test suite, the smoke check and the
- * CPU-baseline leg of the benchmark, as the checker; the GPU library neither
- * links nor calls it.
+ * CPU-baseline leg of the benchmark, to compare results against; the GPU
+ * library neither links nor calls it.
  *
  * What pins it (every function has at least one check in tests/):
  *   - orc_check, orc_classify: compared code-for-code and array-for-array

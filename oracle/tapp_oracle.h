@@ -2,9 +2,9 @@
  * Oracle for TAPP_execute_product: a plain-C, single-threaded restatement of
  * the reference's algorithm (reference_implementation/src/product.c).
  *
- * TEST INFRASTRUCTURE ONLY.  It is the checker for the GPU library: only the
- * test suite, the smoke check and the CPU-baseline leg of the benchmark load
- * it.  The product library never links, loads or calls it.
+ * TEST INFRASTRUCTURE ONLY.  The GPU library's results are compared with
+ * it: only the test suite, the smoke check and the CPU-baseline leg of the
+ * benchmark load it.  The product library never links, loads or calls it.
  *
  * Symbols use the prefix orc_ so the oracle can sit in one process beside
  * both the product library and the compiled reference, which all export

@@ -1001,8 +1001,9 @@ __global__ __launch_bounds__(256, (lp_waves<AL, BL, MODE>())) void k_gett_lp(tk_
 }
 
 // ---------------------------------------------------------------------------
-// k_small_f32: FP32 products with few outputs and a moderate K (configs[0]'s
-// 256 x 256 outputs at K = 256 gives 128 x 128 tiles only 4 workgroups).
+// k_small_f32: FP32 products with few outputs and a moderate K (the 256 x
+// 256 outputs of a 16^6 four-index product at K = 256 give 128 x 128 tiles
+// only 4 workgroups).
 // One workgroup per 16 x 16 output tile, so the grid has M N L / 256
 // workgroups; its 4 waves each take a quarter of K with
 // v_mfma_f32_16x16x4f32 (lane l supplies row l % 16 at k = l / 16 of a
@@ -1230,31 +1231,46 @@ extern "C" int tk_launch_gett(const tk_gett* gp, void* stream)
             return tk_launch_gett_wide(&g, st);
         }
         return launch_gett_f32(g, st);
-    case TK_F64:
+    // F64 and complex: large tiles while they give every CU 4 workgroups,
+    // else 4x smaller ones (more workgroups and waves in flight for short-K
+    // problems).  Tall tiles -- the small tile's columns, twice its rows --
+    // run when asked for (tile variant 3, an autotuner candidate): for a
+    // complex-double product with M = 40 and K = 24, one 64-row tile per
+    // column block instead of a 32-row and an 8-row one measured 64.6 us
+    // against 59.9 us, the halved workgroup count costing more than the
+    // shorter chains save.
+    case TK_F64: {
         if (kern == TK_KERNEL_SIMPLE) return launch_simple<double, false>(g, st);
         if (kern == TK_KERNEL_SPLITK) return launch_splitk<double, false>(g, st);
-        // large tiles while they give every CU 4 workgroups, else 4x smaller
-        // ones: more workgroups and waves in flight for short-K problems
-        if (g.tile_variant == 1 || (g.tile_variant == 0 && mfma_workgroups<double, false, 16, 4, 4, 8>(g) >= 4 * (int64_t)g.cus))
-            return launch_mfma<double, false, 16, 4, 4, 8>(g, st);
+        const bool large = g.tile_variant == 1 ||
+                           (g.tile_variant == 0 && mfma_workgroups<double, false, 16, 4, 4, 8>(g) >= 4 * (int64_t)g.cus);
+        if (large) return launch_mfma<double, false, 16, 4, 4, 8>(g, st);
+        if (g.tile_variant == 3) return launch_mfma<double, false, 16, 4, 2, 8>(g, st);
         return launch_mfma<double, false, 16, 2, 2, 8>(g, st);
-    case TK_C32:
+    }
+    case TK_C32: {
         if (kern == TK_KERNEL_SIMPLE || g.mode != TK_MODE_PLAIN) return launch_simple<float, true>(g, st);
         if (kern == TK_KERNEL_SPLITK) return launch_splitk<float, true>(g, st);
-        if (g.tile_variant == 1 || (g.tile_variant == 0 && mfma_workgroups<float, true, 32, 2, 2, 8>(g) >= 4 * (int64_t)g.cus))
-            return launch_mfma<float, true, 32, 2, 2, 8>(g, st);
+        const bool large = g.tile_variant == 1 ||
+                           (g.tile_variant == 0 && mfma_workgroups<float, true, 32, 2, 2, 8>(g) >= 4 * (int64_t)g.cus);
+        if (large) return launch_mfma<float, true, 32, 2, 2, 8>(g, st);
+        if (g.tile_variant == 3) return launch_mfma<float, true, 32, 2, 1, 8>(g, st);
         return launch_mfma<float, true, 32, 1, 1, 8>(g, st);
-    case TK_C64:
+    }
+    case TK_C64: {
         if (kern == TK_KERNEL_SIMPLE) return launch_simple<double, true>(g, st);
         if (kern == TK_KERNEL_SPLITK) return launch_splitk<double, true>(g, st);
-        if (g.tile_variant == 1 || (g.tile_variant == 0 && mfma_workgroups<double, true, 16, 2, 2, 8>(g) >= 4 * (int64_t)g.cus))
-            return launch_mfma<double, true, 16, 2, 2, 8>(g, st);
+        const bool large = g.tile_variant == 1 ||
+                           (g.tile_variant == 0 && mfma_workgroups<double, true, 16, 2, 2, 8>(g) >= 4 * (int64_t)g.cus);
+        if (large) return launch_mfma<double, true, 16, 2, 2, 8>(g, st);
+        if (g.tile_variant == 3) return launch_mfma<double, true, 16, 2, 1, 8>(g, st);
         return launch_mfma<double, true, 16, 1, 1, 8>(g, st);
+    }
     default:
         return (int)hipErrorInvalidValue;
     }
 }
 
-extern "C" int tk_kernel_count(void) { return 13; }
+extern "C" int tk_kernel_count(void) { return 16; }
 
 extern "C" const char* tk_last_gett_name(void) { return last_name; }

@@ -30,7 +30,7 @@ TAPP_error TAPP_attr_set(TAPP_attr attr, TAPP_key key, void* value)
     case TAPP_AMD_ATTR_TIMING: e->timing = *(int*)value != 0; return 0;
     case TAPP_AMD_ATTR_TILES: {
         int t = *(int*)value;
-        if (t < 0 || t > 2) return TAPP_AMD_ERR_INVALID;
+        if (t < 0 || t > 3) return TAPP_AMD_ERR_INVALID;
         e->tiles = t;
         return 0;
     }

@@ -362,6 +362,7 @@ static int tune_gett(struct tapp_exec* ex, tk_gett* g, int64_t ks, int64_t kchun
     } else {
         cand[nc++] = TK_KERNEL_MFMA | (1 << 4);
         cand[nc++] = TK_KERNEL_MFMA | (2 << 4);
+        cand[nc++] = TK_KERNEL_MFMA | (3 << 4);
     }
     if (outs <= 65536 && g->K >= 64) cand[nc++] = TK_KERNEL_SPLITK;
     tk_gett t = *g;
@@ -636,7 +637,7 @@ static int plan_work(const struct tapp_plan* p, const struct tapp_exec* ex, cons
         if (kern != TK_KERNEL_MFMA || ct != TK_F32) mode = TK_MODE_PLAIN;
         else if (!sync || !tune_safe(&P, w->src, w->need[2])) mode = TK_MODE_SPLIT6_BF16;
     }
-    /* few outputs and a moderate K (configs[0]: 4 tiles of 128 x 128): 16 x
+    /* few outputs and a moderate K (a 16^6 four-index product: 4 tiles of 128 x 128): 16 x
      * 16 tiles with K split over each workgroup's waves, in one launch,
      * instead of split-K's partial sums and second kernel */
     if (kern == TK_KERNEL_MFMA && ct == TK_F32 && mode == TK_MODE_PLAIN && ex->force_kernel == TK_KERNEL_AUTO) {

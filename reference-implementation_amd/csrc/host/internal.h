@@ -71,7 +71,7 @@ struct tapp_exec {
     /* attributes */
     int split_bf16;
     int force_kernel;
-    int tiles;                 /* TAPP_AMD_ATTR_TILES: 0 by shape, 1 large, 2 small */
+    int tiles;                 /* TAPP_AMD_ATTR_TILES: 0 by shape, 1 large, 2 small, 3 tall */
     int autotune;              /* 0 rules, 1 measure unknown shape classes, 2 cache only */
     int last_reused;           /* the last batched call reused its plan's prepared batch */
     int timing;

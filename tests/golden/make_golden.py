@@ -9,11 +9,11 @@ has the reference sources at /root/reference and oracle/_ref built).
    to the compiled reference library.  The exercise's documented expected
    output (exercise_contraction_answers.c:229-236) is checked here against
    the captured D before the file is written.
-2. ref_configs0.npz: the configs[0] contraction (D[a,b,c,d] = 1.5 sum_{e,f}
+2. ref_four_index16.npz: the four-index contraction (D[a,b,c,d] = 1.5 sum_{e,f}
    A[a,e,b,f] B[d,f,c,e] + 0.5 C, extents 16, FP32, seed 1) computed by the
    compiled reference, full D.
-3. ref_configs1_sample.npz: the configs[1] contraction (complex double,
-   TAPP_F64F64_ACCUM_F64: D[a,b,h,i] = alpha sum_{c,u,v} A[a,c,c,h,u]
+3. ref_c64_feature_sample.npz: the complex-double feature contraction
+   (TAPP_F64F64_ACCUM_F64: D[a,b,h,i] = alpha sum_{c,u,v} A[a,c,c,h,u]
    conj(B[h,b,c,i,v]) + beta C, seed 1) computed by the compiled reference;
    its D storage (26 MB) is sampled at 4096 positions drawn with seed 11,
    which are stored with their values.
@@ -79,23 +79,23 @@ def check_exercise(cases):
     ex["documented_expected_rows"] = documented.tolist()
 
 
-def configs0(tmp):
+def four_index16(tmp):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import tapp_cases as tc  # noqa: E402
-    case = tc.configs0_case(seed=1)
+    case = tc.four_index16_case(seed=1)
     out = tc.run_reference(REFLIB, case)
-    np.savez_compressed(os.path.join(HERE, "ref_configs0.npz"), D=out, seed=np.int64(1))
-    print("configs0: D", out.shape, float(np.abs(out).max()))
+    np.savez_compressed(os.path.join(HERE, "ref_four_index16.npz"), D=out, seed=np.int64(1))
+    print("four_index16: D", out.shape, float(np.abs(out).max()))
 
 
-def configs1_sample():
+def c64_feature_sample():
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import tapp_cases as tc  # noqa: E402
-    case = tc.configs1_case(seed=1)
+    case = tc.c64_feature_case(seed=1)
     out = tc.run_reference(REFLIB, case)
     idx = np.sort(np.random.default_rng(11).choice(out.size, 4096, replace=False)).astype(np.int64)
-    np.savez_compressed(os.path.join(HERE, "ref_configs1_sample.npz"), idx=idx, D=out[idx], seed=np.int64(1))
-    print("configs1: D sample", idx.size, "of", out.size, float(np.abs(out[idx]).max()))
+    np.savez_compressed(os.path.join(HERE, "ref_c64_feature_sample.npz"), idx=idx, D=out[idx], seed=np.int64(1))
+    print("c64_feature: D sample", idx.size, "of", out.size, float(np.abs(out[idx]).max()))
 
 
 def main():
@@ -105,12 +105,12 @@ def main():
         with open(os.path.join(HERE, "kat_reference_programs.json"), "w") as f:
             json.dump({"source": "reference programs run against the compiled reference library",
                        "cases": cases}, f, indent=None, separators=(",", ":"))
-        configs0(tmp)
-    configs1_sample()
+        four_index16(tmp)
+    c64_feature_sample()
 
 
 if __name__ == "__main__":
-    if sys.argv[1:] == ["configs1"]:
-        configs1_sample()
+    if sys.argv[1:] == ["c64_feature"]:
+        c64_feature_sample()
     else:
         main()

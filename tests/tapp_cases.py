@@ -347,7 +347,7 @@ def normwise_error(got, ref, mask=None):
 # --------------------------------------------------------------------------
 # the benchmark configurations as cases
 # --------------------------------------------------------------------------
-def configs0_case(seed=1, dtype=tc.F32):
+def four_index16_case(seed=1, dtype=tc.F32):
     """D[a,b,c,d] = 1.5 sum_{e,f} A[a,e,b,f] B[d,f,c,e] + 0.5 C[a,b,c,d],
     all extents 16, dense column-major, values uniform [-1, 1)."""
     e = [16] * 4
@@ -355,10 +355,10 @@ def configs0_case(seed=1, dtype=tc.F32):
     B = make_operand(dtype, e, "dfce", seed * 4 + 1)
     C = make_operand(dtype, e, "abcd", seed * 4 + 2)
     D = make_operand(dtype, e, "abcd", seed * 4 + 3)
-    return Case(A, B, C, D, tc.DEFAULT_PREC, 1.5, 0.5, name="configs0")
+    return Case(A, B, C, D, tc.DEFAULT_PREC, 1.5, 0.5, name="four_index16")
 
 
-def configs1_case(seed=1, scale=1):
+def c64_feature_case(seed=1, scale=1):
     """Complex double feature coverage:
     D[a,b,h,i] = alpha sum_{c,u,v} A[a,c,c,h,u] conj(B[h,b,c,i,v]) + beta C[a,b,h,i]
     h=32 Hadamard, c=24 contracted and repeated in A, a=40, b=40, i=32 free,
@@ -369,7 +369,7 @@ def configs1_case(seed=1, scale=1):
     B = make_operand(tc.C64, [h, b, c, i, v], "hbciv", seed * 4 + 1, pad=1, op=tc.CONJUGATE)
     C = make_operand(tc.C64, [a, b, h, i], "abhi", seed * 4 + 2)
     D = make_operand(tc.C64, [a, b, h, i], "abhi", seed * 4 + 3)
-    return Case(A, B, C, D, tc.F64F64_ACCUM_F64, 0.7 - 0.3j, 1.1 + 0.2j, name="configs1")
+    return Case(A, B, C, D, tc.F64F64_ACCUM_F64, 0.7 - 0.3j, 1.1 + 0.2j, name="c64_feature")
 
 
 def ladder_case(seed=1, v=24, o=8, prec=tc.DEFAULT_PREC, beta=0.0):
@@ -541,7 +541,7 @@ def tucker_case(seed, i, j, k, r, dtype=tc.F32, prec=tc.DEFAULT_PREC):
 
 
 def four_index_case(seed, ext, dtype=tc.F32, prec=tc.DEFAULT_PREC):
-    """configs[0] form D[a,b,c,d] = sum_{e,f} A[a,e,b,f] B[d,f,c,e] with
+    """Four-index form D[a,b,c,d] = sum_{e,f} A[a,e,b,f] B[d,f,c,e] with
     per-label extents ext = (a, b, c, d, e, f)."""
     a, b, c, d, e, f = ext
     A = make_operand(dtype, [a, e, b, f], "aebf", seed * 4 + 0)
@@ -554,7 +554,7 @@ def four_index_case(seed, ext, dtype=tc.F32, prec=tc.DEFAULT_PREC):
 def batching_mix(seed, count, tucker_range=(4, 64), r_range=(2, 16), c0_range=(6, 24), tucker_frac=0.8,
                  prec=tc.DEFAULT_PREC, per_index=True):
     """The uneven batching workload: `tucker_frac` Tucker mode products with
-    i, j, k ~ U{tucker_range} and r ~ U{r_range}, the rest configs[0]-form
+    i, j, k ~ U{tucker_range} and r ~ U{r_range}, the rest four-index
     products with each extent ~ U{c0_range} (drawn per index)."""
     rng = np.random.default_rng(seed)
     cases = []

@@ -121,10 +121,10 @@ def test_validation_and_groups_on_random_cases(lib, oracle, dtype, scenario):
             lib.destroy_info(h)
 
 
-def test_groups_of_benchmark_configs(lib):
+def test_groups_of_benchmark_products(lib):
     cases = {
-        "c0": (T.configs0_case(1), [1, 256, 256, 256, 1, 1], 0),
-        "c1": (T.configs1_case(1), [32, 40, 40 * 32, 24, 8, 6], 3),
+        "c0": (T.four_index16_case(1), [1, 256, 256, 256, 1, 1], 0),
+        "c1": (T.c64_feature_case(1), [32, 40, 40 * 32, 24, 8, 6], 3),
     }
     for name, (c, want, ct_want) in cases.items():
         plan, infos = _plan(lib, c)

@@ -122,21 +122,21 @@ def test_reference_program_known_answers(lib, gpu_exec):
         assert np.allclose(g.astype(np.float64), want, rtol=1e-6, atol=1e-5), (k["program"], k["call"])
 
 
-def test_configs0_against_reference_golden(lib, gpu_exec):
+def test_four_index16_against_reference_golden(lib, gpu_exec):
     import os
-    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "ref_configs0.npz"))
-    c = T.configs0_case(seed=int(g["seed"]))
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "ref_four_index16.npz"))
+    c = T.four_index16_case(seed=int(g["seed"]))
     rc, got = T.run_lib(lib, c, gpu_exec)
     assert rc == 0
     assert_parity(got, g["D"], c)
 
 
-def test_configs1_against_reference_golden_sample(lib, gpu_exec):
-    """configs[1] on the GPU against the compiled reference's D at 4096
+def test_c64_feature_against_reference_golden_sample(lib, gpu_exec):
+    """The complex-double feature product on the GPU against the compiled reference's D at 4096
     sampled positions (FP64 tolerance, normwise over the sample)."""
     import os
-    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "ref_configs1_sample.npz"))
-    c = T.configs1_case(seed=int(g["seed"]))
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "ref_c64_feature_sample.npz"))
+    c = T.c64_feature_case(seed=int(g["seed"]))
     rc, got = T.run_lib(lib, c, gpu_exec)
     assert rc == 0
     want = g["D"]
@@ -146,10 +146,10 @@ def test_configs1_against_reference_golden_sample(lib, gpu_exec):
 
 def test_small_tile_kernel_by_rule_and_batches(lib, gpu_exec, oracle):
     """Few outputs and a moderate K take the 16 x 16 small-tile kernel by
-    rule (configs[0] itself; a Hadamard batch; ragged edges; K not a
+    rule (the 16^6 four-index product itself; a Hadamard batch; ragged edges; K not a
     multiple of 4; beta != 0 with a transposed A), each against the oracle,
     and the same product run twice gives the same bits."""
-    cases = [T.configs0_case(seed=3),
+    cases = [T.four_index16_case(seed=3),
              T.four_index_case(5, (9, 7, 13, 6, 11, 5)),
              _gemm_case(tc.F32, 100, 90, 301, 21, transA=True),
              _gemm_case(tc.F32, 17, 250, 1000, 22, transB=True, beta=0.0)]
@@ -167,8 +167,8 @@ def test_small_tile_kernel_by_rule_and_batches(lib, gpu_exec, oracle):
         assert rc == 0 and np.array_equal(again.view(np.uint32), got.view(np.uint32)), c.name
 
 
-def test_configs1_full_size(lib, gpu_exec, oracle):
-    c = T.configs1_case(seed=1)
+def test_c64_feature_full_size(lib, gpu_exec, oracle):
+    c = T.c64_feature_case(seed=1)
     rc, got, want = run_both(lib, gpu_exec, oracle, c)
     assert rc == 0
     assert_parity(got, want, c)
@@ -389,6 +389,37 @@ def test_large_tile_variants(lib, gpu_exec, oracle, dtype, M):
     assert rc == 0
     assert lib.attr_get_int(gpu_exec, tc.ATTR_LAST_KERNEL)[1] == tc.KERNEL_MFMA
     assert_parity(got, want, c)
+
+
+@pytest.mark.parametrize("dtype", [tc.F64, tc.C32, tc.C64])
+def test_tile_sizes_give_the_same_bits(lib, oracle, dtype):
+    """F64 and complex products on each MFMA tile size (TAPP_AMD_ATTR_TILES
+    1 large, 2 small, 3 tall: the small tile's columns and twice its rows),
+    with ragged M, N and K, a Hadamard batch and conj(B): every size gives
+    the oracle's result within the compute type's bound and the same bits as
+    the others."""
+    M, N, K, L = 75, 41, 19, 3
+    A = T.make_operand(dtype, [M, K, L], "mkl", 91)
+    B = T.make_operand(dtype, [K, N, L], "knl", 92, op=tc.CONJUGATE if dtype in T.COMPLEX_TYPES else 0)
+    C = T.make_operand(dtype, [M, N, L], "mnl", 93)
+    D = T.make_operand(dtype, [M, N, L], "mnl", 94, pad=1)
+    alpha, beta = (0.5 - 0.25j, 0.75 + 0.5j) if dtype in T.COMPLEX_TYPES else (0.5, 0.75)
+    c = T.Case(A, B, C, D, tc.DEFAULT_PREC, alpha, beta, name=f"tile_sizes_{dtype}")
+    _, want = oracle.run(c, gpu=True)
+    outs, names = [], []
+    for v in (1, 2, 3):
+        _, ex = lib.create_executor()
+        assert lib.attr_set_int(ex, tc.ATTR_KERNEL, tc.KERNEL_MFMA) == 0
+        assert lib.attr_set_int(ex, tc.ATTR_TILES, v) == 0
+        rc, got = T.run_lib(lib, c, ex)
+        assert rc == 0
+        names.append(lib.last_kernel_name(ex)[1])
+        lib.destroy_executor(ex)
+        assert_parity(got, want, c)
+        outs.append(got)
+    assert len(set(names)) == 3, names
+    for got in outs[1:]:
+        assert np.array_equal(got.view(np.uint8), outs[0].view(np.uint8))
 
 
 @pytest.mark.parametrize("mode", ["plain", "bf16", "f16", "split_bf16", "split6_bf16", "split_f16"])

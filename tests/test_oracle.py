@@ -32,19 +32,19 @@ def test_oracle_bitexact_vs_reference(oracle, ref, dtype, prec, scenario):
             assert same_bits(d_r, d_o), (c.name, np.abs(d_r - d_o).max())
 
 
-def test_oracle_configs0_golden(oracle):
-    g = np.load(os.path.join(GOLDEN, "ref_configs0.npz"))
-    rc, d = oracle.run(T.configs0_case(seed=int(g["seed"])))
+def test_oracle_four_index16_golden(oracle):
+    g = np.load(os.path.join(GOLDEN, "ref_four_index16.npz"))
+    rc, d = oracle.run(T.four_index16_case(seed=int(g["seed"])))
     assert rc == 0
     assert same_bits(d, g["D"])
 
 
-def test_oracle_configs1_golden_sample(oracle):
-    """configs[1] (complex double, a diagonal, unary labels on both sides,
+def test_oracle_c64_feature_golden_sample(oracle):
+    """The complex-double feature product (a diagonal, unary labels on both sides,
     conj(B), negative and padded strides): the oracle's D equals the
     compiled reference's, bit for bit, at 4096 sampled positions."""
-    g = np.load(os.path.join(GOLDEN, "ref_configs1_sample.npz"))
-    rc, d = oracle.run(T.configs1_case(seed=int(g["seed"])))
+    g = np.load(os.path.join(GOLDEN, "ref_c64_feature_sample.npz"))
+    rc, d = oracle.run(T.c64_feature_case(seed=int(g["seed"])))
     assert rc == 0
     assert same_bits(d[g["idx"]], g["D"])
 

@@ -66,7 +66,7 @@ def test_budget_rejection_and_cancel_on_paused_server(lib):
 def test_results_same_alone_and_in_the_mix(lib, gpu_exec, oracle):
     from test_group import DevCase, mixed_cases
     from test_gpu_parity import assert_parity
-    cases = mixed_cases(31) + [T.configs0_case(2)]
+    cases = mixed_cases(31) + [T.four_index16_case(2)]
     dcs = [DevCase(lib, c) for c in cases]
     alone = []
     for d in dcs:

@@ -150,9 +150,9 @@ def test_split_f16_nan_falls_back_to_fp32(lib, torch_mod):
 def test_tiles_attribute_values(lib):
     rc, ex = lib.create_executor()
     assert rc == 0
-    for v in (0, 1, 2):
+    for v in (0, 1, 2, 3):
         assert lib.attr_set_int(ex, ATTR_TILES, v) == 0
         assert lib.attr_get_int(ex, ATTR_TILES) == (0, v)
-    assert lib.attr_set_int(ex, ATTR_TILES, 3) != 0
-    assert lib.attr_get_int(ex, ATTR_TILES) == (0, 2)
+    assert lib.attr_set_int(ex, ATTR_TILES, 4) != 0
+    assert lib.attr_get_int(ex, ATTR_TILES) == (0, 3)
     lib.destroy_executor(ex)

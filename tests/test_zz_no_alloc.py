@@ -59,12 +59,12 @@ def _gemm(M, N, K, seed, beta=0.5):
 
 def _products():
     """Largest scratch use first: split-K partials of a few-tile long-K
-    product; configs[1]'s form at half size (pre-passes of both operands,
+    product; the complex-double feature product at half size (pre-passes of both operands,
     C64); a layout mix; the unary/Hadamard/diagonal scenario in C32."""
     return [
         ("splitk", _gemm(256, 200, 6000, 11), 0),
         ("split_f16", _gemm(136, 72, 2048, 13, beta=0.0), 3),
-        ("configs1_half", T.configs1_case(seed=2, scale=2), 0),
+        ("c64_feature_half", T.c64_feature_case(seed=2, scale=2), 0),
         ("layout_mix", T.layout_mix_case(4242, tc.F32), 0),
         ("everything_c32", T.random_case(77, "everything", tc.C32, max_ext=5, beta_zero=False), 0),
         ("small", T.random_case(78, "contraction", tc.F32, max_ext=4), 0),
