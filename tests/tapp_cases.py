@@ -548,7 +548,7 @@ def four_index_case(seed, ext, dtype=tc.F32, prec=tc.DEFAULT_PREC):
     B = make_operand(dtype, [d, f, c, e], "dfce", seed * 4 + 1)
     C = make_operand(dtype, [a, b, c, d], "abcd", seed * 4 + 2)
     D = make_operand(dtype, [a, b, c, d], "abcd", seed * 4 + 3)
-    return Case(A, B, C, D, prec, 1.5, 0.5, name=f"c0form{ext}")
+    return Case(A, B, C, D, prec, 1.5, 0.5, name=f"four_index{ext}")
 
 
 def batching_mix(seed, count, tucker_range=(4, 64), r_range=(2, 16), c0_range=(6, 24), tucker_frac=0.8,

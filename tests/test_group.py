@@ -238,7 +238,7 @@ class BatchSets:
             d.close()
 
 
-@pytest.mark.parametrize("shape", ["c0form", "tucker"])
+@pytest.mark.parametrize("shape", ["four_index", "tucker"])
 def test_batched_product_reuses_prepared_batch(lib, oracle, shape):
     """Repeated TAPP_execute_batched_product calls on one plan: the second
     call with the same batch count replays the prepared batch (descriptors
@@ -247,7 +247,7 @@ def test_batched_product_reuses_prepared_batch(lib, oracle, shape):
     count build the batch again."""
     import torch
     _, ex = lib.create_executor()
-    base = T.four_index_case(5, (6, 5, 7, 4, 3, 5)) if shape == "c0form" else T.tucker_case(6, 24, 9, 7, 5)
+    base = T.four_index_case(5, (6, 5, 7, 4, 3, 5)) if shape == "four_index" else T.tucker_case(6, 24, 9, 7, 5)
     bs = BatchSets(lib, base, 6, 300)
     rc, reused, want = bs.call(ex, 1.5, 0.5)
     assert (rc, reused) == (0, 0)
@@ -328,6 +328,31 @@ def test_batched_product_reuse_follows_layout_change(lib, oracle):
     bs.check(oracle, want)
     bs.close()
     lib.destroy_executor(ex)
+
+
+def test_group_full_distribution_sample(lib, gpu_exec, oracle):
+    """1024 products drawn from the uneven batch's full distribution (80 %
+    Tucker i, j, k ~ U{4..64}, r ~ U{2..16}; 20 % four-index products with
+    each extent ~ U{6..24}) in one product group, both persistent launches:
+    every Tucker product and 16 four-index products (drawn with a seed
+    among those of at most 3e7 MACs) equal the oracle's."""
+    cases = T.batching_mix(2, 1024)
+    dcs = [DevCase(lib, c) for c in cases]
+    rc, g = make_group(lib, gpu_exec, dcs)
+    assert rc == 0
+    assert lib.group_info(g)[1][0] == len(dcs)
+    assert lib.execute_group(g) == 0
+    tucker = [i for i, c in enumerate(cases) if c.name.startswith("tucker")]
+    four = [i for i, c in enumerate(cases) if c.name.startswith("four_index")
+            and np.prod([float(e) for e in c.A.extents]) * c.D.extents[2] * c.D.extents[3] <= 3e7]
+    assert len(tucker) > 700 and len(four) >= 16
+    pick = sorted(np.random.default_rng(5).choice(four, 16, replace=False).tolist())
+    for i in tucker + pick:
+        _, want = oracle.run(cases[i], gpu=True)
+        assert_parity(dcs[i].d(), want, cases[i])
+    assert lib.destroy_group(g) == 0
+    for d in dcs:
+        d.close()
 
 
 def test_group_thin_tiles_affine_and_table_layouts(lib, gpu_exec, oracle):

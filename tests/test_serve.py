@@ -66,7 +66,9 @@ def test_budget_rejection_and_cancel_on_paused_server(lib):
 def test_results_same_alone_and_in_the_mix(lib, gpu_exec, oracle):
     from test_group import DevCase, mixed_cases
     from test_gpu_parity import assert_parity
-    cases = mixed_cases(31) + [T.four_index16_case(2)]
+    # plus the ladder form D[a,b,i,j] = sum A[a,b,c,d] B[c,i,d,j] of the
+    # serving workload's large class, at a..d = 24, i, j = 8
+    cases = mixed_cases(31) + [T.four_index16_case(2), T.ladder_case(seed=3, v=24, o=8)]
     dcs = [DevCase(lib, c) for c in cases]
     alone = []
     for d in dcs:
