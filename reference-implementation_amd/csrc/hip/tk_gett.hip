@@ -1029,9 +1029,20 @@ __global__ __launch_bounds__(256) void k_small_f32(tk_gett g)
     const int64_t steps = (g.K + 3) / 4, sq = (steps + 3) / 4;
     const int64_t s_begin = wave * sq, s_end = s_begin + sq < steps ? s_begin + sq : steps;
 
+    // the output this thread writes after the reduction
+    const int t = threadIdx.x;
+    const int64_t mo = m0 + (t & 15), no = n0 + (t >> 4);
+    const bool o_in = mo < g.M && no < g.N;
+    const int64_t mi = o_in ? mo : 0, ni = o_in ? no : 0;   // valid table indices for every thread
+
     for (int64_t l = blockIdx.y; l < g.L; l += gridDim.y) {
         const float* Ab = (const float*)g.A + g.lA[l] + ma;
         const float* Bb = (const float*)g.B + g.lB[l] + nb;
+        // the output's C value and D offset, read now: their table and C
+        // loads are in flight with the K loop's, so the epilogue waits on none
+        const int64_t coff = g.beta_zero ? 0 : g.lC[l] + g.mC[mi] + g.nC[ni];
+        const int64_t doff = g.lD[l] + g.mD[mi] + g.nD[ni];
+        const Val<float, false> cv = epilogue_c<float, false>(g, coff);
         f32x4 acc = f32x4{};
         for (int64_t s0 = s_begin; s0 < s_end; s0 += CH) {
             int64_t ka[CH], kb[CH];
@@ -1062,10 +1073,8 @@ __global__ __launch_bounds__(256) void k_small_f32(tk_gett g)
 #pragma unroll
         for (int r = 0; r < 4; ++r) part[wave][(4 * kq + r) * 16 + (lane & 15)] = acc[r];
         __syncthreads();
-        const int t = threadIdx.x;
         const float sum = ((part[0][t] + part[1][t]) + part[2][t]) + part[3][t];
-        const int64_t mo = m0 + (t & 15), no = n0 + (t >> 4);
-        if (mo < g.M && no < g.N) epilogue<float, false>(g, l, mo, no, sum, 0.0f);
+        if (o_in) epilogue_d<float, false>(g, doff, sum, 0.0f, cv);
     }
 }
 
